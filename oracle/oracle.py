@@ -1,0 +1,196 @@
+"""ORACLE — test infrastructure only.
+
+ctypes binding of oracle/_build/liboracle.so, the FP64 CPU restatement of the
+AIFMM_LS 2D Lippmann-Schwinger path.  Imported only by tests/, by
+__graft_entry__.smoke() and by bench.py's cpu_baseline leg, always as the
+checker, never as the measured or shipped path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+SCHED_EFFICIENT, SCHED_EAGER, SCHED_COLORED = 0, 1, 2
+
+
+def build():
+    import subprocess
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        P = ctypes.c_void_p
+        dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+        ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+        lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+        L.orc_create.restype = P
+        L.orc_create.argtypes = [ip, ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_int]
+        L.orc_destroy.argtypes = [P]
+        L.orc_n.restype = ctypes.c_longlong
+        L.orc_n.argtypes = [P]
+        L.orc_points.argtypes = [P, dp, dp]
+        L.orc_tables.argtypes = [P, dp, dp, dp]
+        L.orc_entries.argtypes = [P, ctypes.c_longlong, lp, lp, dp]
+        L.orc_bessel.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.orc_gauss_legendre.argtypes = [ctypes.c_int, dp, dp]
+        L.orc_rhs.argtypes = [P, ctypes.c_double, dp]
+        L.orc_solve.restype = ctypes.c_int
+        L.orc_solve.argtypes = [P, dp, dp, ctypes.c_int, dp]
+        L.orc_ranks.argtypes = [P, ip]
+        L.orc_skeleton.restype = ctypes.c_int
+        L.orc_skeleton.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ip]
+        L.orc_nca.argtypes = [P]
+        L.orc_aca.restype = ctypes.c_int
+        L.orc_aca.argtypes = [P, ctypes.c_int, ctypes.c_int, ip, ctypes.c_int, ip, ip, ip, dp, dp]
+        L.orc_dense.argtypes = [P, dp]
+        L.orc_cpqr.restype = ctypes.c_int
+        L.orc_cpqr.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, ctypes.c_int, dp, dp, ip]
+        L.orc_qr_solve.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp, dp]
+        L.orc_lu_solve.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp, dp]
+        _lib = L
+    return _lib
+
+
+def _c(z):
+    return np.ascontiguousarray(np.asarray(z, dtype=np.complex128)).view(np.float64)
+
+
+class Problem:
+    """One discretised LS problem (tables, points, contrast) in the oracle."""
+
+    def __init__(self, n_levels, kappa, tol_pow=8, qchoice=0, n_cheb=6, degree=6, L=1.0,
+                 bumps=None, quantize=True, exact_bessel=False):
+        ip = np.array([n_cheb, degree, n_levels, tol_pow, qchoice, int(quantize), int(exact_bessel)], np.int32)
+        nb = 0
+        self._bumps = None
+        if bumps is not None:
+            self._bumps = np.ascontiguousarray(np.asarray(bumps, np.float64).reshape(-1, 4))
+            nb = self._bumps.shape[0]
+        ptr = self._bumps.ctypes.data if self._bumps is not None else None
+        self.h = lib().orc_create(ip, float(kappa), float(L), ptr, nb)
+        self.n_levels, self.kappa, self.tol_pow, self.n_cheb, self.degree = n_levels, kappa, tol_pow, n_cheb, degree
+        self.rank = n_cheb * n_cheb
+        self.nbasis = degree * (degree + 1) // 2
+        self.N = int(lib().orc_n(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def points(self):
+        xy = np.zeros(2 * self.N)
+        q = np.zeros(self.N)
+        lib().orc_points(self.h, xy, q)
+        return xy.reshape(-1, 2), q
+
+    def tables(self):
+        M = np.zeros(121 * self.rank * 2)
+        T = np.zeros(9 * self.rank * self.rank * 2)
+        Qp = np.zeros(self.nbasis * self.rank * 2)
+        lib().orc_tables(self.h, M, T, Qp)
+        M = M.view(np.complex128).reshape(self.rank, 121).T
+        T = T.view(np.complex128).reshape(9, self.rank, self.rank).transpose(0, 2, 1)
+        Qp = Qp.view(np.complex128).reshape(self.rank, self.nbasis).T
+        return M, T, Qp
+
+    def entries(self, I, J):
+        I = np.ascontiguousarray(I, np.int64)
+        J = np.ascontiguousarray(J, np.int64)
+        out = np.zeros(2 * I.size)
+        lib().orc_entries(self.h, I.size, I, J, out)
+        return out.view(np.complex128)
+
+    def rhs(self, theta=0.0):
+        b = np.zeros(2 * self.N)
+        lib().orc_rhs(self.h, float(theta), b)
+        return b.view(np.complex128)
+
+    def solve(self, b, schedule=SCHED_EFFICIENT):
+        x = np.zeros(2 * self.N)
+        st = np.zeros(8)
+        lib().orc_solve(self.h, _c(b), x, int(schedule), st)
+        return x.view(np.complex128), st
+
+    def dense(self):
+        A = np.zeros(2 * self.N * self.N)
+        lib().orc_dense(self.h, A)
+        return A.view(np.complex128).reshape(self.N, self.N).T
+
+    def nca(self):
+        lib().orc_nca(self.h)
+
+    def ranks(self):
+        nl = 4 ** self.n_levels
+        out = np.zeros((self.n_levels + 1) * nl, np.int32)
+        lib().orc_ranks(self.h, out)
+        return {j: out[j * nl: j * nl + 4 ** j].copy() for j in range(2, self.n_levels + 1)}
+
+    def skeleton(self, j, k, which):
+        out = np.zeros(4 * self.N, np.int32)
+        n = lib().orc_skeleton(self.h, j, k, which, out)
+        return out[:n].copy()
+
+    def aca(self, rows, cols, tol_pow):
+        rows = np.ascontiguousarray(rows, np.int32)
+        cols = np.ascontiguousarray(cols, np.int32)
+        mn = min(rows.size, cols.size)
+        rb = np.zeros(mn, np.int32)
+        cb = np.zeros(mn, np.int32)
+        Ac = np.zeros(2 * rows.size * mn)
+        Ar = np.zeros(2 * cols.size * mn)
+        r = lib().orc_aca(self.h, tol_pow, rows.size, rows, cols.size, cols, rb, cb, Ac, Ar)
+        Ac = Ac[: 2 * rows.size * r].view(np.complex128).reshape(r, rows.size).T
+        Ar = Ar[: 2 * cols.size * r].view(np.complex128).reshape(cols.size, r).T
+        return rb[:r].copy(), cb[:r].copy(), Ac, Ar
+
+
+def bessel(x, nmax=60, exact=False):
+    J = np.zeros(nmax + 1)
+    Y = np.zeros(nmax + 1)
+    lib().orc_bessel(float(x), nmax, int(exact), J, Y)
+    return J, Y
+
+
+def gauss_legendre(n):
+    x = np.zeros(n)
+    w = np.zeros(n)
+    lib().orc_gauss_legendre(n, x, w)
+    return x, w
+
+
+def cpqr(A, thr, k):
+    A = np.asarray(A, np.complex128)
+    m, n = A.shape
+    Q = np.zeros(2 * m * k)
+    rd = np.zeros(min(m, n))
+    perm = np.zeros(n, np.int32)
+    r = lib().orc_cpqr(m, n, _c(A.T.copy()), float(thr), k, Q, rd, perm)
+    return r, Q.view(np.complex128).reshape(k, m).T, rd, perm
+
+
+def qr_solve(A, B):
+    A = np.asarray(A, np.complex128)
+    B = np.asarray(B, np.complex128).reshape(A.shape[0], -1)
+    n, nr = A.shape[0], B.shape[1]
+    X = np.zeros(2 * n * nr)
+    lib().orc_qr_solve(n, _c(A.T.copy()), nr, _c(B.T.copy()), X)
+    return X.view(np.complex128).reshape(nr, n).T
+
+
+def lu_solve(A, B):
+    A = np.asarray(A, np.complex128)
+    B = np.asarray(B, np.complex128).reshape(A.shape[0], -1)
+    n, nr = A.shape[0], B.shape[1]
+    X = np.zeros(2 * n * nr)
+    lib().orc_lu_solve(n, _c(A.T.copy()), nr, _c(B.T.copy()), X)
+    return X.view(np.complex128).reshape(nr, n).T

@@ -1,0 +1,152 @@
+// Host-side helpers that collect descriptors for the batched kernels,
+// upload them and launch.
+#pragma once
+#include <vector>
+
+#include "arena.h"
+#include "kernels.h"
+
+namespace aifmm {
+
+template <class T>
+struct Blk {
+  T* p = nullptr;
+  int r = 0, c = 0, ld = 0;
+  size_t size() const { return (size_t)r * c; }
+  bool empty() const { return r == 0 || c == 0; }
+  Blk sub(int i0, int j0, int nr, int nc) const {
+    Blk b;
+    b.p = p ? p + i0 + (size_t)ld * j0 : nullptr;
+    b.r = nr;
+    b.c = nc;
+    b.ld = ld;
+    return b;
+  }
+};
+
+template <class T>
+Blk<T> new_blk(Arena& a, int r, int c) {
+  Blk<T> b;
+  b.r = r;
+  b.c = c;
+  b.ld = r > 0 ? r : 1;
+  if ((size_t)r * c > 0) b.p = a.get<T>((size_t)r * c);
+  return b;
+}
+
+// Upload a host vector into a transient device buffer (synchronous copy;
+// the buffer lives in the given scratch arena).
+template <class D>
+D* upload(Arena& a, const std::vector<D>& v, hipStream_t s) {
+  if (v.empty()) return nullptr;
+  D* d = (D*)a.alloc(v.size() * sizeof(D));
+  HIP_OK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(D), hipMemcpyHostToDevice, s));
+  HIP_OK(hipStreamSynchronize(s));
+  return d;
+}
+
+template <class T>
+struct GemmBatch {
+  std::vector<GemmDesc<T>> d;
+  // C = alpha op(A) op(B) + beta C ; dims from the blocks
+  void add(const Blk<T>& A, int opA, const Blk<T>& B, int opB, const Blk<T>& C, double2 alpha, double2 beta) {
+    int m = opA ? A.c : A.r;
+    int k = opA ? A.r : A.c;
+    int n = opB ? B.r : B.c;
+    if (m == 0 || n == 0) return;
+    GemmDesc<T> g;
+    g.A = A.p;
+    g.B = B.p;
+    g.C = C.p;
+    g.m = m;
+    g.n = n;
+    g.k = k;
+    g.lda = A.ld;
+    g.ldb = B.ld;
+    g.ldc = C.ld;
+    g.opA = opA;
+    g.opB = opB;
+    g.alpha = alpha;
+    g.beta = beta;
+    if (k == 0) {  // pure scaling of C
+      if (beta.x == 1.0 && beta.y == 0.0) return;
+      g.k = 0;
+    }
+    d.push_back(g);
+  }
+  void run(Arena& scratch, hipStream_t s) {
+    if (d.empty()) return;
+    std::vector<TileTask> tiles;
+    for (int i = 0; i < (int)d.size(); ++i) {
+      int tm = (d[i].m + 31) / 32, tn = (d[i].n + 31) / 32;
+      for (int a = 0; a < tm; ++a)
+        for (int b = 0; b < tn; ++b) tiles.push_back(TileTask{i, a, b});
+    }
+    auto* dd = upload(scratch, d, s);
+    auto* dt = upload(scratch, tiles, s);
+    launch_gemm<T>((int)tiles.size(), dt, dd, s);
+    d.clear();
+  }
+};
+
+template <class T>
+struct CopyBatch {
+  std::vector<CopyDesc<T>> d;
+  void add(const Blk<T>& src, const Blk<T>& dst, int trans = 0, double2 alpha = make_double2(1, 0), int acc = 0) {
+    int nr = dst.r, nc = dst.c;
+    if (nr == 0 || nc == 0) return;
+    CopyDesc<T> c;
+    c.src = src.p;
+    c.dst = dst.p;
+    c.nr = nr;
+    c.nc = nc;
+    c.lds = src.ld;
+    c.ldd = dst.ld;
+    c.trans = trans;
+    c.accumulate = acc;
+    c.alpha = alpha;
+    d.push_back(c);
+  }
+  void run(Arena& scratch, hipStream_t s) {
+    if (d.empty()) return;
+    auto* dd = upload(scratch, d, s);
+    launch_copy<T>((int)d.size(), dd, s);
+    d.clear();
+  }
+};
+
+template <class T>
+struct LUBatch {
+  std::vector<LUDesc<T>> d;
+  void add(const Blk<T>& A, int* piv) {
+    if (A.r == 0) return;
+    d.push_back(LUDesc<T>{A.p, piv, A.r, A.ld});
+  }
+  void run(Arena& scratch, hipStream_t s) {
+    if (d.empty()) return;
+    auto* dd = upload(scratch, d, s);
+    launch_lu<T>((int)d.size(), dd, s);
+    d.clear();
+  }
+};
+
+template <class T>
+struct SolveBatch {
+  std::vector<SolveDesc<T>> d;
+  int maxc = 0;
+  void add(const Blk<T>& LU, const int* piv, const Blk<T>& B) {
+    if (LU.r == 0 || B.c == 0) return;
+    d.push_back(SolveDesc<T>{LU.p, piv, B.p, LU.r, B.c, LU.ld, B.ld});
+    int ch = (B.c + 63) / 64;
+    if (ch > maxc) maxc = ch;
+  }
+  void run(Arena& scratch, hipStream_t s) {
+    if (d.empty()) return;
+    auto* dd = upload(scratch, d, s);
+    launch_lu_solve<T>((int)d.size(), maxc, dd, s);
+    d.clear();
+    maxc = 0;
+  }
+};
+
+}  // namespace aifmm

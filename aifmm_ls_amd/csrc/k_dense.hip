@@ -1,0 +1,517 @@
+// Batched small dense complex kernels for the hierarchical elimination:
+// LU with partial pivoting, multi-RHS LU solves, tiled GEMM with variable
+// sizes, column-pivoted Householder QR with a relative rank threshold, Q
+// formation and strided block copies.  One workgroup (4 wave64) per matrix
+// or per output tile; every batch is described by a device array of
+// descriptors built on the host.
+#include "common.h"
+#include "kernels.h"
+
+namespace aifmm {
+
+template <class T>
+__device__ __forceinline__ T czero();
+template <>
+__device__ __forceinline__ double2 czero<double2>() {
+  return mk(0.0, 0.0);
+}
+template <>
+__device__ __forceinline__ float2 czero<float2>() {
+  return mkf(0.f, 0.f);
+}
+template <class T>
+__device__ __forceinline__ T from2(double2 a);
+template <>
+__device__ __forceinline__ double2 from2<double2>(double2 a) {
+  return a;
+}
+template <>
+__device__ __forceinline__ float2 from2<float2>(double2 a) {
+  return mkf((float)a.x, (float)a.y);
+}
+template <class T>
+__device__ __forceinline__ T cdiv(T a, T b) {
+  using R = typename CplxOf<T>::real;
+  R d = b.x * b.x + b.y * b.y;
+  T r;
+  r.x = (a.x * b.x + a.y * b.y) / d;
+  r.y = (a.y * b.x - a.x * b.y) / d;
+  return r;
+}
+template <class T>
+__device__ __forceinline__ T cfma(T a, T b, T c) {  // c + a*b
+  T r;
+  r.x = c.x + a.x * b.x - a.y * b.y;
+  r.y = c.y + a.x * b.y + a.y * b.x;
+  return r;
+}
+template <class T>
+__device__ __forceinline__ T cfms(T a, T b, T c) {  // c - a*b
+  T r;
+  r.x = c.x - a.x * b.x + a.y * b.y;
+  r.y = c.y - a.x * b.y - a.y * b.x;
+  return r;
+}
+template <class T>
+__device__ __forceinline__ typename CplxOf<T>::real cabsT(T a) {
+  return hypot(a.x, a.y);
+}
+
+constexpr int NT = 256;
+
+template <class R>
+__device__ void block_argmax_first(R v, int i, R* shv, int* shi, R& outv, int& outi) {
+  // larger wins, ties -> lower index; i < 0 marks "none"
+  for (int o = 32; o > 0; o >>= 1) {
+    R ov = __shfl_down(v, o);
+    int oi = __shfl_down(i, o);
+    if (oi >= 0 && (i < 0 || ov > v || (ov == v && oi < i))) v = ov, i = oi;
+  }
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) shv[w] = v, shi[w] = i;
+  __syncthreads();
+  R bv = shv[0];
+  int bi = shi[0];
+  for (int k = 1; k < NT / 64; ++k) {
+    R ov = shv[k];
+    int oi = shi[k];
+    if (oi >= 0 && (bi < 0 || ov > bv || (ov == bv && oi < bi))) bv = ov, bi = oi;
+  }
+  __syncthreads();
+  outv = bv;
+  outi = bi;
+}
+
+template <class R>
+__device__ R block_sumR(R v, R* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  R r = 0;
+  for (int k = 0; k < NT / 64; ++k) r += sh[k];
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------------ GEMM
+// C = alpha op(A) op(B) + beta C, 32x32 output tile per workgroup, K step 16.
+template <class T>
+__global__ __launch_bounds__(NT) void k_gemm(const TileTask* __restrict__ tiles, const GemmDesc<T>* __restrict__ ds) {
+  TileTask tt = tiles[blockIdx.x];
+  const GemmDesc<T> d = ds[tt.g];
+  __shared__ T As[16][33];
+  __shared__ T Bs[16][33];
+  int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;  // 16 x 16 threads, 2x2 outputs each
+  int m0 = tt.tm * 32, n0 = tt.tn * 32;
+  T acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = czero<T>();
+  for (int k0 = 0; k0 < d.k; k0 += 16) {
+    for (int e = threadIdx.x; e < 512; e += NT) {
+      int mm = e & 31, kk = e >> 5;
+      int gm = m0 + mm, gk = k0 + kk;
+      T v = czero<T>();
+      if (gm < d.m && gk < d.k) {
+        if (d.opA == 0) v = d.A[gm + (size_t)d.lda * gk];
+        else v = cconj(d.A[gk + (size_t)d.lda * gm]);
+      }
+      As[kk][mm] = v;
+      int nn = e & 31;
+      int gn = n0 + nn;
+      T w = czero<T>();
+      if (gn < d.n && gk < d.k) {
+        if (d.opB == 0) w = d.B[gk + (size_t)d.ldb * gn];
+        else w = cconj(d.B[gn + (size_t)d.ldb * gk]);
+      }
+      Bs[kk][nn] = w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      T a0 = As[kk][tx], a1 = As[kk][tx + 16];
+      T b0 = Bs[kk][ty], b1 = Bs[kk][ty + 16];
+      acc[0][0] = cfma(a0, b0, acc[0][0]);
+      acc[0][1] = cfma(a0, b1, acc[0][1]);
+      acc[1][0] = cfma(a1, b0, acc[1][0]);
+      acc[1][1] = cfma(a1, b1, acc[1][1]);
+    }
+    __syncthreads();
+  }
+  T al = from2<T>(d.alpha), be = from2<T>(d.beta);
+  bool bz = d.beta.x == 0.0 && d.beta.y == 0.0;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      int gm = m0 + tx + 16 * a, gn = n0 + ty + 16 * b;
+      if (gm < d.m && gn < d.n) {
+        T* c = d.C + gm + (size_t)d.ldc * gn;
+        T v = cmul(al, acc[a][b]);
+        if (!bz) v = cfma(be, *c, v);
+        *c = v;
+      }
+    }
+}
+
+// ------------------------------------------------------------------ LU
+template <class T>
+__global__ __launch_bounds__(NT) void k_lu(const LUDesc<T>* __restrict__ ds) {
+  using R = typename CplxOf<T>::real;
+  const LUDesc<T> d = ds[blockIdx.x];
+  __shared__ R shv[4];
+  __shared__ int shi[4];
+  const int n = d.n, ld = d.lda;
+  T* A = d.A;
+  for (int k = 0; k < n; ++k) {
+    R bv = -1;
+    int bi = -1;
+    for (int i = k + threadIdx.x; i < n; i += NT) {
+      R v = cabsT(A[i + (size_t)ld * k]);
+      if (bi < 0 || v > bv) bv = v, bi = i;
+    }
+    R pv;
+    int p;
+    block_argmax_first(bv, bi, shv, shi, pv, p);
+    if (p < 0) p = k;
+    if (p != k)
+      for (int j = threadIdx.x; j < n; j += NT) {
+        T t = A[k + (size_t)ld * j];
+        A[k + (size_t)ld * j] = A[p + (size_t)ld * j];
+        A[p + (size_t)ld * j] = t;
+      }
+    if (threadIdx.x == 0) d.piv[k] = p;
+    __syncthreads();
+    T dk = A[k + (size_t)ld * k];
+    bool nz = dk.x != 0 || dk.y != 0;
+    if (nz)
+      for (int i = k + 1 + threadIdx.x; i < n; i += NT) A[i + (size_t)ld * k] = cdiv(A[i + (size_t)ld * k], dk);
+    __syncthreads();
+    int m = n - k - 1;
+    if (nz && m > 0)
+      for (int e = threadIdx.x; e < m * m; e += NT) {
+        int i = k + 1 + e % m, j = k + 1 + e / m;
+        A[i + (size_t)ld * j] = cfms(A[i + (size_t)ld * k], A[k + (size_t)ld * j], A[i + (size_t)ld * j]);
+      }
+    __syncthreads();
+  }
+}
+
+// grid = (matrix, 64-column chunk of the right-hand sides)
+template <class T>
+__global__ __launch_bounds__(NT) void k_lu_solve(const SolveDesc<T>* __restrict__ ds) {
+  const SolveDesc<T> d = ds[blockIdx.x];
+  int c0 = blockIdx.y * 64;
+  if (c0 >= d.nrhs) return;
+  int nc = d.nrhs - c0 < 64 ? d.nrhs - c0 : 64;
+  const int n = d.n, la = d.lda, lb = d.ldb;
+  const T* A = d.A;
+  T* B = d.B + (size_t)lb * c0;
+  if (threadIdx.x < nc) {
+    T* b = B + (size_t)lb * threadIdx.x;
+    for (int k = 0; k < n; ++k) {
+      int p = d.piv[k];
+      if (p != k) {
+        T t = b[k];
+        b[k] = b[p];
+        b[p] = t;
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = 0; k < n - 1; ++k) {
+    int m = n - k - 1;
+    for (int e = threadIdx.x; e < m * nc; e += NT) {
+      int i = k + 1 + e % m, c = e / m;
+      B[i + (size_t)lb * c] = cfms(A[i + (size_t)la * k], B[k + (size_t)lb * c], B[i + (size_t)lb * c]);
+    }
+    __syncthreads();
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    if (threadIdx.x < nc) B[k + (size_t)lb * threadIdx.x] = cdiv(B[k + (size_t)lb * threadIdx.x], A[k + (size_t)la * k]);
+    __syncthreads();
+    for (int e = threadIdx.x; e < k * nc; e += NT) {
+      int i = e % k, c = e / k;
+      B[i + (size_t)lb * c] = cfms(A[i + (size_t)la * k], B[k + (size_t)lb * c], B[i + (size_t)lb * c]);
+    }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ CPQR
+// Column-pivoted Householder QR.  Column norms are recomputed exactly after
+// each reflection.  Reflector: I - t v v^H with v_0 = 1, beta real,
+// t = conj((beta - c0)/beta), beta = -sign(Re c0) ||x||.
+template <class T>
+__global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
+  using R = typename CplxOf<T>::real;
+  const QRDesc<T> d = ds[blockIdx.x];
+  __shared__ R shv[4];
+  __shared__ int shi[4];
+  __shared__ R shs[4];
+  const int m = d.m, n = d.n, ld = d.lda;
+  T* A = d.A;
+  R* nrm = d.work;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int j = threadIdx.x; j < n; j += NT) d.perm[j] = j;
+  for (int j = w; j < n; j += NT / 64) {
+    R s = 0;
+    for (int i = lane; i < m; i += 64) s += cabs2(A[i + (size_t)ld * j]);
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) nrm[j] = sqrt(s);
+  }
+  __syncthreads();
+  int size = m < n ? m : n;
+  R maxpiv = 0;
+  for (int k = 0; k < size; ++k) {
+    R bv = -1;
+    int bi = -1;
+    for (int j = k + threadIdx.x; j < n; j += NT) {
+      R v = nrm[j];
+      if (bi < 0 || v > bv) bv = v, bi = j;
+    }
+    R pv;
+    int big;
+    block_argmax_first(bv, bi, shv, shi, pv, big);
+    if (big < 0) big = k;
+    if (big != k) {
+      for (int i = threadIdx.x; i < m; i += NT) {
+        T t = A[i + (size_t)ld * k];
+        A[i + (size_t)ld * k] = A[i + (size_t)ld * big];
+        A[i + (size_t)ld * big] = t;
+      }
+      if (threadIdx.x == 0) {
+        R t = nrm[k];
+        nrm[k] = nrm[big];
+        nrm[big] = t;
+        int q = d.perm[k];
+        d.perm[k] = d.perm[big];
+        d.perm[big] = q;
+      }
+    }
+    __syncthreads();
+    R tl = 0;
+    for (int i = k + 1 + threadIdx.x; i < m; i += NT) tl += cabs2(A[i + (size_t)ld * k]);
+    tl = block_sumR(tl, shs);
+    T c0 = A[k + (size_t)ld * k];
+    R beta;
+    T t;
+    const R tiny = (R)1e-300 > 0 ? (R)2.2250738585072014e-308 : (R)0;
+    (void)tiny;
+    if (tl == 0 && c0.y == 0) {
+      t = czero<T>();
+      beta = c0.x;
+    } else {
+      beta = sqrt(c0.x * c0.x + c0.y * c0.y + tl);
+      if (c0.x >= 0) beta = -beta;
+      T den = c0;
+      den.x -= beta;
+      for (int i = k + 1 + threadIdx.x; i < m; i += NT) A[i + (size_t)ld * k] = cdiv(A[i + (size_t)ld * k], den);
+      T bm = c0;
+      bm.x = beta - c0.x;
+      bm.y = -c0.y;
+      t = bm;
+      t.x /= beta;
+      t.y /= beta;
+      t = cconj(t);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      d.tau[k] = t;
+      T bb = czero<T>();
+      bb.x = beta;
+      A[k + (size_t)ld * k] = bb;
+    }
+    if (fabs(beta) > maxpiv) maxpiv = fabs(beta);
+    __syncthreads();
+    // apply to trailing columns, one wave per column; refresh the norms
+    for (int j = k + 1 + w; j < n; j += NT / 64) {
+      T* col = A + (size_t)ld * j;
+      const T* v = A + (size_t)ld * k;
+      T s = czero<T>();
+      for (int i = k + 1 + lane; i < m; i += 64) s = cfma(cconj(v[i]), col[i], s);
+      for (int o = 32; o > 0; o >>= 1) {
+        s.x += __shfl_xor(s.x, o);
+        s.y += __shfl_xor(s.y, o);
+      }
+      s = cadd(s, col[k]);
+      s = cmul(t, s);
+      if (lane == 0) col[k] = csub(col[k], s);
+      R nn = 0;
+      for (int i = k + 1 + lane; i < m; i += 64) {
+        T x = cfms(s, v[i], col[i]);
+        col[i] = x;
+        nn += cabs2(x);
+      }
+      for (int o = 32; o > 0; o >>= 1) nn += __shfl_xor(nn, o);
+      if (lane == 0) nrm[j] = sqrt(nn);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    int r = 0;
+    R pre = maxpiv * (R)d.thr;
+    for (int i = 0; i < size; ++i)
+      if (cabsT(A[i + (size_t)ld * i]) > pre) ++r;
+    *d.rank_out = r;
+    if (d.maxpiv_out) *d.maxpiv_out = (double)maxpiv;
+  }
+}
+
+// Q[:, :k] = H_0^H H_1^H ... H_{s-1}^H I[:, :k]; one wave per column.
+template <class T>
+__global__ __launch_bounds__(NT) void k_qform(const QFormDesc<T>* __restrict__ ds) {
+  const QFormDesc<T> d = ds[blockIdx.x];
+  const int m = d.m, lda = d.lda, ldq = d.ldq;
+  int size = m < d.nref ? m : d.nref;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int j = w; j < d.k; j += NT / 64) {
+    T* q = d.Q + (size_t)ldq * j;
+    for (int i = lane; i < m; i += 64) {
+      T z = czero<T>();
+      if (i == j) z.x = 1;
+      q[i] = z;
+    }
+    for (int h = size - 1; h >= 0; --h) {
+      const T* v = d.A + (size_t)lda * h;
+      T s = czero<T>();
+      for (int i = h + 1 + lane; i < m; i += 64) s = cfma(cconj(v[i]), q[i], s);
+      for (int o = 32; o > 0; o >>= 1) {
+        s.x += __shfl_xor(s.x, o);
+        s.y += __shfl_xor(s.y, o);
+      }
+      s = cadd(s, q[h]);
+      s = cmul(cconj(d.tau[h]), s);
+      if (lane == 0) q[h] = csub(q[h], s);
+      for (int i = h + 1 + lane; i < m; i += 64) q[i] = cfms(s, v[i], q[i]);
+    }
+  }
+}
+
+template <class T>
+__global__ __launch_bounds__(NT) void k_copy(const CopyDesc<T>* __restrict__ ds) {
+  const CopyDesc<T> d = ds[blockIdx.x];
+  T al = from2<T>(d.alpha);
+  int tot = d.nr * d.nc;
+  for (int e = threadIdx.x + blockIdx.y * NT; e < tot; e += NT * gridDim.y) {
+    int r = e % d.nr, c = e / d.nr;
+    T v;
+    if (d.trans == 1) v = cconj(d.src[c + (size_t)d.lds * r]);
+    else if (d.trans == 2) v = d.src[c + (size_t)d.lds * r];
+    else if (d.trans == 3) {
+      v = czero<T>();
+      if (r == c) v.x = 1;
+    } else if (d.trans == 4) v = czero<T>();
+    else v = d.src[r + (size_t)d.lds * c];
+    v = cmul(al, v);
+    T* o = d.dst + r + (size_t)d.ldd * c;
+    if (d.accumulate) v = cadd(v, *o);
+    *o = v;
+  }
+}
+
+// Frobenius norms of a batch of blocks (one workgroup each).
+template <class T>
+__global__ __launch_bounds__(NT) void k_fro(const CopyDesc<T>* __restrict__ ds, double* __restrict__ out) {
+  using R = typename CplxOf<T>::real;
+  __shared__ R sh[4];
+  const CopyDesc<T> d = ds[blockIdx.x];
+  R s = 0;
+  int tot = d.nr * d.nc;
+  for (int e = threadIdx.x; e < tot; e += NT) {
+    int r = e % d.nr, c = e / d.nr;
+    s += cabs2(d.src[r + (size_t)d.lds * c]);
+  }
+  s = block_sumR(s, sh);
+  if (threadIdx.x == 0) out[blockIdx.x] = sqrt((double)s);
+}
+
+template <class T>
+__global__ __launch_bounds__(NT) void k_cvt(const CvtDesc<T>* __restrict__ ds) {
+  const CvtDesc<T> d = ds[blockIdx.x];
+  for (int64_t e = threadIdx.x; e < d.n; e += NT) d.dst[e] = from2<T>(d.src[e]);
+}
+
+__global__ void k_d2f(const double2* __restrict__ s, float2* __restrict__ d, int64_t n) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t < n) d[t] = mkf((float)s[t].x, (float)s[t].y);
+}
+__global__ void k_f2d(const float2* __restrict__ s, double2* __restrict__ d, int64_t n) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t < n) d[t] = mk((double)s[t].x, (double)s[t].y);
+}
+
+// ------------------------------------------------------------ launchers
+template <class T>
+void launch_gemm(int ntiles, const TileTask* tiles, const GemmDesc<T>* d, hipStream_t s) {
+  if (!ntiles) return;
+  hipLaunchKernelGGL(k_gemm<T>, dim3(ntiles), dim3(NT), 0, s, tiles, d);
+  HIP_OK(hipGetLastError());
+}
+template <class T>
+void launch_lu(int nb, const LUDesc<T>* d, hipStream_t s) {
+  if (!nb) return;
+  hipLaunchKernelGGL(k_lu<T>, dim3(nb), dim3(NT), 0, s, d);
+  HIP_OK(hipGetLastError());
+}
+template <class T>
+void launch_lu_solve(int nb, int max_chunks, const SolveDesc<T>* d, hipStream_t s) {
+  if (!nb || !max_chunks) return;
+  hipLaunchKernelGGL(k_lu_solve<T>, dim3(nb, max_chunks), dim3(NT), 0, s, d);
+  HIP_OK(hipGetLastError());
+}
+template <class T>
+void launch_cpqr(int nb, const QRDesc<T>* d, hipStream_t s) {
+  if (!nb) return;
+  hipLaunchKernelGGL(k_cpqr<T>, dim3(nb), dim3(NT), 0, s, d);
+  HIP_OK(hipGetLastError());
+}
+template <class T>
+void launch_qform(int nb, const QFormDesc<T>* d, hipStream_t s) {
+  if (!nb) return;
+  hipLaunchKernelGGL(k_qform<T>, dim3(nb), dim3(NT), 0, s, d);
+  HIP_OK(hipGetLastError());
+}
+template <class T>
+void launch_copy(int nb, const CopyDesc<T>* d, hipStream_t s) {
+  if (!nb) return;
+  hipLaunchKernelGGL(k_copy<T>, dim3(nb, 4), dim3(NT), 0, s, d);
+  HIP_OK(hipGetLastError());
+}
+template <class T>
+void launch_fro(int nb, const CopyDesc<T>* d, double* out, hipStream_t s) {
+  if (!nb) return;
+  hipLaunchKernelGGL(k_fro<T>, dim3(nb), dim3(NT), 0, s, d, out);
+  HIP_OK(hipGetLastError());
+}
+template <class T>
+void launch_cvt(int nb, const CvtDesc<T>* d, hipStream_t s) {
+  if (!nb) return;
+  hipLaunchKernelGGL(k_cvt<T>, dim3(nb), dim3(NT), 0, s, d);
+  HIP_OK(hipGetLastError());
+}
+void launch_d2f(const double2* src, float2* dst, int64_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_d2f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+  HIP_OK(hipGetLastError());
+}
+void launch_f2d(const float2* src, double2* dst, int64_t n, hipStream_t s) {
+  if (!n) return;
+  hipLaunchKernelGGL(k_f2d, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);
+  HIP_OK(hipGetLastError());
+}
+
+#define INST(T)                                                                         \
+  template void launch_gemm<T>(int, const TileTask*, const GemmDesc<T>*, hipStream_t);  \
+  template void launch_lu<T>(int, const LUDesc<T>*, hipStream_t);                       \
+  template void launch_lu_solve<T>(int, int, const SolveDesc<T>*, hipStream_t);         \
+  template void launch_cpqr<T>(int, const QRDesc<T>*, hipStream_t);                     \
+  template void launch_qform<T>(int, const QFormDesc<T>*, hipStream_t);                 \
+  template void launch_copy<T>(int, const CopyDesc<T>*, hipStream_t);                   \
+  template void launch_fro<T>(int, const CopyDesc<T>*, double*, hipStream_t);            \
+  template void launch_cvt<T>(int, const CvtDesc<T>*, hipStream_t);
+INST(double2)
+INST(float2)
+
+}  // namespace aifmm

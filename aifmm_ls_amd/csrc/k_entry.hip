@@ -1,0 +1,393 @@
+// Matrix-entry evaluation A(i,j) of the 2D Lippmann–Schwinger operator on
+// gfx950, and the kernels built on it: entry lists, block assembly from
+// index lists, and the batched ACA ("NCA") used to pick skeletons.
+//
+// A(i,j) follows LS_Assembly.hpp:474-541.  Near (source leaf adjacent to or
+// equal to the target leaf): table T[slot](i%R, j%R) * kappa * kappa * q(x_i)
+// (+1 on the diagonal).  Far: (i/4) * kappa^2 q(x_i) * sum_{n=-60}^{60}
+// M(n+60, j%R) H_n^(1)(kappa rho) e^{i n phi}, (rho, phi) = polar(x_i - c_j).
+// Entries below 1e-16 in modulus are zeroed.
+#include "common.h"
+#include "kernels.h"
+
+namespace aifmm {
+
+__device__ __forceinline__ double hypot2(double2 a) { return hypot(a.x, a.y); }
+
+__constant__ int c_nb_off[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {1, 0}, {1, 1}, {0, 1}, {-1, 1}, {-1, 0}};
+
+// J_n(x), n = 0..60, by Miller's backward recurrence normalised with
+// J_0 + 2 sum_k J_2k = 1.  All scalings are by powers of two so they do not
+// change rounding; contraction into FMA is disabled so the recurrence
+// rounds exactly as a plain IEEE evaluation.
+__device__ __forceinline__ void bessel_J61(double x, double (&J)[SERIES + 1]) {
+#pragma clang fp contract(off)
+  int big = x > (double)SERIES ? (int)x : SERIES;
+  int Mst = big + 30 + (int)sqrt(40.0 * big);
+  if (Mst & 1) ++Mst;
+  double jp1 = 0.0, jc = 1e-300, sum = 0.0;
+  for (int k = Mst; k > SERIES + 1; --k) {
+    double jm1 = (2.0 * k / x) * jc - jp1;
+    jp1 = jc;
+    jc = jm1;
+    if (fabs(jc) > 0x1p800) {
+      jc *= 0x1p-800;
+      jp1 *= 0x1p-800;
+      sum *= 0x1p-800;
+    }
+    if (((k - 1) & 1) == 0) sum += 2.0 * jc;
+  }
+  // bring j_61 to ~2^-500 with an exact power-of-two scale
+  {
+    int e = ilogb(jc);
+    double f = ldexp(1.0, -500 - e);
+    jc *= f;
+    jp1 *= f;
+    sum *= f;
+  }
+#pragma unroll
+  for (int k = SERIES + 1; k >= 1; --k) {
+    double jm1 = (2.0 * k / x) * jc - jp1;
+    jp1 = jc;
+    jc = jm1;
+    J[k - 1] = jc;
+    if (k - 1 > 0 && ((k - 1) & 1) == 0) sum += 2.0 * jc;
+  }
+  sum += jc;
+#pragma unroll
+  for (int n = 0; n <= SERIES; ++n) J[n] = J[n] / sum;
+}
+
+// sum_{n=-60}^{60} M(n+60, col) H_n(x) e^{i n phi} with H_{-n} = (-1)^n H_n.
+__device__ __forceinline__ double2 graf_sum(const double2* __restrict__ Mcol, double x, double phi) {
+#pragma clang fp contract(off)
+  double J[SERIES + 1];
+  bessel_J61(x, J);
+  double y0v = y0(x), y1v = y1(x);
+  double sn, cs;
+  sincos(phi, &sn, &cs);
+  double2 e1 = mk(cs, sn);
+  double2 pw = mk(1.0, 0.0);
+  double2 acc = mk(0.0, 0.0);
+  double ym1 = 0.0, yc = y0v;
+#pragma unroll
+  for (int n = 0; n <= SERIES; ++n) {
+    if (n == 1) {
+      ym1 = y0v;
+      yc = y1v;
+    } else if (n > 1) {
+      double yn = (2.0 * (n - 1) / x) * yc - ym1;
+      ym1 = yc;
+      yc = yn;
+    }
+    double2 H = mk(J[n], yc);
+    double2 c = cmul(Mcol[SERIES + n], pw);
+    if (n > 0) {
+      double2 t = cmul(Mcol[SERIES - n], cconj(pw));
+      if (n & 1) c = csub(c, t);
+      else c = cadd(c, t);
+    }
+    acc = cadd(acc, cmul(H, c));
+    pw = cmul(pw, e1);
+  }
+  return acc;
+}
+
+__device__ double2 ls_entry(const EntryCtx& C, int64_t i, int64_t j) {
+#pragma clang fp contract(off)
+  int R = C.R;
+  int li = (int)(i / R), ii = (int)(i - (int64_t)li * R);
+  int lj = (int)(j / R), jj = (int)(j - (int64_t)lj * R);
+  int xi, yi, xj, yj;
+  box_xy(C.L, li, xi, yi);
+  box_xy(C.L, lj, xj, yj);
+  int dx = xj - xi, dy = yj - yi;
+  double qi = C.q[i];
+  if (dx >= -1 && dx <= 1 && dy >= -1 && dy <= 1) {
+    int nb = 8;
+    if (dx || dy)
+      for (int s = 0; s < 8; ++s)
+        if (c_nb_off[s][0] == dx && c_nb_off[s][1] == dy) nb = s;
+    double2 e = C.T[(size_t)nb * R * R + ii + (size_t)R * jj];
+    e = cscale(e, C.kappa);
+    e = cscale(e, C.kappa);
+    e = cscale(e, qi);
+    if (i == j) e.x = 1.0 + e.x;
+    if (hypot(e.x, e.y) < 1e-16) e = mk(0.0, 0.0);
+    return e;
+  }
+  double cx = -C.Ldom + (2 * xj + 1) * C.rleaf, cy = -C.Ldom + (2 * yj + 1) * C.rleaf;
+  double ddy = C.py[i] - cy, ddx = C.px[i] - cx;
+  double phi = atan2(ddy, ddx);
+  double R2 = ddy * ddy + ddx * ddx;
+  double arg = C.kappa * sqrt(R2);
+  double2 s = graf_sum(C.M + (size_t)NTERM * jj, arg, phi);
+  s = cmul(s, mk(0.0, 0.25));
+  s = cscale(s, C.kappa * C.kappa * qi);
+  if (i == j) s.x += 1.0;
+  if (hypot(s.x, s.y) < 1e-16) s = mk(0.0, 0.0);
+  return s;
+}
+
+__global__ void k_entries(EntryCtx C, int64_t n, const int64_t* __restrict__ I, const int64_t* __restrict__ Jv,
+                          double2* __restrict__ out) {
+  int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  if (t < n) out[t] = ls_entry(C, I[t], Jv[t]);
+}
+
+// One workgroup per block descriptor: out(r, c) = A(rows[r], cols[c]),
+// column-major with leading dimension ld.
+__global__ void k_assemble(EntryCtx C, const BlockDesc* __restrict__ d) {
+  BlockDesc b = d[blockIdx.x];
+  int tot = b.nr * b.nc;
+  for (int t = threadIdx.x; t < tot; t += blockDim.x) {
+    int r = t % b.nr, c = t / b.nr;
+    b.out[r + (size_t)b.ld * c] = ls_entry(C, b.rows[r], b.cols[c]);
+  }
+}
+
+// ---------------------------------------------------------------- ACA
+// LowRank::ACA_only_nodes (ACA.hpp:368-577) in "primary/secondary" form:
+// the primary index set P is the shorter dimension (rows when N1 < N2, else
+// columns); a "line" is A restricted to one P index (length ns), a "cross"
+// is A restricted to one S index (length np).  One workgroup per problem.
+constexpr int ACA_THREADS = 256;
+
+struct RedPair {
+  double v;
+  int i;
+};
+
+__device__ __forceinline__ RedPair red_max(RedPair a, RedPair b) {
+  // larger magnitude wins; ties go to the lower index (first strict max)
+  if (b.i < 0) return a;
+  if (a.i < 0) return b;
+  if (b.v > a.v || (b.v == a.v && b.i < a.i)) return b;
+  return a;
+}
+
+__device__ RedPair block_argmax(RedPair p, RedPair* sh) {
+  int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) {
+    RedPair q;
+    q.v = __shfl_down(p.v, o);
+    q.i = __shfl_down(p.i, o);
+    p = red_max(p, q);
+  }
+  if (lane == 0) sh[w] = p;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    RedPair r = sh[0];
+    for (int k = 1; k < ACA_THREADS / 64; ++k) r = red_max(r, sh[k]);
+    sh[4] = r;
+  }
+  __syncthreads();
+  RedPair r = sh[4];
+  __syncthreads();
+  return r;
+}
+
+__device__ double block_sum(double v, double* sh) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0;
+  for (int k = 0; k < ACA_THREADS / 64; ++k) r += sh[k];
+  __syncthreads();
+  return r;
+}
+
+__device__ __forceinline__ double2 aca_entry(const EntryCtx& C, const AcaProblem& p, int pi, int si) {
+  // primary index pi, secondary index si -> A(row, col)
+  if (p.prim_rows) return ls_entry(C, p.rows[pi], p.cols[si]);
+  return ls_entry(C, p.rows[si], p.cols[pi]);
+}
+
+__global__ __launch_bounds__(ACA_THREADS) void k_aca(EntryCtx C, const AcaProblem* __restrict__ probs, double tol,
+                                                     double tol_robust) {
+#pragma clang fp contract(off)
+  const AcaProblem p = probs[blockIdx.x];
+  const int np = p.np, ns = p.ns;
+  __shared__ RedPair shp[8];
+  __shared__ double shd[8];
+  __shared__ int s_flag, s_pi, s_si, s_rank;
+  __shared__ double s_normS, s_prev;
+  __shared__ double2 s_piv;
+  double2* line = p.line;    // ns
+  double2* crossr = p.cross; // np (residual)
+  char* remP = p.remP;
+  char* remS = p.remS;
+  for (int t = threadIdx.x; t < np; t += blockDim.x) remP[t] = 1;
+  for (int t = threadIdx.x; t < ns; t += blockDim.x) remS[t] = 1;
+  if (threadIdx.x == 0) s_rank = 0;
+  __syncthreads();
+  if (np == 0 || ns == 0) {
+    if (threadIdx.x == 0) *p.rank_out = 0;
+    return;
+  }
+  // first primary index whose line has a pivot above tol_robust
+  int pi = -1, si = 0;
+  for (int l = 0; l < np; ++l) {
+    RedPair best{-1.0, -1};
+    for (int t = threadIdx.x; t < ns; t += blockDim.x) {
+      double2 e = aca_entry(C, p, l, t);
+      line[t] = e;
+      RedPair c{hypot2(e), t};
+      best = red_max(best, c);
+    }
+    __syncthreads();
+    RedPair r = block_argmax(best, shp);
+    if (r.v > tol_robust) {
+      pi = l;
+      si = r.i;
+      break;
+    }
+  }
+  if (pi < 0) {
+    if (threadIdx.x == 0) *p.rank_out = 0;
+    return;
+  }
+  const int mn = np < ns ? np : ns;
+  double2 piv = line[si];
+  // v_0 = line / pivot ; cross_0 = raw cross at si ; u_0 = cross_0
+  for (int t = threadIdx.x; t < ns; t += blockDim.x) {
+    double2 a = line[t];
+    double d = cabs2(piv);
+    p.V[t] = mk((a.x * piv.x + a.y * piv.y) / d, (a.y * piv.x - a.x * piv.y) / d);
+    p.Lraw[t] = a;
+  }
+  for (int t = threadIdx.x; t < np; t += blockDim.x) {
+    double2 e = aca_entry(C, p, t, si);
+    p.Craw[t] = e;
+    p.U[t] = e;
+    crossr[t] = e;
+  }
+  if (threadIdx.x == 0) {
+    p.pb[0] = pi;
+    p.sb[0] = si;
+    remS[si] = 0;
+    remP[pi] = 0;
+    s_normS = 0.0;
+    s_prev = 1.7976931348623157e308;
+    s_rank = 1;
+  }
+  __syncthreads();
+  int rank = 1;
+  // next primary candidate from the cross
+  {
+    RedPair best{-1.0, -1};
+    for (int t = threadIdx.x; t < np; t += blockDim.x)
+      if (remP[t]) best = red_max(best, RedPair{hypot2(crossr[t]), t});
+    RedPair r = block_argmax(best, shp);
+    pi = r.i < 0 ? 0 : r.i;
+  }
+  double lastpiv = hypot2(piv);
+  while (true) {
+    double cpi = hypot2(crossr[pi]);
+    if (!(lastpiv > tol && cpi > tol && fabs(s_prev - s_normS) >= tol * s_prev && rank < mn)) break;
+    if (rank >= p.rmax) break;
+    // residual line at pi
+    for (int t = threadIdx.x; t < ns; t += blockDim.x) {
+      double2 e = aca_entry(C, p, pi, t);
+      p.Lraw[(size_t)rank * ns + t] = e;
+      for (int l = 0; l < rank; ++l) e = csub(e, cmul(p.U[(size_t)l * np + pi], p.V[(size_t)l * ns + t]));
+      line[t] = e;
+    }
+    __syncthreads();
+    RedPair best{-1.0, -1};
+    for (int t = threadIdx.x; t < ns; t += blockDim.x)
+      if (remS[t]) best = red_max(best, RedPair{hypot2(line[t]), t});
+    RedPair r = block_argmax(best, shp);
+    si = r.i < 0 ? 0 : r.i;
+    piv = line[si];
+    double2* Vn = p.V + (size_t)rank * ns;
+    double vv = 0;
+    for (int t = threadIdx.x; t < ns; t += blockDim.x) {
+      double2 a = line[t];
+      double d = cabs2(piv);
+      double2 v = mk((a.x * piv.x + a.y * piv.y) / d, (a.y * piv.x - a.x * piv.y) / d);
+      Vn[t] = v;
+      vv += cabs2(v);
+    }
+    double2* Un = p.U + (size_t)rank * np;
+    double uu = 0;
+    for (int t = threadIdx.x; t < np; t += blockDim.x) {
+      double2 e = aca_entry(C, p, t, si);
+      p.Craw[(size_t)rank * np + t] = e;
+      for (int l = 0; l < rank; ++l) e = csub(e, cmul(p.V[(size_t)l * ns + si], p.U[(size_t)l * np + t]));
+      Un[t] = e;
+      crossr[t] = e;
+      uu += cabs2(e);
+    }
+    __syncthreads();
+    double unorm = sqrt(block_sum(uu, shd));
+    double vnorm = sqrt(block_sum(vv, shd));
+    if (unorm < tol_robust || vnorm < tol_robust) break;
+    // normS update: normS^2 + (|u||v|)^2 + sum_l 2|(U_l^H u)(V_l^H v)| over
+    // l = 0..rank (the new pair included, as the reference does).  One wave
+    // per l, no workgroup barriers inside the loop.
+    double accw = 0;
+    {
+      int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+      for (int l = w; l <= rank; l += ACA_THREADS / 64) {
+        double2 du = mk(0, 0), dv = mk(0, 0);
+        for (int t = lane; t < np; t += 64) du = cadd(du, cmul(cconj(p.U[(size_t)l * np + t]), Un[t]));
+        for (int t = lane; t < ns; t += 64) dv = cadd(dv, cmul(cconj(p.V[(size_t)l * ns + t]), Vn[t]));
+        for (int o = 32; o > 0; o >>= 1) {
+          du.x += __shfl_xor(du.x, o);
+          du.y += __shfl_xor(du.y, o);
+          dv.x += __shfl_xor(dv.x, o);
+          dv.y += __shfl_xor(dv.y, o);
+        }
+        double2 pr = cmul(du, dv);
+        accw += 2.0 * hypot(pr.x, pr.y);
+      }
+      if (lane == 0) shd[w] = accw;
+      __syncthreads();
+    }
+    double accn = 0;
+    for (int k = 0; k < ACA_THREADS / 64; ++k) accn += shd[k];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      p.pb[rank] = pi;
+      p.sb[rank] = si;
+      remS[si] = 0;
+      remP[pi] = 0;
+    }
+    ++rank;
+    if (rank != 2) s_prev = s_normS;  // uniform across threads
+    double ns2 = s_normS * s_normS + (unorm * vnorm) * (unorm * vnorm) + accn;
+    __syncthreads();
+    if (threadIdx.x == 0) s_normS = sqrt(ns2);
+    lastpiv = hypot2(piv);
+    __syncthreads();
+    RedPair b2{-1.0, -1};
+    for (int t = threadIdx.x; t < np; t += blockDim.x)
+      if (remP[t]) b2 = red_max(b2, RedPair{hypot2(crossr[t]), t});
+    RedPair r2 = block_argmax(b2, shp);
+    pi = r2.i < 0 ? 0 : r2.i;
+  }
+  if (threadIdx.x == 0) *p.rank_out = rank;
+}
+
+// ------------------------------------------------------------ launchers
+void launch_entries(const EntryCtx& C, int64_t n, const int64_t* I, const int64_t* J, double2* out, hipStream_t s) {
+  if (n == 0) return;
+  int th = 128;
+  hipLaunchKernelGGL(k_entries, dim3((unsigned)((n + th - 1) / th)), dim3(th), 0, s, C, n, I, J, out);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_assemble(const EntryCtx& C, int nblk, const BlockDesc* d, hipStream_t s) {
+  if (nblk == 0) return;
+  hipLaunchKernelGGL(k_assemble, dim3(nblk), dim3(256), 0, s, C, d);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_aca(const EntryCtx& C, int nprob, const AcaProblem* d, double tol, double tol_robust, hipStream_t s) {
+  if (nprob == 0) return;
+  hipLaunchKernelGGL(k_aca, dim3(nprob), dim3(ACA_THREADS), 0, s, C, d, tol, tol_robust);
+  HIP_OK(hipGetLastError());
+}
+
+}  // namespace aifmm

@@ -1,0 +1,150 @@
+// Descriptor structs and launchers of the gfx950 kernels.
+#pragma once
+#include "common.h"
+
+namespace aifmm {
+
+// A(rows[r], cols[c]) -> out[r + ld*c]
+struct BlockDesc {
+  const int64_t* rows;
+  const int64_t* cols;
+  double2* out;
+  int nr, nc, ld;
+};
+
+// One ACA problem (see k_aca).  Primary = shorter dimension.
+struct AcaProblem {
+  const int64_t* rows;  // N1 point indices
+  const int64_t* cols;  // N2 point indices
+  int np, ns;           // primary / secondary lengths
+  int prim_rows;        // 1: primary = rows (N1 < N2)
+  int rmax;             // capacity of U/V/Craw
+  double2* U;           // rmax x np   residual crosses
+  double2* V;           // rmax x ns   normalised lines
+  double2* Craw;        // rmax x np   raw crosses
+  double2* Lraw;        // rmax x ns   raw lines
+  double2* line;        // ns
+  double2* cross;       // np
+  char* remP;           // np
+  char* remS;           // ns
+  int* pb;              // rmax primary pivots
+  int* sb;              // rmax secondary pivots
+  int* rank_out;
+};
+
+void launch_entries(const EntryCtx& C, int64_t n, const int64_t* I, const int64_t* J, double2* out, hipStream_t s);
+void launch_assemble(const EntryCtx& C, int nblk, const BlockDesc* d, hipStream_t s);
+void launch_aca(const EntryCtx& C, int nprob, const AcaProblem* d, double tol, double tol_robust, hipStream_t s);
+
+// ----------------------------------------------------------- quadrature
+struct QuadTask {
+  int kind;  // 0: far C(n,l) ; 1: near V(p,l) for colleague slot nb
+  int n;     // Bessel order (far) or neighbour slot (near)
+  int p;     // target Chebyshev point (near)
+  int l;     // basis index
+};
+struct QuadCtx {
+  const int* basis_a;
+  const int* basis_b;
+  const double* cheb1d;
+  int n_cheb;
+  double kr;  // kappa * r_leaf
+  double eps;
+  const double* gl_t;  // 10 nodes
+  const double* gl_w;  // 10 weights
+};
+void launch_quadrature(const QuadCtx& Q, int ntask, const QuadTask* tasks, double2* out, int* depth_flag,
+                       hipStream_t s);
+
+// ----------------------------------------------------------- dense batched
+// op: 0 = N, 1 = conjugate transpose
+template <class T>
+struct GemmDesc {
+  const T* A;
+  const T* B;
+  T* C;
+  int m, n, k;
+  int lda, ldb, ldc;
+  int opA, opB;
+  double2 alpha, beta;
+};
+struct TileTask {
+  int g, tm, tn;
+};
+
+template <class T>
+struct LUDesc {
+  T* A;
+  int* piv;
+  int n, lda;
+};
+
+template <class T>
+struct SolveDesc {
+  const T* A;  // LU factors
+  const int* piv;
+  T* B;
+  int n, nrhs, lda, ldb;
+};
+
+template <class T>
+struct QRDesc {
+  T* A;         // m x n, overwritten with R and Householder vectors
+  T* tau;       // min(m,n)
+  int* perm;    // n
+  typename CplxOf<T>::real* work;  // n column norms
+  int m, n, lda;
+  double thr;   // relative rank threshold
+  int* rank_out;
+  double* maxpiv_out;
+};
+
+template <class T>
+struct QFormDesc {
+  const T* A;  // Householder vectors from QRDesc
+  const T* tau;
+  T* Q;        // m x k output
+  int m, nref, k, lda, ldq;
+};
+
+// dst(r, c) = alpha * op(src)(r, c) (+ dst if accumulate); op: trans 0 = none,
+// 1 = conjugate transpose, 2 = transpose, 3 = identity (src unused),
+// 4 = zero (src unused).
+template <class T>
+struct CopyDesc {
+  const T* src;
+  T* dst;
+  int nr, nc, lds, ldd;
+  int trans;
+  int accumulate;
+  double2 alpha;
+};
+
+// dst[i] = (T) src[i], i < n (FP64 operator blocks -> working precision)
+template <class T>
+struct CvtDesc {
+  const double2* src;
+  T* dst;
+  int64_t n;
+};
+template <class T>
+void launch_cvt(int nb, const CvtDesc<T>* d, hipStream_t s);
+
+template <class T>
+void launch_gemm(int ntiles, const TileTask* tiles, const GemmDesc<T>* d, hipStream_t s);
+template <class T>
+void launch_lu(int nb, const LUDesc<T>* d, hipStream_t s);
+template <class T>
+void launch_lu_solve(int nb, int max_chunks, const SolveDesc<T>* d, hipStream_t s);
+template <class T>
+void launch_cpqr(int nb, const QRDesc<T>* d, hipStream_t s);
+template <class T>
+void launch_qform(int nb, const QFormDesc<T>* d, hipStream_t s);
+template <class T>
+void launch_copy(int nb, const CopyDesc<T>* d, hipStream_t s);
+template <class T>
+void launch_fro(int nb, const CopyDesc<T>* d, double* out, hipStream_t s);
+void launch_d2f(const double2* src, float2* dst, int64_t n, hipStream_t s);
+void launch_f2d(const float2* src, double2* dst, int64_t n, hipStream_t s);
+
+}  // namespace aifmm

@@ -1,0 +1,187 @@
+"""MI355X-native 2D Lippmann-Schwinger direct solver (AIFMM_LS solve path).
+
+Thin ctypes binding of the in-tree C ABI library ``_lib/libaifmm_ls.so``
+(declared in ``include/aifmm_ls.h``).  All arithmetic runs in hand-written
+gfx950 HIP kernels; there is no CPU fallback: if the library or a GPU is
+missing, every entry point raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libaifmm_ls.so")
+_lib = None
+
+PREC_C128, PREC_C64 = 0, 1
+
+
+class AifmmError(RuntimeError):
+    pass
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("n_cheb", ctypes.c_int), ("degree", ctypes.c_int), ("n_levels", ctypes.c_int),
+                ("tol_pow", ctypes.c_int), ("qchoice", ctypes.c_int), ("quantize_tables", ctypes.c_int),
+                ("kappa", ctypes.c_double), ("L", ctypes.c_double),
+                ("bumps", ctypes.POINTER(ctypes.c_double)), ("n_bumps", ctypes.c_int)]
+
+
+def lib():
+    """Load the C ABI library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AifmmError(f"{LIB_PATH} missing: run `python -m aifmm_ls_amd.build` (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        L.aifmm_ls_create.argtypes = [ctypes.POINTER(_Params), ctypes.POINTER(ctypes.c_void_p)]
+        L.aifmm_ls_destroy.argtypes = [P]
+        L.aifmm_ls_last_error.restype = ctypes.c_char_p
+        L.aifmm_ls_size.restype = ctypes.c_int64
+        L.aifmm_ls_size.argtypes = [P]
+        L.aifmm_ls_points.argtypes = [P, dp, dp]
+        L.aifmm_ls_tables.argtypes = [P, dp, dp, ctypes.POINTER(ctypes.c_int)]
+        L.aifmm_ls_entries.argtypes = [P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.POINTER(ctypes.c_int64), dp]
+        L.aifmm_ls_rhs_planewave.argtypes = [P, ctypes.c_double, dp]
+        L.aifmm_ls_assemble.argtypes = [P]
+        L.aifmm_ls_ranks.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.aifmm_ls_skeleton.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)]
+        L.aifmm_ls_factor.argtypes = [P, ctypes.c_int]
+        L.aifmm_ls_solve.argtypes = [P, ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_int, dp]
+        L.aifmm_ls_set_rhs.argtypes = [P, ctypes.c_int, dp]
+        L.aifmm_ls_solve_resident.argtypes = [P, ctypes.c_double, ctypes.c_int, dp]
+        L.aifmm_ls_get_solution.argtypes = [P, dp]
+        L.aifmm_ls_stats.argtypes = [P, dp, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _chk(rc):
+    if rc != 0:
+        raise AifmmError(lib().aifmm_ls_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+STAT_NAMES = ["t_tables", "t_assemble", "t_factor", "factor_flops", "factor_bytes", "base_n", "max_rank",
+              "solve_bytes_1rhs", "recompressed_boxes", "operator_bytes", "t_solve", "t_nca"]
+
+
+class LSProblem:
+    """The reference's userkernel + FMM2DTree pipeline on one GPU.
+
+    Arguments follow the reference driver's argv (testFMM2D_find_error.cpp:44-52):
+    n_cheb, kappa, degree, L, qchoice, tol_pow, n_levels; qchoice 6 takes
+    ``bumps`` = rows of (a, w, cx, cy).
+    """
+
+    def __init__(self, n_levels, kappa, tol_pow=8, qchoice=0, n_cheb=6, degree=6, L=1.0, bumps=None,
+                 quantize=True):
+        p = _Params()
+        p.n_cheb, p.degree, p.n_levels, p.tol_pow, p.qchoice = n_cheb, degree, n_levels, tol_pow, qchoice
+        p.quantize_tables = int(quantize)
+        p.kappa, p.L = float(kappa), float(L)
+        self._bumps = None
+        if bumps is not None:
+            self._bumps = np.ascontiguousarray(np.asarray(bumps, np.float64).reshape(-1, 4))
+            p.bumps = _dp(self._bumps)
+            p.n_bumps = self._bumps.shape[0]
+        h = ctypes.c_void_p()
+        _chk(lib().aifmm_ls_create(ctypes.byref(p), ctypes.byref(h)))
+        self.h = h
+        self.N = int(lib().aifmm_ls_size(h))
+        self.n_levels, self.kappa, self.tol_pow = n_levels, kappa, tol_pow
+        self.R = n_cheb * n_cheb
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().aifmm_ls_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def points(self):
+        xy = np.zeros(2 * self.N)
+        q = np.zeros(self.N)
+        _chk(lib().aifmm_ls_points(self.h, _dp(xy), _dp(q)))
+        return xy.reshape(-1, 2), q
+
+    def tables(self):
+        M = np.zeros(2 * 121 * self.R)
+        T = np.zeros(2 * 9 * self.R * self.R)
+        c = ctypes.c_int()
+        _chk(lib().aifmm_ls_tables(self.h, _dp(M), _dp(T), ctypes.byref(c)))
+        M = M.view(np.complex128).reshape(self.R, 121).T
+        T = T.view(np.complex128).reshape(9, self.R, self.R).transpose(0, 2, 1)
+        return M, T, c.value
+
+    def entries(self, I, J):
+        I = np.ascontiguousarray(I, np.int64)
+        J = np.ascontiguousarray(J, np.int64)
+        out = np.zeros(2 * I.size)
+        _chk(lib().aifmm_ls_entries(self.h, I.size, I.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                    J.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _dp(out)))
+        return out.view(np.complex128)
+
+    def rhs(self, theta=0.0):
+        b = np.zeros(2 * self.N)
+        _chk(lib().aifmm_ls_rhs_planewave(self.h, float(theta), _dp(b)))
+        return b.view(np.complex128)
+
+    def assemble(self):
+        _chk(lib().aifmm_ls_assemble(self.h))
+
+    def ranks(self, level):
+        out = np.zeros(4 ** level, np.int32)
+        _chk(lib().aifmm_ls_ranks(self.h, level, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int))))
+        return out
+
+    def skeleton(self, level, box, which):
+        out = np.zeros(4 * 36 * 64, np.int64)
+        n = ctypes.c_int(out.size)
+        _chk(lib().aifmm_ls_skeleton(self.h, level, box, which, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                     ctypes.byref(n)))
+        return out[:n.value].copy()
+
+    def factor(self, precision=PREC_C128):
+        _chk(lib().aifmm_ls_factor(self.h, precision))
+
+    def solve(self, b, tol=1e-6, max_refine=0):
+        b = np.asarray(b, np.complex128)
+        nrhs = 1 if b.ndim == 1 else b.shape[1]
+        bb = np.ascontiguousarray(b.reshape(self.N, nrhs).T).view(np.float64)  # column-major N x nrhs
+        x = np.zeros(2 * self.N * nrhs)
+        st = np.zeros(8)
+        _chk(lib().aifmm_ls_solve(self.h, nrhs, _dp(bb), _dp(x), tol, max_refine, _dp(st)))
+        x = x.view(np.complex128).reshape(nrhs, self.N).T
+        return (x[:, 0] if b.ndim == 1 else x), st
+
+    def set_rhs(self, b):
+        b = np.asarray(b, np.complex128)
+        nrhs = 1 if b.ndim == 1 else b.shape[1]
+        bb = np.ascontiguousarray(b.reshape(self.N, nrhs).T).view(np.float64)
+        _chk(lib().aifmm_ls_set_rhs(self.h, nrhs, _dp(bb)))
+        self._nrhs = nrhs
+
+    def solve_resident(self, tol=1e-6, max_refine=0):
+        st = np.zeros(8)
+        _chk(lib().aifmm_ls_solve_resident(self.h, tol, max_refine, _dp(st)))
+        return st
+
+    def get_solution(self):
+        x = np.zeros(2 * self.N * self._nrhs)
+        _chk(lib().aifmm_ls_get_solution(self.h, _dp(x)))
+        return x.view(np.complex128).reshape(self._nrhs, self.N).T
+
+    def stats(self):
+        out = np.zeros(len(STAT_NAMES))
+        _chk(lib().aifmm_ls_stats(self.h, _dp(out), len(STAT_NAMES)))
+        return dict(zip(STAT_NAMES, out.tolist()))

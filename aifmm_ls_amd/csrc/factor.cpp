@@ -1,0 +1,804 @@
+// Level-synchronous colour-batched elimination (9 colour classes per level:
+// boxes with equal (ix mod 3, iy mod 3) have disjoint 3x3 neighbourhoods, so
+// their eliminations commute and write disjoint blocks), followed after each
+// colour by batched recompression of the distance-2 fill-ins.
+//
+// Per box k (reference: eliminate_x / eliminate_z,
+// FMM2DTree_find_error.hpp:1814-2372), with A = P2P[k][k]:
+//   W   = A^{-1} [L2P | E1 row blocks of k]             (eliminate x_k)
+//   G   = -P2M W ;  L2M_f = G[:, :r]                       (fill-ins of E2_k)
+//   X   = L2M_f^{-1} [-I | G[:, r:]]                       (eliminate z_k)
+//         -> M2L[k][k], P2L[k][nn] / M2L[k][nn]            (new E3_k)
+//   V   = [0 | W[:, r:]] - W[:, :r] X
+//   every neighbour equation nn:  row(nn) -= C_nn V  with C_nn = P2P[nn][k]
+//   (nn not yet eliminated) or P2L[nn][k] (nn eliminated).
+#include "factor.h"
+
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+
+namespace aifmm {
+
+static double now_s() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+static const double2 ONE = {1.0, 0.0}, MONE = {-1.0, 0.0}, ZERO = {0.0, 0.0};
+
+template <class T>
+Factor<T>::Factor(Problem& p) : pb(p), L(p.L), R(p.R) {}
+
+template <class T>
+void Factor<T>::init_level(int j) {
+  int nb = 1 << (2 * j);
+  B[j].assign(nb, FBox());
+  colors[j].assign(9, {});
+  std::vector<CvtDesc<T>> cv;
+  auto conv = [&](const Blk<double2>& s) {
+    Blk<T> d = new_blk<T>(ar, s.r, s.c);
+    if (s.size()) cv.push_back(CvtDesc<T>{s.p, d.p, (int64_t)s.size()});
+    return d;
+  };
+  for (int k = 0; k < nb; ++k) {
+    FBox& F = B[j][k];
+    const SkelBox& S = pb.sk[j][k];
+    F.m = S.m;
+    F.r = S.r;
+    int ix, iy;
+    box_xy(j, k, ix, iy);
+    F.color = (ix % 3) + 3 * (iy % 3);
+    colors[j][F.color].push_back(k);
+    F.L2P = conv(S.L2P);
+    F.P2M = conv(S.P2M);
+    for (auto& e : S.M2L) F.M2L[e.first] = conv(e.second);
+    if (j == L)
+      for (auto& e : pb.P2P[k]) F.P2P[e.first] = conv(e.second);
+  }
+  auto* d = upload(sc, cv, pb.stream);
+  launch_cvt<T>((int)cv.size(), d, pb.stream);
+  HIP_OK(hipStreamSynchronize(pb.stream));
+  sc.reset();
+}
+
+// Parent P2P blocks from the children's M2L blocks
+// (initialise_P2P_NonLeafLevel, FMM2DTree_find_error.hpp:1743-1807).
+template <class T>
+void Factor<T>::nonleaf_p2p(int j) {
+  int nb = 1 << (2 * j);
+  CopyBatch<T> cp, zero;
+  for (int k = 0; k < nb; ++k) {
+    FBox& K = B[j][k];
+    int mk = 0;
+    for (int a = 0; a < 4; ++a) mk += B[j + 1][4 * k + a].r;
+    if (mk != K.m || K.L2P.r != mk)
+      throw std::runtime_error("level " + std::to_string(j) + " box " + std::to_string(k) + ": size mismatch");
+    std::vector<int> tg;
+    for (int n = 0; n < 8; ++n)
+      if (pb.lists[j][k].nb[n] != -1) tg.push_back(pb.lists[j][k].nb[n]);
+    tg.push_back(k);
+    for (int nn : tg) {
+      Blk<T> P = new_blk<T>(ar, K.m, B[j][nn].m);
+      zero.add(P, P, 4);
+      int ri = 0;
+      for (int a = 0; a < 4; ++a) {
+        FBox& C = B[j + 1][4 * k + a];
+        int ci = 0;
+        for (int b = 0; b < 4; ++b) {
+          int cc = 4 * nn + b;
+          int bc = B[j + 1][cc].r;
+          auto it = C.M2L.find(cc);
+          if (it != C.M2L.end() && it->second.r == C.r && it->second.c == bc && C.r && bc)
+            cp.add(it->second, P.sub(ri, ci, C.r, bc));
+          ci += bc;
+        }
+        ri += C.r;
+      }
+      K.P2P[nn] = P;
+    }
+  }
+  zero.run(sc, pb.stream);
+  cp.run(sc, pb.stream);
+  HIP_OK(hipStreamSynchronize(pb.stream));
+  sc.reset();
+}
+
+template <class T>
+void Factor<T>::eliminate_color(int j, int c) {
+  auto& Bx = B[j];
+  const auto& LS = pb.lists[j];
+  const std::vector<int>& S = colors[j][c];
+  std::vector<std::array<int, 8>> woff(S.size());
+  std::vector<int> ncols(S.size());
+  CopyBatch<T> cp;
+  LUBatch<T> lu;
+  SolveBatch<T> sv;
+  GemmBatch<T> g;
+  double fl = 0;
+  // 1. W = [L2P | row blocks], LU of A, W <- A^{-1} W
+  for (size_t t = 0; t < S.size(); ++t) {
+    int k = S[t];
+    FBox& K = Bx[k];
+    int m = K.m, r = K.r;
+    int off = r;
+    for (int n = 0; n < 8; ++n) {
+      int nn = LS[k].nb[n];
+      woff[t][n] = off;
+      if (nn < 0) continue;
+      off += Bx[nn].elim ? Bx[nn].r : Bx[nn].m;
+    }
+    ncols[t] = off;
+    K.W = new_blk<T>(ar, m, off);
+    if (m == 0) continue;
+    cp.add(K.L2P, K.W.sub(0, 0, m, r));
+    for (int n = 0; n < 8; ++n) {
+      int nn = LS[k].nb[n];
+      if (nn < 0) continue;
+      int w = Bx[nn].elim ? Bx[nn].r : Bx[nn].m;
+      if (!w) continue;
+      auto& mp = Bx[nn].elim ? K.M2P : K.P2P;
+      auto it = mp.find(nn);
+      Blk<T> dst = K.W.sub(0, woff[t][n], m, w);
+      if (it != mp.end() && it->second.r == m && it->second.c == w) cp.add(it->second, dst);
+      else cp.add(dst, dst, 4);
+    }
+    K.A = K.P2P[k];
+    K.piv = ar.get<int>(m);
+    lu.add(K.A, K.piv);
+    fl += 8.0 * m * m * m / 3.0 + 8.0 * m * m * off;
+  }
+  cp.run(sc, pb.stream);
+  lu.run(sc, pb.stream);
+  for (size_t t = 0; t < S.size(); ++t) {
+    FBox& K = Bx[S[t]];
+    if (K.m) sv.add(K.A, K.piv, K.W);
+  }
+  sv.run(sc, pb.stream);
+  // 2. G = -P2M W
+  for (size_t t = 0; t < S.size(); ++t) {
+    FBox& K = Bx[S[t]];
+    K.G = new_blk<T>(ar, K.r, ncols[t]);
+    if (K.m && K.r) {
+      g.add(K.P2M, 0, K.W, 0, K.G, MONE, ZERO);
+      fl += 8.0 * K.r * K.m * ncols[t];
+    }
+  }
+  g.run(sc, pb.stream);
+  // 3. LU of L2M_f ; 4. X = L2M_f^{-1} [-I | G[:, r:]]
+  std::vector<Blk<T>> X(S.size());
+  for (size_t t = 0; t < S.size(); ++t) {
+    FBox& K = Bx[S[t]];
+    int r = K.r;
+    K.piv2 = ar.get<int>(r > 0 ? r : 1);
+    X[t] = new_blk<T>(ar, r, ncols[t]);
+    if (!r) continue;
+    lu.add(K.G.sub(0, 0, r, r), K.piv2);
+    cp.add(X[t], X[t].sub(0, 0, r, r), 3, MONE);
+    if (ncols[t] > r) cp.add(K.G.sub(0, r, r, ncols[t] - r), X[t].sub(0, r, r, ncols[t] - r));
+    fl += 8.0 * r * r * r / 3.0 + 8.0 * r * r * ncols[t];
+  }
+  lu.run(sc, pb.stream);
+  cp.run(sc, pb.stream);
+  for (size_t t = 0; t < S.size(); ++t) {
+    FBox& K = Bx[S[t]];
+    if (K.r) sv.add(K.G.sub(0, 0, K.r, K.r), K.piv2, X[t]);
+  }
+  sv.run(sc, pb.stream);
+  // 5. V = [0 | W[:, r:]] - W[:, :r] X
+  std::vector<Blk<T>> V(S.size());
+  for (size_t t = 0; t < S.size(); ++t) {
+    int k = S[t];
+    FBox& K = Bx[k];
+    int m = K.m, r = K.r, nc = ncols[t];
+    V[t] = new_blk<T>(ar, m, nc);
+    // new E3_k blocks (views into X)
+    K.M2L[k] = X[t].sub(0, 0, r, r);
+    for (int n = 0; n < 8; ++n) {
+      int nn = LS[k].nb[n];
+      if (nn < 0) continue;
+      if (Bx[nn].elim) K.M2L[nn] = X[t].sub(0, woff[t][n], r, Bx[nn].r);
+      else K.P2L[nn] = X[t].sub(0, woff[t][n], r, Bx[nn].m);
+    }
+    if (!m) continue;
+    if (r) cp.add(V[t], V[t].sub(0, 0, m, r), 4);
+    if (nc > r) cp.add(K.W.sub(0, r, m, nc - r), V[t].sub(0, r, m, nc - r));
+  }
+  cp.run(sc, pb.stream);
+  for (size_t t = 0; t < S.size(); ++t) {
+    FBox& K = Bx[S[t]];
+    if (K.m && K.r) {
+      g.add(K.W.sub(0, 0, K.m, K.r), 0, X[t], 0, V[t], MONE, ONE);
+      fl += 8.0 * K.m * K.r * ncols[t];
+    }
+  }
+  g.run(sc, pb.stream);
+  // 6. neighbour equations: target -= C_nn V[:, block]
+  CopyBatch<T> zero;
+  auto ensure = [&](std::map<int, Blk<T>>& mp, int key, int rows, int cols) -> Blk<T> {
+    auto it = mp.find(key);
+    if (it != mp.end() && it->second.r == rows && it->second.c == cols && (it->second.p || !rows || !cols))
+      return it->second;
+    if (it != mp.end() && it->second.size())
+      throw std::runtime_error("block size mismatch in neighbour update");
+    Blk<T> b = new_blk<T>(ar, rows, cols);
+    zero.add(b, b, 4);
+    mp[key] = b;
+    return b;
+  };
+  for (size_t t = 0; t < S.size(); ++t) {
+    int k = S[t];
+    FBox& K = Bx[k];
+    int m = K.m, r = K.r;
+    for (int n = 0; n < 8; ++n) {
+      int nn = LS[k].nb[n];
+      if (nn < 0) continue;
+      FBox& Nb = Bx[nn];
+      bool ne = Nb.elim;
+      int rows = ne ? Nb.r : Nb.m;
+      auto& cm = ne ? Nb.P2L : Nb.P2P;
+      Blk<T> C;
+      auto it = cm.find(k);
+      if (it != cm.end()) C = it->second;
+      if (!rows || !m || C.size() == 0) {
+        // still create the structural blocks so later steps find them
+        ensure(ne ? Nb.M2L : Nb.M2P, k, rows, r);
+        for (int p = 0; p < 8; ++p) {
+          int pp = LS[k].nb[p];
+          if (pp < 0) continue;
+          bool pe = Bx[pp].elim;
+          auto& tm = ne ? (pe ? Nb.M2L : Nb.P2L) : (pe ? Nb.M2P : Nb.P2P);
+          ensure(tm, pp, rows, pe ? Bx[pp].r : Bx[pp].m);
+        }
+        continue;
+      }
+      Blk<T> ty = ensure(ne ? Nb.M2L : Nb.M2P, k, rows, r);
+      if (r) g.add(C, 0, V[t].sub(0, 0, m, r), 0, ty, MONE, ONE), fl += 8.0 * rows * m * r;
+      for (int p = 0; p < 8; ++p) {
+        int pp = LS[k].nb[p];
+        if (pp < 0) continue;
+        bool pe = Bx[pp].elim;
+        int cols = pe ? Bx[pp].r : Bx[pp].m;
+        auto& tm = ne ? (pe ? Nb.M2L : Nb.P2L) : (pe ? Nb.M2P : Nb.P2P);
+        Blk<T> tg = ensure(tm, pp, rows, cols);
+        if (cols) g.add(C, 0, V[t].sub(0, woff[t][p], m, cols), 0, tg, MONE, ONE), fl += 8.0 * rows * m * cols;
+      }
+    }
+  }
+  zero.run(sc, pb.stream);
+  g.run(sc, pb.stream);
+  for (int k : S) Bx[k].elim = true;
+  HIP_OK(hipStreamSynchronize(pb.stream));
+  sc.reset();
+  st.flops += fl;
+}
+
+// Batched recompression of distance-2 fill-ins (the colour-schedule form of
+// compress_P2P_Two_Ways / compress_M2P_P2L, FMM2DTree_find_error.hpp:1424-1699):
+// per un-eliminated box u, row basis = CPQR([L2P_u | row fill-ins]), column
+// basis = CPQR([P2M_u^H | column fill-ins^H]); rank = min of the two
+// thresholded ranks (the reference's equalisation).
+template <class T>
+void Factor<T>::recompress(int j) {
+  using Rl = typename CplxOf<T>::real;
+  auto& Bx = B[j];
+  const auto& LS = pb.lists[j];
+  int nb = 1 << (2 * j);
+  struct Fill {
+    int u, p, kind;
+  };
+  std::vector<Fill> cand;
+  std::vector<CopyDesc<T>> nd;
+  auto blk_of = [&](const Fill& f) -> Blk<T>& {
+    FBox& U = Bx[f.u];
+    return f.kind == 0 ? U.P2P[f.p] : f.kind == 1 ? U.M2P[f.p] : U.P2L[f.p];
+  };
+  for (int u = 0; u < nb; ++u) {
+    FBox& U = Bx[u];
+    for (int s = 0; s < 16; ++s) {
+      int p = LS[u].inner[s];
+      if (p < 0) continue;
+      bool ue = U.elim, pe = Bx[p].elim;
+      std::map<int, Blk<T>>* mp = nullptr;
+      int kind = -1;
+      if (!ue && !pe) mp = &U.P2P, kind = 0;
+      else if (!ue && pe) mp = &U.M2P, kind = 1;
+      else if (ue && !pe) mp = &U.P2L, kind = 2;
+      if (!mp) continue;
+      auto it = mp->find(p);
+      if (it == mp->end()) continue;
+      if (it->second.size() == 0) {
+        mp->erase(it);
+        continue;
+      }
+      cand.push_back(Fill{u, p, kind});
+      CopyDesc<T> d{};
+      d.src = it->second.p;
+      d.nr = it->second.r;
+      d.nc = it->second.c;
+      d.lds = it->second.ld;
+      nd.push_back(d);
+    }
+  }
+  if (cand.empty()) return;
+  std::vector<double> norms(cand.size());
+  {
+    auto* dd = upload(sc, nd, pb.stream);
+    double* dn = sc.get<double>(cand.size());
+    launch_fro<T>((int)nd.size(), dd, dn, pb.stream);
+    HIP_OK(hipMemcpyAsync(norms.data(), dn, 8 * cand.size(), hipMemcpyDeviceToHost, pb.stream));
+    HIP_OK(hipStreamSynchronize(pb.stream));
+  }
+  std::vector<Fill> fills;
+  for (size_t i = 0; i < cand.size(); ++i) {
+    if (norms[i] > 1e-10) fills.push_back(cand[i]);
+    else {
+      FBox& U = Bx[cand[i].u];
+      (cand[i].kind == 0 ? U.P2P : cand[i].kind == 1 ? U.M2P : U.P2L).erase(cand[i].p);
+    }
+  }
+  if (fills.empty()) {
+    sc.reset();
+    return;
+  }
+  std::vector<char> touched(nb, 0);
+  for (auto& f : fills) {
+    if (f.kind == 0) touched[f.u] = touched[f.p] = 1;
+    else if (f.kind == 1) touched[f.u] = 1;
+    else touched[f.p] = 1;
+  }
+  std::vector<int> tl;
+  for (int u = 0; u < nb; ++u)
+    if (touched[u]) tl.push_back(u);
+  int nt = (int)tl.size();
+  st.recompressions += nt;
+  // stacked matrices
+  std::vector<Blk<T>> Arow(nt), Acol(nt);
+  CopyBatch<T> cp;
+  for (int i = 0; i < nt; ++i) {
+    int u = tl[i];
+    FBox& U = Bx[u];
+    int m = U.m;
+    int nr = U.r, ncl = U.r;
+    for (auto& f : fills) {
+      if (f.u == u && f.kind == 0) nr += Bx[f.p].m;
+      if (f.u == u && f.kind == 1) nr += Bx[f.p].r;
+      if (f.p == u && f.kind == 0) ncl += Bx[f.u].m;
+      if (f.p == u && f.kind == 2) ncl += Bx[f.u].r;
+    }
+    Arow[i] = new_blk<T>(ar, m, nr);
+    Acol[i] = new_blk<T>(ar, m, ncl);
+    if (!m) continue;
+    int oR = 0, oC = 0;
+    if (U.r) cp.add(U.L2P, Arow[i].sub(0, 0, m, U.r)), cp.add(U.P2M, Acol[i].sub(0, 0, m, U.r), 1);
+    oR = oC = U.r;
+    for (auto& f : fills) {
+      if (f.u == u && (f.kind == 0 || f.kind == 1)) {
+        Blk<T>& F = blk_of(f);
+        if (F.c) cp.add(F, Arow[i].sub(0, oR, m, F.c));
+        oR += F.c;
+      }
+      if (f.p == u && (f.kind == 0 || f.kind == 2)) {
+        Blk<T>& F = blk_of(f);
+        if (F.r) cp.add(F, Acol[i].sub(0, oC, m, F.r), 1);
+        oC += F.r;
+      }
+    }
+  }
+  cp.run(sc, pb.stream);
+  // CPQR of both stacks
+  std::vector<QRDesc<T>> qd;
+  int* dr = sc.get<int>(2 * nt);
+  std::vector<T*> taus(2 * nt);
+  for (int i = 0; i < 2 * nt; ++i) {
+    Blk<T>& A = (i % 2 == 0) ? Arow[i / 2] : Acol[i / 2];
+    QRDesc<T> d{};
+    d.A = A.p;
+    d.m = A.r;
+    d.n = A.c;
+    d.lda = A.ld;
+    int mn = std::min(A.r, A.c);
+    taus[i] = ar.get<T>(mn > 0 ? mn : 1);
+    d.tau = taus[i];
+    d.perm = sc.get<int>(A.c > 0 ? A.c : 1);
+    d.work = sc.get<Rl>(A.c > 0 ? A.c : 1);
+    d.thr = pow(10.0, -1.0 * pb.P.tol_pow);
+    d.rank_out = dr + i;
+    d.maxpiv_out = nullptr;
+    if (A.r && A.c) qd.push_back(d);
+    else HIP_OK(hipMemsetAsync(dr + i, 0, 4, pb.stream));
+    double mm = A.r, nn = A.c;
+    st.flops += mm >= nn ? 8.0 * (mm * nn * nn - nn * nn * nn / 3) : 8.0 * (nn * mm * mm - mm * mm * mm / 3);
+  }
+  {
+    auto* dq = upload(sc, qd, pb.stream);
+    launch_cpqr<T>((int)qd.size(), dq, pb.stream);
+  }
+  std::vector<int> rk(2 * nt);
+  HIP_OK(hipMemcpyAsync(rk.data(), dr, 8 * nt, hipMemcpyDeviceToHost, pb.stream));
+  HIP_OK(hipStreamSynchronize(pb.stream));
+  std::vector<int> rnew(nb, -1);
+  std::vector<Blk<T>> QR(nt), QC(nt), Rrow(nt), Rcol(nt);
+  std::vector<QFormDesc<T>> qf;
+  GemmBatch<T> g;
+  for (int i = 0; i < nt; ++i) {
+    int u = tl[i];
+    FBox& U = Bx[u];
+    int r = std::min(rk[2 * i], rk[2 * i + 1]);
+    if (!U.m) r = 0;
+    rnew[u] = r;
+    QR[i] = new_blk<T>(ar, U.m, r);
+    QC[i] = new_blk<T>(ar, U.m, r);
+    if (r) {
+      qf.push_back(QFormDesc<T>{Arow[i].p, taus[2 * i], QR[i].p, U.m, Arow[i].c, r, Arow[i].ld, QR[i].ld});
+      qf.push_back(QFormDesc<T>{Acol[i].p, taus[2 * i + 1], QC[i].p, U.m, Acol[i].c, r, Acol[i].ld, QC[i].ld});
+      st.flops += 2 * 8.0 * U.m * r * std::min(U.m, std::max(Arow[i].c, Acol[i].c));
+    }
+    Rrow[i] = new_blk<T>(ar, r, U.r);
+    Rcol[i] = new_blk<T>(ar, U.r, r);
+  }
+  {
+    auto* d = upload(sc, qf, pb.stream);
+    launch_qform<T>((int)qf.size(), d, pb.stream);
+  }
+  for (int i = 0; i < nt; ++i) {
+    FBox& U = Bx[tl[i]];
+    if (rnew[tl[i]] && U.r) {
+      g.add(QR[i], 1, U.L2P, 0, Rrow[i], ONE, ZERO);
+      g.add(U.P2M, 0, QC[i], 0, Rcol[i], ONE, ZERO);
+    }
+  }
+  g.run(sc, pb.stream);
+  std::vector<int> ti(nb, -1);
+  for (int i = 0; i < nt; ++i) ti[tl[i]] = i;
+  auto il_of = [&](int u, std::vector<int>& v) {
+    v.clear();
+    for (int s = 0; s < 16; ++s)
+      if (LS[u].inner[s] != -1) v.push_back(LS[u].inner[s]);
+    for (int s = 0; s < 24; ++s)
+      if (LS[u].outer[s] != -1) v.push_back(LS[u].outer[s]);
+  };
+  std::vector<int> il;
+  // row rotation of M2L[u][*]
+  for (int i = 0; i < nt; ++i) {
+    int u = tl[i];
+    FBox& U = Bx[u];
+    il_of(u, il);
+    for (int b : il) {
+      Blk<T>& M = U.M2L[b];
+      Blk<T> nw = new_blk<T>(ar, rnew[u], M.c);
+      if (nw.size()) g.add(Rrow[i], 0, M, 0, nw, ONE, ZERO), st.flops += 8.0 * nw.r * nw.c * U.r;
+      M = nw;
+    }
+  }
+  g.run(sc, pb.stream);
+  // column rotation of M2L[*][u]
+  for (int i = 0; i < nt; ++i) {
+    int u = tl[i];
+    FBox& U = Bx[u];
+    il_of(u, il);
+    for (int b : il) {
+      Blk<T>& M = Bx[b].M2L[u];
+      Blk<T> nw = new_blk<T>(ar, M.r, rnew[u]);
+      if (nw.size()) g.add(M, 0, Rcol[i], 0, nw, ONE, ZERO), st.flops += 8.0 * nw.r * nw.c * U.r;
+      M = nw;
+    }
+  }
+  g.run(sc, pb.stream);
+  // fill-ins projected onto M2L
+  std::vector<Blk<T>> tmp(fills.size());
+  for (size_t q = 0; q < fills.size(); ++q) {
+    auto& f = fills[q];
+    if (f.kind != 0) continue;
+    int iu = ti[f.u];
+    Blk<T>& F = blk_of(f);
+    tmp[q] = new_blk<T>(ar, rnew[f.u], F.c);
+    if (tmp[q].size()) g.add(QR[iu], 1, F, 0, tmp[q], ONE, ZERO);
+  }
+  g.run(sc, pb.stream);
+  for (size_t q = 0; q < fills.size(); ++q) {
+    auto& f = fills[q];
+    Blk<T>& M = Bx[f.u].M2L[f.p];
+    Blk<T>& F = blk_of(f);
+    if (!M.size()) continue;
+    if (f.kind == 0) g.add(tmp[q], 0, QC[ti[f.p]], 0, M, ONE, ONE);
+    if (f.kind == 1) g.add(QR[ti[f.u]], 1, F, 0, M, ONE, ONE);
+    if (f.kind == 2) g.add(F, 0, QC[ti[f.p]], 0, M, ONE, ONE);
+  }
+  g.run(sc, pb.stream);
+  // parent transfer blocks
+  if (j > 2) {
+    for (int P = 0; P < nb / 4; ++P) {
+      bool any = false;
+      for (int c = 0; c < 4; ++c) any |= touched[4 * P + c] != 0;
+      if (!any) continue;
+      FBox& Pb = B[j - 1][P];
+      int totr = 0;
+      for (int c = 0; c < 4; ++c) totr += touched[4 * P + c] ? rnew[4 * P + c] : Bx[4 * P + c].r;
+      Blk<T> NL = new_blk<T>(ar, totr, Pb.L2P.c), NM = new_blk<T>(ar, Pb.P2M.r, totr);
+      int ri = 0, ro = 0;
+      for (int c = 0; c < 4; ++c) {
+        int ch = 4 * P + c;
+        int ol = Bx[ch].r, nl = touched[ch] ? rnew[ch] : ol;
+        if (touched[ch]) {
+          int i = ti[ch];
+          if (NL.c && nl) g.add(Rrow[i], 0, Pb.L2P.sub(ri, 0, ol, Pb.L2P.c), 0, NL.sub(ro, 0, nl, NL.c), ONE, ZERO);
+          if (NM.r && nl) g.add(Pb.P2M.sub(0, ri, Pb.P2M.r, ol), 0, Rcol[i], 0, NM.sub(0, ro, NM.r, nl), ONE, ZERO);
+        } else {
+          if (NL.c && nl) cp.add(Pb.L2P.sub(ri, 0, ol, Pb.L2P.c), NL.sub(ro, 0, nl, NL.c));
+          if (NM.r && nl) cp.add(Pb.P2M.sub(0, ri, Pb.P2M.r, ol), NM.sub(0, ro, NM.r, nl));
+        }
+        ri += ol;
+        ro += nl;
+      }
+      Pb.L2P = NL;
+      Pb.P2M = NM;
+      Pb.m = totr;
+    }
+    g.run(sc, pb.stream);
+    cp.run(sc, pb.stream);
+  }
+  // new bases
+  for (int i = 0; i < nt; ++i) {
+    int u = tl[i];
+    FBox& U = Bx[u];
+    U.L2P = QR[i];
+    U.P2M = new_blk<T>(ar, rnew[u], U.m);
+    if (U.P2M.size()) cp.add(QC[i], U.P2M, 1);
+    U.r = rnew[u];
+  }
+  cp.run(sc, pb.stream);
+  for (auto& f : fills) {
+    FBox& U = Bx[f.u];
+    (f.kind == 0 ? U.P2P : f.kind == 1 ? U.M2P : U.P2L).erase(f.p);
+  }
+  HIP_OK(hipStreamSynchronize(pb.stream));
+  sc.reset();
+}
+
+// solve_particles_at_base_level (FMM2DTree_find_error.hpp:2453-2485): one
+// dense LU of the level-2 system.
+template <class T>
+void Factor<T>::base_factor() {
+  auto& Bx = B[2];
+  boff.assign(17, 0);
+  for (int k = 0; k < 16; ++k) boff[k + 1] = boff[k] + Bx[k].r;
+  int n = boff[16];
+  st.base_n = n;
+  base = new_blk<T>(ar, n, n);
+  bpiv = ar.get<int>(n > 0 ? n : 1);
+  if (!n) return;
+  CopyBatch<T> cp;
+  cp.add(base, base, 4);
+  cp.run(sc, pb.stream);
+  for (int r = 0; r < 16; ++r)
+    for (int s = 0; s < 16; ++s) {
+      auto it = Bx[r].M2L.find(s);
+      if (it == Bx[r].M2L.end()) continue;
+      if (it->second.r == Bx[r].r && it->second.c == Bx[s].r && it->second.size())
+        cp.add(it->second, base.sub(boff[r], boff[s], Bx[r].r, Bx[s].r));
+    }
+  cp.run(sc, pb.stream);
+  LUBatch<T> lu;
+  lu.add(base, bpiv);
+  lu.run(sc, pb.stream);
+  st.flops += 8.0 * n * (double)n * n / 3.0;
+  HIP_OK(hipStreamSynchronize(pb.stream));
+  sc.reset();
+}
+
+template <class T>
+void Factor<T>::factor() {
+  if (!pb.assembled) pb.assemble();
+  double t0 = now_s();
+  B.assign(L + 1, {});
+  colors.assign(L + 1, {});
+  for (int j = 2; j <= L; ++j) init_level(j);
+  for (int j = L; j >= 2; --j) {
+    if (j != L) nonleaf_p2p(j);
+    for (int c = 0; c < 9; ++c) {
+      if (colors[j][c].empty()) continue;
+      eliminate_color(j, c);
+      recompress(j);
+    }
+  }
+  base_factor();
+  HIP_OK(hipStreamSynchronize(pb.stream));
+  st.t_factor = now_s() - t0;
+  st.bytes = ar.used();
+  for (int j = 2; j <= L; ++j)
+    for (auto& F : B[j]) st.max_rank = std::max(st.max_rank, F.r);
+}
+
+// Forward sweep (the reference's fused RHS updates inside eliminate_*),
+// dense base solve, and back-substitution (back_substitution_phase,
+// FMM2DTree_find_error.hpp:2487-2532) over the stored factors.
+template <class T>
+void Factor<T>::solve(const double2* b, double2* x, int nrhs) {
+  auto& st_ = pb.stream;
+  sv.reset();
+  int64_t N = pb.N;
+  Blk<T> Bt = new_blk<T>(sv, (int)N, nrhs);
+  {
+    std::vector<CvtDesc<T>> cv{CvtDesc<T>{b, Bt.p, N * nrhs}};
+    auto* d = upload(sc, cv, st_);
+    launch_cvt<T>(1, d, st_);
+  }
+  Blk<T> Xt = new_blk<T>(sv, (int)N, nrhs);
+  std::vector<std::vector<Blk<T>>> bk(L + 1), bs(L + 1), mr(L + 1), xk(L + 1), yk(L + 1);
+  CopyBatch<T> cp;
+  SolveBatch<T> svb;
+  GemmBatch<T> g;
+  for (int j = L; j >= 2; --j) {
+    int nb = 1 << (2 * j);
+    bk[j].resize(nb);
+    bs[j].resize(nb);
+    mr[j].resize(nb);
+    xk[j].resize(nb);
+    yk[j].resize(nb);
+    for (int k = 0; k < nb; ++k) {
+      FBox& F = B[j][k];
+      if (j == L) {
+        bk[j][k] = Bt.sub(k * R, 0, R, nrhs);
+        xk[j][k] = Xt.sub(k * R, 0, R, nrhs);
+      } else {
+        bk[j][k] = new_blk<T>(sv, F.m, nrhs);
+        xk[j][k] = new_blk<T>(sv, F.m, nrhs);
+        int o = 0;
+        for (int c = 0; c < 4; ++c) {
+          int cr = B[j + 1][4 * k + c].r;
+          if (cr) cp.add(mr[j + 1][4 * k + c], bk[j][k].sub(o, 0, cr, nrhs));
+          o += cr;
+        }
+      }
+      bs[j][k] = new_blk<T>(sv, F.m, nrhs);
+      mr[j][k] = new_blk<T>(sv, F.r, nrhs);
+      yk[j][k] = new_blk<T>(sv, F.r, nrhs);
+    }
+    cp.run(sc, st_);
+    const auto& LS = pb.lists[j];
+    for (int c = 0; c < 9; ++c) {
+      const auto& S = colors[j][c];
+      for (int k : S)
+        if (B[j][k].m) cp.add(bk[j][k], bs[j][k]);
+      cp.run(sc, st_);
+      for (int k : S)
+        if (B[j][k].m) svb.add(B[j][k].A, B[j][k].piv, bk[j][k]);
+      svb.run(sc, st_);
+      for (int k : S) {
+        FBox& F = B[j][k];
+        if (F.m && F.r) g.add(F.P2M, 0, bk[j][k], 0, mr[j][k], MONE, ZERO);
+        else if (F.r) cp.add(mr[j][k], mr[j][k], 4);
+      }
+      g.run(sc, st_);
+      cp.run(sc, st_);
+      for (int k : S)
+        if (B[j][k].r) svb.add(B[j][k].G.sub(0, 0, B[j][k].r, B[j][k].r), B[j][k].piv2, mr[j][k]);
+      svb.run(sc, st_);
+      for (int k : S) {
+        FBox& F = B[j][k];
+        if (F.m && F.r) g.add(F.W.sub(0, 0, F.m, F.r), 0, mr[j][k], 0, bk[j][k], MONE, ONE);
+      }
+      g.run(sc, st_);
+      for (int k : S) {
+        FBox& F = B[j][k];
+        if (!F.m) continue;
+        for (int n = 0; n < 8; ++n) {
+          int nn = LS[k].nb[n];
+          if (nn < 0) continue;
+          FBox& Nb = B[j][nn];
+          bool before = Nb.color < c;
+          auto& cm = before ? Nb.P2L : Nb.P2P;
+          auto it = cm.find(k);
+          if (it == cm.end() || !it->second.size()) continue;
+          g.add(it->second, 0, bk[j][k], 0, before ? mr[j][nn] : bk[j][nn], MONE, ONE);
+        }
+      }
+      g.run(sc, st_);
+    }
+  }
+  // base
+  if (boff.size() == 17 && boff[16] > 0) {
+    Blk<T> yb = new_blk<T>(sv, boff[16], nrhs);
+    for (int k = 0; k < 16; ++k)
+      if (B[2][k].r) cp.add(mr[2][k], yb.sub(boff[k], 0, B[2][k].r, nrhs));
+    cp.run(sc, st_);
+    svb.add(base, bpiv, yb);
+    svb.run(sc, st_);
+    for (int k = 0; k < 16; ++k)
+      if (B[2][k].r) cp.add(yb.sub(boff[k], 0, B[2][k].r, nrhs), yk[2][k]);
+    cp.run(sc, st_);
+  }
+  // backward
+  for (int j = 2; j <= L; ++j) {
+    int nb = 1 << (2 * j);
+    const auto& LS = pb.lists[j];
+    if (j > 2)
+      for (int K = 0; K < nb / 4; ++K) {
+        int o = 0;
+        for (int c = 0; c < 4; ++c) {
+          int ch = 4 * K + c, cr = B[j][ch].r;
+          if (cr) cp.add(xk[j - 1][K].sub(o, 0, cr, nrhs), yk[j][ch]);
+          o += cr;
+        }
+      }
+    cp.run(sc, st_);
+    for (int c = 8; c >= 0; --c) {
+      const auto& S = colors[j][c];
+      for (int k : S)
+        if (B[j][k].m) cp.add(bs[j][k], xk[j][k]);
+      cp.run(sc, st_);
+      for (int k : S) {
+        FBox& F = B[j][k];
+        if (!F.m) continue;
+        for (int n = 0; n < 8; ++n) {
+          int nn = LS[k].nb[n];
+          if (nn < 0) continue;
+          bool after = B[j][nn].color > c;
+          auto& rm = after ? F.P2P : F.M2P;
+          auto it = rm.find(nn);
+          if (it == rm.end() || !it->second.size()) continue;
+          g.add(it->second, 0, after ? xk[j][nn] : yk[j][nn], 0, xk[j][k], MONE, ONE);
+        }
+      }
+      g.run(sc, st_);
+      for (int k : S)
+        if (B[j][k].m) svb.add(B[j][k].A, B[j][k].piv, xk[j][k]);
+      svb.run(sc, st_);
+      std::vector<Blk<T>> zt(S.size());
+      for (size_t t = 0; t < S.size(); ++t) {
+        FBox& F = B[j][S[t]];
+        zt[t] = new_blk<T>(sv, F.r, nrhs);
+        if (F.r) cp.add(yk[j][S[t]], zt[t]);
+      }
+      cp.run(sc, st_);
+      for (size_t t = 0; t < S.size(); ++t) {
+        FBox& F = B[j][S[t]];
+        if (F.r && F.m) g.add(F.P2M, 0, xk[j][S[t]], 0, zt[t], MONE, ONE);
+      }
+      g.run(sc, st_);
+      for (size_t t = 0; t < S.size(); ++t) {
+        FBox& F = B[j][S[t]];
+        if (F.r) svb.add(F.G.sub(0, 0, F.r, F.r), F.piv2, zt[t]);
+      }
+      svb.run(sc, st_);
+      for (size_t t = 0; t < S.size(); ++t) {
+        FBox& F = B[j][S[t]];
+        if (F.r && F.m) g.add(F.W.sub(0, 0, F.m, F.r), 0, zt[t], 0, xk[j][S[t]], MONE, ONE);
+      }
+      g.run(sc, st_);
+    }
+  }
+  // leaf x (already in Xt) -> FP64
+  if (std::is_same<T, double2>::value) {
+    HIP_OK(hipMemcpyAsync(x, Xt.p, N * nrhs * 16, hipMemcpyDeviceToDevice, st_));
+  } else {
+    launch_f2d((const float2*)Xt.p, x, N * nrhs, st_);
+  }
+  HIP_OK(hipStreamSynchronize(st_));
+  sc.reset();
+}
+
+template <class T>
+size_t Factor<T>::solve_bytes(int nrhs) const {
+  size_t e = 0;
+  for (int j = 2; j <= L; ++j)
+    for (int k = 0; k < (int)B[j].size(); ++k) {
+      const FBox& F = B[j][k];
+      size_t m = F.m, r = F.r;
+      e += 2 * (m * m + m * r + r * m + r * r);  // A, W_L, P2M, L2M_f: both sweeps
+      for (auto& x : F.P2P)
+        if (x.first != k) e += x.second.size();
+      for (auto& x : F.M2P) e += x.second.size();
+      for (auto& x : F.P2L) e += x.second.size();
+      e += 0;
+    }
+  e += base.size();
+  return e * sizeof(T) + (size_t)pb.N * nrhs * 32;
+}
+
+template class Factor<double2>;
+template class Factor<float2>;
+
+}  // namespace aifmm

@@ -1,0 +1,78 @@
+/*
+ * C ABI of the MI355X (gfx950) 2D Lippmann-Schwinger direct solver.
+ *
+ * Drop-in boundary for the reference's solve path (vaishna77/AIFMM_LS):
+ * every entry point names the reference interface it replaces.  Plain
+ * pointers and sizes only; complex arrays are interleaved (re, im) doubles,
+ * matrices column-major.  Every function returns 0 on success and a
+ * negative code on failure; aifmm_ls_last_error() then describes it.
+ */
+#ifndef AIFMM_LS_H
+#define AIFMM_LS_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct aifmm_ls_problem aifmm_ls_problem;
+
+typedef struct {
+  int n_cheb;          /* Chebyshev nodes per leaf side (argv[1]) */
+  int degree;          /* basis degree p (argv[3]) */
+  int n_levels;        /* uniform tree depth (argv[7]) */
+  int tol_pow;         /* ACA / RRQR tolerance 10^-tol_pow (argv[6]) */
+  int qchoice;         /* contrast: 0..5 as LS_functions.hpp, 6 = bump sum (argv[5]) */
+  int quantize_tables; /* 1: tables rounded to 6 significant digits like the text cache */
+  double kappa;        /* wavenumber (argv[2]) */
+  double L;            /* domain semi-length (argv[4]) */
+  const double* bumps; /* qchoice 6: n_bumps x (a, w, cx, cy) */
+  int n_bumps;
+} aifmm_ls_params;
+
+/* userkernel ctor + LS2DTree ctor (kernel.hpp:106-128, LS_Assembly.hpp:234-277):
+ * points, contrast, far/near tables by device quadrature. */
+int aifmm_ls_create(const aifmm_ls_params* p, aifmm_ls_problem** out);
+void aifmm_ls_destroy(aifmm_ls_problem* h);
+const char* aifmm_ls_last_error(void);
+
+/* userkernel::N (kernel.hpp:103) */
+int64_t aifmm_ls_size(const aifmm_ls_problem* h);
+/* userkernel::gridPoints (N x 2 interleaved) and contrast q(x_i) (LS_functions.hpp:99-112) */
+int aifmm_ls_points(aifmm_ls_problem* h, double* xy, double* q);
+/* precomputed tables: M (121 x R) and colleague operators T (9 x R x R); quad_capped
+ * counts integrals that hit the refinement depth cap */
+int aifmm_ls_tables(aifmm_ls_problem* h, double* M, double* T, int* quad_capped);
+/* userkernel::getMatrixEntry batch (kernel.hpp:130-136, LS_Assembly.hpp:474-541) */
+int aifmm_ls_entries(aifmm_ls_problem* h, int64_t n, const int64_t* I, const int64_t* J, double* out);
+/* userKernel::RHSFunction with incidence direction (cos th, sin th) (LS_functions.hpp:113-120) */
+int aifmm_ls_rhs_planewave(aifmm_ls_problem* h, double theta, double* b);
+
+/* FMM2DTree::getNodes + assemble_M2L + initialise_P2P_Leaf_Level
+ * (FMM2DTree_find_error.hpp:1137-1361, 1729-1741) */
+int aifmm_ls_assemble(aifmm_ls_problem* h);
+/* skeleton ranks per box at a level after NCA truncation */
+int aifmm_ls_ranks(aifmm_ls_problem* h, int level, int* ranks);
+/* skeleton point lists (which: 0 in_chk, 1 in_chg, 2 out_chk, 3 out_chg); *n in: capacity, out: length */
+int aifmm_ls_skeleton(aifmm_ls_problem* h, int level, int box, int which, int64_t* out, int* n);
+
+/* FMM2DTree::eliminate_phase_efficient (FMM2DTree_find_error.hpp:2407-2451), here the
+ * level-synchronous colour schedule.  precision: 0 = complex128, 1 = complex64. */
+int aifmm_ls_factor(aifmm_ls_problem* h, int precision);
+/* forward sweep + base solve + back_substitution_phase (FMM2DTree_find_error.hpp:2487-2532)
+ * for nrhs right-hand sides (N x nrhs).  stats (>= 8 doubles, optional): solve seconds,
+ * refinement steps, final relative residual, ... */
+int aifmm_ls_solve(aifmm_ls_problem* h, int nrhs, const double* b, double* x, double tol, int max_refine,
+                   double* stats);
+/* device-resident variant for timing: stage RHS, solve in HBM, fetch solution */
+int aifmm_ls_set_rhs(aifmm_ls_problem* h, int nrhs, const double* b);
+int aifmm_ls_solve_resident(aifmm_ls_problem* h, double tol, int max_refine, double* stats);
+int aifmm_ls_get_solution(aifmm_ls_problem* h, double* x);
+/* statistics: out[0..] = t_tables, t_assemble, t_factor, factor flops, factor bytes,
+ * base size, max rank, solve bytes (nrhs=1), recompressed boxes, operator bytes */
+int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -237,3 +237,16 @@ void orc_lu_solve(int n, const double* A, int nrhs, const double* B, double* X) 
 }
 
 }  // extern "C"
+
+extern "C" {
+// Interaction lists of one box (8 neighbour, 16 inner, 24 outer slots; -1 = none).
+void orc_lists(void* p, int level, int box, int* out) {
+  Handle* h = (Handle*)p;
+  Tree t;
+  t.create(&h->D, h->D.P.tol_pow);
+  const Box& B = t.T[level][box];
+  for (int s = 0; s < 8; ++s) out[s] = B.nb[s];
+  for (int s = 0; s < 16; ++s) out[8 + s] = B.inner[s];
+  for (int s = 0; s < 24; ++s) out[24 + s] = B.outer[s];
+}
+}

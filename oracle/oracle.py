@@ -56,6 +56,7 @@ def lib():
         L.orc_cpqr.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_double, ctypes.c_int, dp, dp, ip]
         L.orc_qr_solve.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp, dp]
         L.orc_lu_solve.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp, dp]
+        L.orc_lists.argtypes = [P, ctypes.c_int, ctypes.c_int, ip]
         _lib = L
     return _lib
 
@@ -139,6 +140,11 @@ class Problem:
         out = np.zeros(4 * self.N, np.int32)
         n = lib().orc_skeleton(self.h, j, k, which, out)
         return out[:n].copy()
+
+    def lists(self, level, box):
+        out = np.zeros(48, np.int32)
+        lib().orc_lists(self.h, level, box, out)
+        return out[:8].copy(), out[8:24].copy(), out[24:].copy()
 
     def aca(self, rows, cols, tol_pow):
         rows = np.ascontiguousarray(rows, np.int32)

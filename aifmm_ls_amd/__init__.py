@@ -57,6 +57,12 @@ def lib():
         L.aifmm_ls_solve_resident.argtypes = [P, ctypes.c_double, ctypes.c_int, dp]
         L.aifmm_ls_get_solution.argtypes = [P, dp]
         L.aifmm_ls_stats.argtypes = [P, dp, ctypes.c_int]
+        L.aifmm_ls_matvec.argtypes = [P, ctypes.c_int, dp, dp]
+        ip64 = ctypes.POINTER(ctypes.c_int64)
+        ip = ctypes.POINTER(ctypes.c_int)
+        L.aifmm_ls_aca.argtypes = [P, ctypes.c_int, ctypes.c_int, ip64, ctypes.c_int, ip64, ip, ip, dp, dp, ip]
+        L.aifmm_ls_selftest.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp,
+                                        ctypes.POINTER(ctypes.c_int), ctypes.c_double]
         _lib = L
     return _lib
 
@@ -181,7 +187,62 @@ class LSProblem:
         _chk(lib().aifmm_ls_get_solution(self.h, _dp(x)))
         return x.view(np.complex128).reshape(self._nrhs, self.N).T
 
+    def matvec(self, x):
+        x = np.asarray(x, np.complex128)
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        xx = np.ascontiguousarray(x.reshape(self.N, nrhs).T).view(np.float64)
+        y = np.zeros(2 * self.N * nrhs)
+        _chk(lib().aifmm_ls_matvec(self.h, nrhs, _dp(xx), _dp(y)))
+        y = y.view(np.complex128).reshape(nrhs, self.N).T
+        return y[:, 0] if x.ndim == 1 else y
+
+    def aca(self, rows, cols, tol_pow):
+        rows = np.ascontiguousarray(rows, np.int64)
+        cols = np.ascontiguousarray(cols, np.int64)
+        mn = min(rows.size, cols.size)
+        rb = np.zeros(max(mn, 1), np.int32)
+        cb = np.zeros(max(mn, 1), np.int32)
+        Ac = np.zeros(2 * rows.size * max(mn, 1))
+        Ar = np.zeros(2 * cols.size * max(mn, 1))
+        r = ctypes.c_int()
+        ip = ctypes.POINTER(ctypes.c_int)
+        _chk(lib().aifmm_ls_aca(self.h, tol_pow, rows.size, rows.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                cols.size, cols.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                rb.ctypes.data_as(ip), cb.ctypes.data_as(ip), _dp(Ac), _dp(Ar), ctypes.byref(r)))
+        r = r.value
+        Ac = Ac[: 2 * rows.size * r].view(np.complex128).reshape(r, rows.size).T
+        Ar = Ar[: 2 * cols.size * r].view(np.complex128).reshape(cols.size, r).T
+        return rb[:r].copy(), cb[:r].copy(), Ac, Ar
+
     def stats(self):
         out = np.zeros(len(STAT_NAMES))
         _chk(lib().aifmm_ls_stats(self.h, _dp(out), len(STAT_NAMES)))
         return dict(zip(STAT_NAMES, out.tolist()))
+
+
+def _cm(a):
+    return np.ascontiguousarray(np.asarray(a, np.complex128).T).view(np.float64)
+
+
+def selftest_gemm(A, B, c64=False):
+    m, k = A.shape
+    n = B.shape[1]
+    C = np.zeros(2 * m * n)
+    _chk(lib().aifmm_ls_selftest(10 * c64 + 0, m, n, k, _dp(_cm(A)), _dp(_cm(B)), _dp(C), None, 0.0))
+    return C.view(np.complex128).reshape(n, m).T
+
+
+def selftest_lu_solve(A, B, c64=False):
+    m, n = B.shape
+    C = np.zeros(2 * m * n)
+    _chk(lib().aifmm_ls_selftest(10 * c64 + 1, m, n, 0, _dp(_cm(A)), _dp(_cm(B)), _dp(C), None, 0.0))
+    return C.view(np.complex128).reshape(n, m).T
+
+
+def selftest_cpqr(A, thr, k, c64=False):
+    m, n = A.shape
+    C = np.zeros(2 * m * k)
+    r = ctypes.c_int()
+    dummy = np.zeros(2)
+    _chk(lib().aifmm_ls_selftest(10 * c64 + 2, m, n, k, _dp(_cm(A)), _dp(dummy), _dp(C), ctypes.byref(r), thr))
+    return r.value, C.view(np.complex128).reshape(k, m).T

@@ -45,6 +45,8 @@ D* upload(Arena& a, const std::vector<D>& v, hipStream_t s) {
   return d;
 }
 
+// Every descriptor of one launch must write a distinct output block: tiles of
+// different descriptors run concurrently.
 template <class T>
 struct GemmBatch {
   std::vector<GemmDesc<T>> d;

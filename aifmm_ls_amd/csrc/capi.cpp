@@ -4,9 +4,13 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <vector>
+#include <algorithm>
+#include <type_traits>
 
 #include "../../include/aifmm_ls.h"
 #include "factor.h"
+#include "h2.h"
 
 using namespace aifmm;
 
@@ -14,7 +18,10 @@ struct aifmm_ls_problem {
   std::unique_ptr<Problem> pb;
   std::unique_ptr<Factor<double2>> f64;
   std::unique_ptr<Factor<float2>> f32;
+  std::unique_ptr<H2Op> h2;
   int precision = -1;
+  double2 *tmp = nullptr, *res = nullptr, *cor = nullptr;
+  Arena sc{size_t(16) << 20};
   Arena io{size_t(64) << 20};
   double2* rhs = nullptr;
   double2* sol = nullptr;
@@ -161,6 +168,7 @@ int aifmm_ls_set_rhs(aifmm_ls_problem* h, int nrhs, const double* b) {
     if (nrhs < 1) throw std::runtime_error("nrhs must be >= 1");
     int64_t N = h->pb->N;
     h->io.reset();
+    h->tmp = h->res = h->cor = nullptr;
     h->rhs = h->io.get<double2>(N * nrhs);
     h->sol = h->io.get<double2>(N * nrhs);
     h->nrhs = nrhs;
@@ -170,19 +178,54 @@ int aifmm_ls_set_rhs(aifmm_ls_problem* h, int nrhs, const double* b) {
 
 int aifmm_ls_solve_resident(aifmm_ls_problem* h, double tol, int max_refine, double* stats) {
   GUARD({
-    (void)tol;
-    (void)max_refine;
     if (h->precision < 0) throw std::runtime_error("factor first");
     if (!h->rhs) throw std::runtime_error("set_rhs first");
+    int64_t N = h->pb->N;
+    int nr = h->nrhs;
+    auto S = [&](const double2* b, double2* x) {
+      if (h->precision == 0) h->f64->solve(b, x, nr);
+      else h->f32->solve(b, x, nr);
+    };
     double t0 = now_s();
-    if (h->precision == 0) h->f64->solve(h->rhs, h->sol, h->nrhs);
-    else h->f32->solve(h->rhs, h->sol, h->nrhs);
+    S(h->rhs, h->sol);
+    int steps = 0;
+    double relres = -1;
+    if (max_refine > 0) {
+      // iterative refinement against the FP64 hierarchical operator
+      if (!h->h2) h->h2.reset(new H2Op(*h->pb));
+      if (!h->tmp) {
+        h->tmp = h->io.get<double2>(N * nr);
+        h->res = h->io.get<double2>(N * nr);
+        h->cor = h->io.get<double2>(N * nr);
+      }
+      std::vector<double> bn(nr), rn(nr);
+      col_norms(h->sc, h->rhs, N, nr, bn.data(), h->pb->stream);
+      Blk<double2> Bb{h->rhs, (int)N, nr, (int)N}, Tt{h->tmp, (int)N, nr, (int)N}, Rr{h->res, (int)N, nr, (int)N},
+          Cc{h->cor, (int)N, nr, (int)N}, Xx{h->sol, (int)N, nr, (int)N};
+      for (int it = 0;; ++it) {
+        h->h2->apply(h->sol, h->tmp, nr);
+        CopyBatch<double2> cp;
+        cp.add(Bb, Rr);
+        cp.run(h->sc, h->pb->stream);
+        cp.add(Tt, Rr, 0, make_double2(-1, 0), 1);
+        cp.run(h->sc, h->pb->stream);
+        col_norms(h->sc, h->res, N, nr, rn.data(), h->pb->stream);
+        relres = 0;
+        for (int c = 0; c < nr; ++c) relres = std::max(relres, bn[c] > 0 ? rn[c] / bn[c] : rn[c]);
+        if (relres <= tol || it >= max_refine) break;
+        S(h->res, h->cor);
+        cp.add(Cc, Xx, 0, make_double2(1, 0), 1);
+        cp.run(h->sc, h->pb->stream);
+        ++steps;
+      }
+      h->sc.reset();
+    }
     HIP_OK(hipDeviceSynchronize());
     h->t_solve = now_s() - t0;
     if (stats) {
       stats[0] = h->t_solve;
-      stats[1] = 0;
-      stats[2] = 0;
+      stats[1] = steps;
+      stats[2] = relres;
     }
   })
 }
@@ -201,6 +244,20 @@ int aifmm_ls_solve(aifmm_ls_problem* h, int nrhs, const double* b, double* x, do
   rc = aifmm_ls_solve_resident(h, tol, max_refine, stats);
   if (rc) return rc;
   return aifmm_ls_get_solution(h, x);
+}
+
+int aifmm_ls_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y) {
+  GUARD({
+    h->pb->assemble();
+    if (!h->h2) h->h2.reset(new H2Op(*h->pb));
+    int64_t N = h->pb->N;
+    Arena a{size_t(64) << 20};
+    double2* dx = a.get<double2>(N * nrhs);
+    double2* dy = a.get<double2>(N * nrhs);
+    HIP_OK(hipMemcpy(dx, x, N * nrhs * 16, hipMemcpyHostToDevice));
+    h->h2->apply(dx, dy, nrhs);
+    HIP_OK(hipMemcpy(y, dy, N * nrhs * 16, hipMemcpyDeviceToHost));
+  })
 }
 
 int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
@@ -228,3 +285,148 @@ int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
 }
 
 }  // extern "C"
+
+// Kernel self-test hook: runs one batched dense kernel on host inputs.
+// op 0: C = A(m x k) B(k x n)          op 1: C = A(m x m)^{-1} B(m x n) (LU)
+// op 2: CPQR of A (m x n) at threshold thr -> iout[0] = rank, C = Q[:, :k]
+// op +10: the same in complex64.  Column-major interleaved complex.
+template <class T>
+static void selftest_t(int op, int m, int n, int k, const double* A, const double* B, double* C, int* iout,
+                       double thr) {
+  Arena a{size_t(64) << 20}, sc{size_t(16) << 20};
+  hipStream_t s;
+  HIP_OK(hipStreamCreate(&s));
+  auto up = [&](const double* h, int r, int c) {
+    Blk<double2> d = new_blk<double2>(a, r, c);
+    HIP_OK(hipMemcpy(d.p, h, (size_t)r * c * 16, hipMemcpyHostToDevice));
+    Blk<T> t = new_blk<T>(a, r, c);
+    std::vector<CvtDesc<T>> cv{CvtDesc<T>{d.p, t.p, (int64_t)r * c}};
+    auto* dd = upload(sc, cv, s);
+    launch_cvt<T>(1, dd, s);
+    HIP_OK(hipStreamSynchronize(s));
+    return t;
+  };
+  auto down = [&](const Blk<T>& t, double* h) {
+    Blk<double2> d = new_blk<double2>(a, t.r, t.c);
+    if (std::is_same<T, double2>::value) HIP_OK(hipMemcpy(d.p, t.p, t.size() * 16, hipMemcpyDeviceToDevice));
+    else launch_f2d((const float2*)t.p, d.p, t.size(), s);
+    HIP_OK(hipStreamSynchronize(s));
+    HIP_OK(hipMemcpy(h, d.p, t.size() * 16, hipMemcpyDeviceToHost));
+  };
+  if (op == 0) {
+    Blk<T> dA = up(A, m, k), dB = up(B, k, n), dC = new_blk<T>(a, m, n);
+    GemmBatch<T> g;
+    g.add(dA, 0, dB, 0, dC, make_double2(1, 0), make_double2(0, 0));
+    g.run(sc, s);
+    down(dC, C);
+  } else if (op == 1) {
+    Blk<T> dA = up(A, m, m), dB = up(B, m, n);
+    int* piv = a.get<int>(m);
+    LUBatch<T> lu;
+    lu.add(dA, piv);
+    lu.run(sc, s);
+    SolveBatch<T> sv;
+    sv.add(dA, piv, dB);
+    sv.run(sc, s);
+    down(dB, C);
+  } else if (op == 2) {
+    using Rl = typename CplxOf<T>::real;
+    Blk<T> dA = up(A, m, n);
+    int mn = std::min(m, n);
+    QRDesc<T> d{};
+    d.A = dA.p;
+    d.m = m;
+    d.n = n;
+    d.lda = m;
+    d.tau = a.get<T>(mn);
+    d.perm = a.get<int>(n);
+    d.work = a.get<Rl>(n);
+    d.thr = thr;
+    d.rank_out = a.get<int>(1);
+    d.maxpiv_out = nullptr;
+    std::vector<QRDesc<T>> v{d};
+    launch_cpqr<T>(1, upload(sc, v, s), s);
+    Blk<T> Q = new_blk<T>(a, m, k);
+    std::vector<QFormDesc<T>> f{QFormDesc<T>{dA.p, d.tau, Q.p, m, n, k, m, m}};
+    launch_qform<T>(1, upload(sc, f, s), s);
+    HIP_OK(hipStreamSynchronize(s));
+    HIP_OK(hipMemcpy(iout, d.rank_out, 4, hipMemcpyDeviceToHost));
+    down(Q, C);
+  }
+  HIP_OK(hipStreamSynchronize(s));
+  (void)hipStreamDestroy(s);
+}
+
+// One ACA (NCA pivot search) on explicit point lists, as getNodes_* run it.
+extern "C" int aifmm_ls_aca(aifmm_ls_problem* h, int tol_pow, int n1, const int64_t* rows, int n2,
+                            const int64_t* cols, int* rb, int* cb, double* Ac, double* Ar, int* rank) {
+  GUARD({
+    Problem& P = *h->pb;
+    Arena a{size_t(256) << 20};
+    int np = std::min(n1, n2), ns = std::max(n1, n2);
+    int64_t* dr = a.get<int64_t>(n1 > 0 ? n1 : 1);
+    int64_t* dc = a.get<int64_t>(n2 > 0 ? n2 : 1);
+    HIP_OK(hipMemcpy(dr, rows, 8 * n1, hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(dc, cols, 8 * n2, hipMemcpyHostToDevice));
+    AcaProblem p{};
+    p.rows = dr;
+    p.cols = dc;
+    p.np = np;
+    p.ns = ns;
+    p.prim_rows = n1 < n2;
+    p.rmax = np;
+    size_t rm = np > 0 ? np : 1;
+    p.U = a.get<double2>(rm * rm);
+    p.V = a.get<double2>(rm * (ns ? ns : 1));
+    p.Craw = a.get<double2>(rm * rm);
+    p.Lraw = a.get<double2>(rm * (ns ? ns : 1));
+    p.line = a.get<double2>(ns ? ns : 1);
+    p.cross = a.get<double2>(rm);
+    p.remP = a.get<char>(rm);
+    p.remS = a.get<char>(ns ? ns : 1);
+    p.pb = a.get<int>(rm);
+    p.sb = a.get<int>(rm);
+    p.rank_out = a.get<int>(1);
+    AcaProblem* dp = a.get<AcaProblem>(1);
+    HIP_OK(hipMemcpy(dp, &p, sizeof(p), hipMemcpyHostToDevice));
+    launch_aca(P.ctx, 1, dp, pow(10, -1.0 * tol_pow), 1e-12, P.stream);
+    HIP_OK(hipStreamSynchronize(P.stream));
+    int r = 0;
+    HIP_OK(hipMemcpy(&r, p.rank_out, 4, hipMemcpyDeviceToHost));
+    *rank = r;
+    std::vector<int> pbv(r), sbv(r);
+    if (r) {
+      HIP_OK(hipMemcpy(pbv.data(), p.pb, 4 * r, hipMemcpyDeviceToHost));
+      HIP_OK(hipMemcpy(sbv.data(), p.sb, 4 * r, hipMemcpyDeviceToHost));
+    }
+    for (int q = 0; q < r; ++q) {
+      rb[q] = p.prim_rows ? pbv[q] : sbv[q];
+      cb[q] = p.prim_rows ? sbv[q] : pbv[q];
+    }
+    // Ac (N1 x r): crosses if primary = rows else lines; Ar^T (N2 x r): the other
+    std::vector<double2> cr((size_t)r * np), ln((size_t)r * ns);
+    if (r) {
+      HIP_OK(hipMemcpy(cr.data(), p.Craw, cr.size() * 16, hipMemcpyDeviceToHost));
+      HIP_OK(hipMemcpy(ln.data(), p.Lraw, ln.size() * 16, hipMemcpyDeviceToHost));
+    }
+    const std::vector<double2>& acs = p.prim_rows ? cr : ln;
+    const std::vector<double2>& ars = p.prim_rows ? ln : cr;
+    if (Ac) std::memcpy(Ac, acs.data(), (size_t)r * n1 * 16);
+    if (Ar)
+      for (int q = 0; q < r; ++q)
+        for (int c = 0; c < n2; ++c) {
+          Ar[2 * (q + (size_t)r * c)] = ars[(size_t)q * n2 + c].x;
+          Ar[2 * (q + (size_t)r * c) + 1] = ars[(size_t)q * n2 + c].y;
+        }
+  })
+}
+
+extern "C" int aifmm_ls_selftest(int op, int m, int n, int k, const double* A, const double* B, double* C, int* iout,
+                      double thr) {
+  GUARD({
+    if (op >= 10) selftest_t<float2>(op - 10, m, n, k, A, B, C, iout, thr);
+    else selftest_t<double2>(op, m, n, k, A, B, C, iout, thr);
+  })
+}
+
+

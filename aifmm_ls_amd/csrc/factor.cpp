@@ -18,6 +18,8 @@
 #include <array>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <type_traits>
 
@@ -419,6 +421,12 @@ void Factor<T>::recompress(int j) {
   HIP_OK(hipMemcpyAsync(rk.data(), dr, 8 * nt, hipMemcpyDeviceToHost, pb.stream));
   HIP_OK(hipStreamSynchronize(pb.stream));
   std::vector<int> rnew(nb, -1);
+  static const bool dbg = getenv("AIFMM_DEBUG") != nullptr;
+  if (dbg) {
+    fprintf(stderr, "[gpu] level %d: %zu candidates, %zu fills, %d touched:", j, cand.size(), fills.size(), nt);
+    for (int i = 0; i < nt; ++i) fprintf(stderr, " %d(%d,%d|r%d)", tl[i], rk[2 * i], rk[2 * i + 1], Bx[tl[i]].r);
+    fprintf(stderr, "\n");
+  }
   std::vector<Blk<T>> QR(nt), QC(nt), Rrow(nt), Rcol(nt);
   std::vector<QFormDesc<T>> qf;
   GemmBatch<T> g;
@@ -729,10 +737,12 @@ void Factor<T>::solve(const double2* b, double2* x, int nrhs) {
       for (int k : S)
         if (B[j][k].m) cp.add(bs[j][k], xk[j][k]);
       cp.run(sc, st_);
-      for (int k : S) {
-        FBox& F = B[j][k];
-        if (!F.m) continue;
-        for (int n = 0; n < 8; ++n) {
+      // one launch per neighbour slot: the updates of one box accumulate into
+      // the same output block, which must not happen inside one launch
+      for (int n = 0; n < 8; ++n) {
+        for (int k : S) {
+          FBox& F = B[j][k];
+          if (!F.m) continue;
           int nn = LS[k].nb[n];
           if (nn < 0) continue;
           bool after = B[j][nn].color > c;
@@ -741,8 +751,8 @@ void Factor<T>::solve(const double2* b, double2* x, int nrhs) {
           if (it == rm.end() || !it->second.size()) continue;
           g.add(it->second, 0, after ? xk[j][nn] : yk[j][nn], 0, xk[j][k], MONE, ONE);
         }
+        g.run(sc, st_);
       }
-      g.run(sc, st_);
       for (int k : S)
         if (B[j][k].m) svb.add(B[j][k].A, B[j][k].piv, xk[j][k]);
       svb.run(sc, st_);

@@ -197,6 +197,37 @@ __device__ double block_sum(double v, double* sh) {
   return r;
 }
 
+// First allowed index whose modulus is within a relative 1e-12 of the
+// maximum modulus (maxAbsVector, ACA.hpp:27-40, with ties taken robustly:
+// mirror-symmetric geometry produces exact ties that rounding would break
+// arbitrarily).  -1 when nothing is allowed.
+constexpr double ACA_TIE = 1e-12;
+__device__ RedPair tol_argmax(const double2* v, const char* allowed, int n, RedPair* sh, double* shd) {
+  double mx = -1.0;
+  for (int t = threadIdx.x; t < n; t += blockDim.x)
+    if (!allowed || allowed[t]) mx = fmax(mx, hypot(v[t].x, v[t].y));
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+  if ((threadIdx.x & 63) == 0) shd[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  mx = shd[0];
+  for (int k = 1; k < ACA_THREADS / 64; ++k) mx = fmax(mx, shd[k]);
+  __syncthreads();
+  if (mx < 0) return RedPair{-1.0, -1};
+  double thr = mx * (1.0 - ACA_TIE);
+  int best = 0x7fffffff;
+  for (int t = threadIdx.x; t < n; t += blockDim.x)
+    if ((!allowed || allowed[t]) && hypot(v[t].x, v[t].y) >= thr) {
+      best = t;
+      break;
+    }
+  for (int o = 32; o > 0; o >>= 1) best = min(best, __shfl_xor(best, o));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6].i = best;
+  __syncthreads();
+  for (int k = 0; k < ACA_THREADS / 64; ++k) best = min(best, sh[k].i);
+  __syncthreads();
+  return RedPair{mx, best == 0x7fffffff ? -1 : best};
+}
+
 __device__ __forceinline__ double2 aca_entry(const EntryCtx& C, const AcaProblem& p, int pi, int si) {
   // primary index pi, secondary index si -> A(row, col)
   if (p.prim_rows) return ls_entry(C, p.rows[pi], p.cols[si]);
@@ -228,15 +259,9 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(EntryCtx C, const AcaProble
   // first primary index whose line has a pivot above tol_robust
   int pi = -1, si = 0;
   for (int l = 0; l < np; ++l) {
-    RedPair best{-1.0, -1};
-    for (int t = threadIdx.x; t < ns; t += blockDim.x) {
-      double2 e = aca_entry(C, p, l, t);
-      line[t] = e;
-      RedPair c{hypot2(e), t};
-      best = red_max(best, c);
-    }
+    for (int t = threadIdx.x; t < ns; t += blockDim.x) line[t] = aca_entry(C, p, l, t);
     __syncthreads();
-    RedPair r = block_argmax(best, shp);
+    RedPair r = tol_argmax(line, remS, ns, shp, shd);
     if (r.v > tol_robust) {
       pi = l;
       si = r.i;
@@ -275,10 +300,7 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(EntryCtx C, const AcaProble
   int rank = 1;
   // next primary candidate from the cross
   {
-    RedPair best{-1.0, -1};
-    for (int t = threadIdx.x; t < np; t += blockDim.x)
-      if (remP[t]) best = red_max(best, RedPair{hypot2(crossr[t]), t});
-    RedPair r = block_argmax(best, shp);
+    RedPair r = tol_argmax(crossr, remP, np, shp, shd);
     pi = r.i < 0 ? 0 : r.i;
   }
   double lastpiv = hypot2(piv);
@@ -294,10 +316,7 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(EntryCtx C, const AcaProble
       line[t] = e;
     }
     __syncthreads();
-    RedPair best{-1.0, -1};
-    for (int t = threadIdx.x; t < ns; t += blockDim.x)
-      if (remS[t]) best = red_max(best, RedPair{hypot2(line[t]), t});
-    RedPair r = block_argmax(best, shp);
+    RedPair r = tol_argmax(line, remS, ns, shp, shd);
     si = r.i < 0 ? 0 : r.i;
     piv = line[si];
     double2* Vn = p.V + (size_t)rank * ns;
@@ -361,10 +380,7 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(EntryCtx C, const AcaProble
     if (threadIdx.x == 0) s_normS = sqrt(ns2);
     lastpiv = hypot2(piv);
     __syncthreads();
-    RedPair b2{-1.0, -1};
-    for (int t = threadIdx.x; t < np; t += blockDim.x)
-      if (remP[t]) b2 = red_max(b2, RedPair{hypot2(crossr[t]), t});
-    RedPair r2 = block_argmax(b2, shp);
+    RedPair r2 = tol_argmax(crossr, remP, np, shp, shd);
     pi = r2.i < 0 ? 0 : r2.i;
   }
   if (threadIdx.x == 0) *p.rank_out = rank;

@@ -68,9 +68,26 @@ int aifmm_ls_solve(aifmm_ls_problem* h, int nrhs, const double* b, double* x, do
 int aifmm_ls_set_rhs(aifmm_ls_problem* h, int nrhs, const double* b);
 int aifmm_ls_solve_resident(aifmm_ls_problem* h, double tol, int max_refine, double* stats);
 int aifmm_ls_get_solution(aifmm_ls_problem* h, double* x);
+/* FP64 product y = A_H x with the assembled hierarchical operator (leaf P2P plus the
+ * nested L2P M2L P2M far field), the operator the factorisation inverts; N x nrhs. */
+int aifmm_ls_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y);
 /* statistics: out[0..] = t_tables, t_assemble, t_factor, factor flops, factor bytes,
  * base size, max rank, solve bytes (nrhs=1), recompressed boxes, operator bytes */
 int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n);
+
+/* LowRank::ACA_only_nodes (ACA.hpp:368-577) on explicit point lists: pivot rows rb
+ * and columns cb (rank entries each), un-updated pivot columns Ac (n1 x rank) and rows
+ * Ar (rank x n2), column-major interleaved complex. */
+int aifmm_ls_aca(aifmm_ls_problem* h, int tol_pow, int n1, const int64_t* rows, int n2, const int64_t* cols,
+                 int* rb, int* cb, double* Ac, double* Ar, int* rank);
+
+/* Kernel self-test (no reference counterpart; for the test suite): runs one
+ * batched dense kernel on host data.  op 0: C = A B (m x k, k x n); op 1:
+ * C = A^{-1} B by LU (A m x m, B m x n); op 2: column-pivoted QR of A (m x n)
+ * with relative threshold thr, iout[0] = rank, C = first k columns of Q.
+ * op + 10 runs the complex64 kernels. */
+int aifmm_ls_selftest(int op, int m, int n, int k, const double* A, const double* B, double* C, int* iout,
+                      double thr);
 
 #ifdef __cplusplus
 }

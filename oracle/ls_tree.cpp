@@ -7,6 +7,8 @@
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
+#include <string>
 
 #include "ls_oracle.hpp"
 
@@ -93,20 +95,27 @@ void Tree::assign_rhs(const cd* b) {  // FMM2DTree_find_error.hpp:1092-1103
 
 // ------------------------------------------------------------------ ACA
 // LowRank::ACA_only_nodes (ACA.hpp:368-577) with maxAbsVector (ACA.hpp:27-40).
+// maxAbsVector returns the first allowed index attaining the maximum modulus.
+// Mirror-symmetric geometry makes exact ties common, and which of two tied
+// entries wins then depends on last-bit rounding; ties are therefore taken
+// within a relative 1e-12 of the maximum, still to the lowest index, so that
+// every FP64 implementation picks the same pivot.
+static const double ACA_TIE = 1e-12;
 static void maxabs(const std::vector<cd>& v, const std::vector<char>& allowed, int& index) {
-  index = -1;
+  double mx = -1;
   for (size_t i = 0; i < allowed.size(); ++i)
-    if (allowed[i]) {
-      index = (int)i;
-      break;
-    }
-  if (index < 0) {  // empty set: libstdc++ header read yields 0
+    if (allowed[i] && std::abs(v[i]) > mx) mx = std::abs(v[i]);
+  if (mx < 0) {  // empty set: libstdc++ header read yields 0
     index = 0;
     return;
   }
-  double mx = std::abs(v[index]);
-  for (size_t i = index + 1; i < allowed.size(); ++i)
-    if (allowed[i] && std::abs(v[i]) > mx) mx = std::abs(v[i]), index = (int)i;
+  double thr = mx * (1.0 - ACA_TIE);
+  for (size_t i = 0; i < allowed.size(); ++i)
+    if (allowed[i] && std::abs(v[i]) >= thr) {
+      index = (int)i;
+      return;
+    }
+  index = 0;
 }
 
 static double vnorm(const std::vector<cd>& v) {
@@ -891,6 +900,9 @@ void Tree::colored_recompress(int j) {
   }
   std::vector<Mat> QR_(nb), QC_(nb);
   std::vector<int> rnew(nb, 0);
+  std::string dbg_line;
+  int ntouch = 0;
+  for (int u = 0; u < nb; ++u) ntouch += touched[u];
   for (int u = 0; u < nb; ++u) {
     if (!touched[u]) continue;
     Box& U = T[j][u];
@@ -908,9 +920,13 @@ void Tree::colored_recompress(int j) {
     // does (FMM2DTree_find_error.hpp:1445-1460, 1505-1512)
     int r = std::min(q1.rank(rrqr_thr), q2.rank(rrqr_thr));
     rnew[u] = r;
+    if (getenv("AIFMM_DEBUG")) dbg_line += " " + std::to_string(u) + "(" + std::to_string(q1.rank(rrqr_thr)) + "," +
+                                            std::to_string(q2.rank(rrqr_thr)) + "|r" + std::to_string(U.L2P.c) + ")";
     QR_[u] = q1.Q(r);
     QC_[u] = q2.Q(r);
   }
+  if (getenv("AIFMM_DEBUG"))
+    fprintf(stderr, "[orc] level %d: %zu fills, %d touched:%s\n", j, fills.size(), ntouch, dbg_line.c_str());
   // rotate M2L rows (row basis change) then columns (column basis change)
   std::vector<Mat> Rrow(nb), Rcol(nb);
   for (int u = 0; u < nb; ++u)

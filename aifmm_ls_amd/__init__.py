@@ -63,6 +63,8 @@ def lib():
         L.aifmm_ls_aca.argtypes = [P, ctypes.c_int, ctypes.c_int, ip64, ctypes.c_int, ip64, ip, ip, dp, dp, ip]
         L.aifmm_ls_selftest.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp,
                                         ctypes.POINTER(ctypes.c_int), ctypes.c_double]
+        L.aifmm_ls_profile_enable.argtypes = [ctypes.c_int]
+        L.aifmm_ls_profile_read.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_char_p, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -74,6 +76,24 @@ def _chk(rc):
 
 def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def profile_enable(on=True):
+    """Record HIP event pairs around every batched launch (see profile_read)."""
+    _chk(lib().aifmm_ls_profile_enable(1 if on else 0))
+
+
+def profile_read(reset=True, max_kernels=64):
+    """Per-kernel totals since the last reset: list of dicts with name,
+    launches, ms (summed event durations), bytes and flops (algorithmic)."""
+    out = np.zeros(5 * max_kernels)
+    names = ctypes.create_string_buffer(64 * max_kernels)
+    n = lib().aifmm_ls_profile_read(1 if reset else 0, max_kernels, _dp(out), names, len(names))
+    if n < 0:
+        raise AifmmError(lib().aifmm_ls_last_error().decode())
+    nm = names.value.decode().split("\n")
+    return [dict(name=nm[i], launches=int(out[5 * i]), ms=out[5 * i + 1], bytes=out[5 * i + 2],
+                 flops=out[5 * i + 3]) for i in range(n)]
 
 
 STAT_NAMES = ["t_tables", "t_assemble", "t_factor", "factor_flops", "factor_bytes", "base_n", "max_rank",

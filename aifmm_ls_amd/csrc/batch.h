@@ -1,12 +1,19 @@
 // Host-side helpers that collect descriptors for the batched kernels,
 // upload them and launch.
 #pragma once
+#include <type_traits>
 #include <vector>
 
 #include "arena.h"
 #include "kernels.h"
+#include "prof.h"
 
 namespace aifmm {
+
+// kernel names for the profiler: string literals, one per element type
+template <class T>
+constexpr bool IS_C64 = std::is_same<T, float2>::value;
+#define KNAME(T, base) (IS_C64<T> ? base "<c64>" : base "<c128>")
 
 template <class T>
 struct Blk {
@@ -86,6 +93,14 @@ struct GemmBatch {
     }
     auto* dd = upload(scratch, d, s);
     auto* dt = upload(scratch, tiles, s);
+    double by = 0, fl = 0;
+    if (prof_on())
+      for (auto& g : d) {
+        bool bz = g.beta.x == 0.0 && g.beta.y == 0.0;
+        by += sizeof(T) * ((double)g.m * g.k + (double)g.k * g.n + (double)g.m * g.n * (bz ? 1 : 2));
+        fl += 8.0 * g.m * g.n * g.k;
+      }
+    ProfScope ps(s, KNAME(T, "k_gemm"), by, fl);
     launch_gemm<T>((int)tiles.size(), dt, dd, s);
     d.clear();
   }
@@ -112,6 +127,10 @@ struct CopyBatch {
   void run(Arena& scratch, hipStream_t s) {
     if (d.empty()) return;
     auto* dd = upload(scratch, d, s);
+    double by = 0;
+    if (prof_on())
+      for (auto& c : d) by += sizeof(T) * (double)c.nr * c.nc * ((c.trans >= 3 ? 1 : 2) + c.accumulate);
+    ProfScope ps(s, KNAME(T, "k_copy"), by, 0);
     launch_copy<T>((int)d.size(), dd, s);
     d.clear();
   }
@@ -127,6 +146,10 @@ struct LUBatch {
   void run(Arena& scratch, hipStream_t s) {
     if (d.empty()) return;
     auto* dd = upload(scratch, d, s);
+    double by = 0, fl = 0;
+    if (prof_on())
+      for (auto& u : d) by += sizeof(T) * 2.0 * u.n * u.n, fl += 8.0 * u.n * u.n * u.n / 3;
+    ProfScope ps(s, KNAME(T, "k_lu"), by, fl);
     launch_lu<T>((int)d.size(), dd, s);
     d.clear();
   }
@@ -145,6 +168,10 @@ struct SolveBatch {
   void run(Arena& scratch, hipStream_t s) {
     if (d.empty()) return;
     auto* dd = upload(scratch, d, s);
+    double by = 0, fl = 0;
+    if (prof_on())
+      for (auto& u : d) by += sizeof(T) * ((double)u.n * u.n + 2.0 * u.n * u.nrhs), fl += 8.0 * u.n * u.n * u.nrhs;
+    ProfScope ps(s, KNAME(T, "k_lu_solve"), by, fl);
     launch_lu_solve<T>((int)d.size(), maxc, dd, s);
     d.clear();
     maxc = 0;

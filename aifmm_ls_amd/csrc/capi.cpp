@@ -10,6 +10,7 @@
 
 #include "../../include/aifmm_ls.h"
 #include "factor.h"
+#include "prof.h"
 #include "h2.h"
 
 using namespace aifmm;
@@ -430,3 +431,34 @@ extern "C" int aifmm_ls_selftest(int op, int m, int n, int k, const double* A, c
 }
 
 
+
+extern "C" int aifmm_ls_profile_enable(int on) {
+  GUARD({ prof_enable(on != 0); })
+}
+
+extern "C" int aifmm_ls_profile_read(int reset, int max, double* out, char* names, int names_len) {
+  try {
+    std::vector<KernelProfile> ks = prof_collect(reset != 0);
+    std::string nm;
+    int n = 0;
+    for (auto& k : ks) {
+      if (n >= max) break;
+      out[5 * n] = (double)k.launches;
+      out[5 * n + 1] = k.ms;
+      out[5 * n + 2] = k.bytes;
+      out[5 * n + 3] = k.flops;
+      out[5 * n + 4] = 0;
+      nm += k.name + "\n";
+      ++n;
+    }
+    if (names && names_len > 0) {
+      size_t c = std::min(nm.size(), (size_t)names_len - 1);
+      std::memcpy(names, nm.data(), c);
+      names[c] = 0;
+    }
+    return n;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
+}

@@ -29,13 +29,20 @@ template <>
 __device__ __forceinline__ float2 from2<float2>(double2 a) {
   return mkf((float)a.x, (float)a.y);
 }
+// a / b by Smith's algorithm (no |b|^2, so no underflow for small b)
 template <class T>
 __device__ __forceinline__ T cdiv(T a, T b) {
   using R = typename CplxOf<T>::real;
-  R d = b.x * b.x + b.y * b.y;
   T r;
-  r.x = (a.x * b.x + a.y * b.y) / d;
-  r.y = (a.y * b.x - a.x * b.y) / d;
+  if (fabs(b.x) >= fabs(b.y)) {
+    R q = b.y / b.x, den = b.x + b.y * q;
+    r.x = (a.x + a.y * q) / den;
+    r.y = (a.y - a.x * q) / den;
+  } else {
+    R q = b.x / b.y, den = b.y + b.x * q;
+    r.x = (a.x * q + a.y) / den;
+    r.y = (a.y * q - a.x) / den;
+  }
   return r;
 }
 template <class T>
@@ -245,33 +252,36 @@ __global__ __launch_bounds__(NT) void k_lu_solve(const SolveDesc<T>* __restrict_
 // t = conj((beta - c0)/beta), beta = -sign(Re c0) ||x||.
 template <class T>
 __global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
+  // norms, reflector scalars and the rank test in double for both element
+  // types: squared moduli of complex64 entries underflow below ~1e-19
   using R = typename CplxOf<T>::real;
   const QRDesc<T> d = ds[blockIdx.x];
-  __shared__ R shv[4];
+  __shared__ double shv[4];
   __shared__ int shi[4];
-  __shared__ R shs[4];
+  __shared__ double shs[4];
   const int m = d.m, n = d.n, ld = d.lda;
   T* A = d.A;
   R* nrm = d.work;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int j = threadIdx.x; j < n; j += NT) d.perm[j] = j;
   for (int j = w; j < n; j += NT / 64) {
-    R s = 0;
-    for (int i = lane; i < m; i += 64) s += cabs2(A[i + (size_t)ld * j]);
+    double s = 0;
+    for (int i = lane; i < m; i += 64) s += (double)A[i + (size_t)ld * j].x * A[i + (size_t)ld * j].x +
+                                          (double)A[i + (size_t)ld * j].y * A[i + (size_t)ld * j].y;
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) nrm[j] = sqrt(s);
+    if (lane == 0) nrm[j] = (R)sqrt(s);
   }
   __syncthreads();
   int size = m < n ? m : n;
-  R maxpiv = 0;
+  double maxpiv = 0;
   for (int k = 0; k < size; ++k) {
-    R bv = -1;
+    double bv = -1;
     int bi = -1;
     for (int j = k + threadIdx.x; j < n; j += NT) {
-      R v = nrm[j];
+      double v = nrm[j];
       if (bi < 0 || v > bv) bv = v, bi = j;
     }
-    R pv;
+    double pv;
     int big;
     block_argmax_first(bv, bi, shv, shi, pv, big);
     if (big < 0) big = k;
@@ -291,36 +301,40 @@ __global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
       }
     }
     __syncthreads();
-    R tl = 0;
-    for (int i = k + 1 + threadIdx.x; i < m; i += NT) tl += cabs2(A[i + (size_t)ld * k]);
+    double tl = 0;
+    for (int i = k + 1 + threadIdx.x; i < m; i += NT)
+      tl += (double)A[i + (size_t)ld * k].x * A[i + (size_t)ld * k].x +
+            (double)A[i + (size_t)ld * k].y * A[i + (size_t)ld * k].y;
     tl = block_sumR(tl, shs);
-    T c0 = A[k + (size_t)ld * k];
-    R beta;
-    T t;
-    const R tiny = (R)1e-300 > 0 ? (R)2.2250738585072014e-308 : (R)0;
-    (void)tiny;
-    if (tl == 0 && c0.y == 0) {
-      t = czero<T>();
-      beta = c0.x;
+    double c0x = A[k + (size_t)ld * k].x, c0y = A[k + (size_t)ld * k].y;
+    double beta = sqrt(c0x * c0x + c0y * c0y + tl), tx, ty;
+    if ((tl == 0 && c0y == 0) || beta == 0) {
+      // nothing to annihilate (or a column below the double range)
+      tx = ty = 0;
+      beta = c0x;
     } else {
-      beta = sqrt(c0.x * c0.x + c0.y * c0.y + tl);
-      if (c0.x >= 0) beta = -beta;
-      T den = c0;
-      den.x -= beta;
-      for (int i = k + 1 + threadIdx.x; i < m; i += NT) A[i + (size_t)ld * k] = cdiv(A[i + (size_t)ld * k], den);
-      T bm = c0;
-      bm.x = beta - c0.x;
-      bm.y = -c0.y;
-      t = bm;
-      t.x /= beta;
-      t.y /= beta;
-      t = cconj(t);
+      if (c0x >= 0) beta = -beta;
+      double2 den = mk(c0x - beta, c0y);
+      for (int i = k + 1 + threadIdx.x; i < m; i += NT) {
+        T a = A[i + (size_t)ld * k];
+        double2 q = cdiv(mk((double)a.x, (double)a.y), den);
+        T o;
+        o.x = (R)q.x;
+        o.y = (R)q.y;
+        A[i + (size_t)ld * k] = o;
+      }
+      // t = conj((beta - c0) / beta)
+      tx = (beta - c0x) / beta;
+      ty = c0y / beta;
     }
+    T t;
+    t.x = (R)tx;
+    t.y = (R)ty;
     __syncthreads();
     if (threadIdx.x == 0) {
       d.tau[k] = t;
       T bb = czero<T>();
-      bb.x = beta;
+      bb.x = (R)beta;
       A[k + (size_t)ld * k] = bb;
     }
     if (fabs(beta) > maxpiv) maxpiv = fabs(beta);
@@ -338,24 +352,24 @@ __global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
       s = cadd(s, col[k]);
       s = cmul(t, s);
       if (lane == 0) col[k] = csub(col[k], s);
-      R nn = 0;
+      double nn = 0;
       for (int i = k + 1 + lane; i < m; i += 64) {
         T x = cfms(s, v[i], col[i]);
         col[i] = x;
-        nn += cabs2(x);
+        nn += (double)x.x * x.x + (double)x.y * x.y;
       }
       for (int o = 32; o > 0; o >>= 1) nn += __shfl_xor(nn, o);
-      if (lane == 0) nrm[j] = sqrt(nn);
+      if (lane == 0) nrm[j] = (R)sqrt(nn);
     }
     __syncthreads();
   }
   if (threadIdx.x == 0) {
     int r = 0;
-    R pre = maxpiv * (R)d.thr;
+    double pre = maxpiv * d.thr;
     for (int i = 0; i < size; ++i)
-      if (cabsT(A[i + (size_t)ld * i]) > pre) ++r;
+      if (hypot((double)A[i + (size_t)ld * i].x, (double)A[i + (size_t)ld * i].y) > pre) ++r;
     *d.rank_out = r;
-    if (d.maxpiv_out) *d.maxpiv_out = (double)maxpiv;
+    if (d.maxpiv_out) *d.maxpiv_out = maxpiv;
   }
 }
 

@@ -1,0 +1,83 @@
+#include "prof.h"
+
+#include <map>
+#include <mutex>
+
+namespace aifmm {
+
+namespace {
+struct Pending {
+  const char* name;
+  double bytes, flops;
+  hipEvent_t e0, e1;
+};
+std::mutex mu;
+bool enabled = false;
+std::vector<Pending> pending;
+std::vector<hipEvent_t> pool;
+std::map<std::string, KernelProfile> totals;
+
+hipEvent_t take() {
+  if (!pool.empty()) {
+    hipEvent_t e = pool.back();
+    pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  HIP_OK(hipEventCreate(&e));
+  return e;
+}
+
+void fold() {
+  for (auto& p : pending) {
+    HIP_OK(hipEventSynchronize(p.e1));
+    float ms = 0;
+    HIP_OK(hipEventElapsedTime(&ms, p.e0, p.e1));
+    KernelProfile& k = totals[p.name];
+    k.name = p.name;
+    k.launches += 1;
+    k.ms += ms;
+    k.bytes += p.bytes;
+    k.flops += p.flops;
+    pool.push_back(p.e0);
+    pool.push_back(p.e1);
+  }
+  pending.clear();
+}
+}  // namespace
+
+void prof_enable(bool on) {
+  std::lock_guard<std::mutex> g(mu);
+  enabled = on;
+}
+
+bool prof_on() { return enabled; }
+
+std::vector<KernelProfile> prof_collect(bool reset) {
+  std::lock_guard<std::mutex> g(mu);
+  fold();
+  std::vector<KernelProfile> out;
+  for (auto& t : totals) out.push_back(t.second);
+  if (reset) totals.clear();
+  return out;
+}
+
+ProfScope::ProfScope(hipStream_t s, const char* name, double bytes, double flops)
+    : s_(s), name_(name), bytes_(bytes), flops_(flops) {
+  if (!enabled) return;
+  std::lock_guard<std::mutex> g(mu);
+  e0_ = take();
+  HIP_OK(hipEventRecord(e0_, s_));
+}
+
+ProfScope::~ProfScope() {
+  if (!e0_) return;
+  std::lock_guard<std::mutex> g(mu);
+  hipEvent_t e1 = take();
+  if (hipEventRecord(e1, s_) != hipSuccess) return;
+  pending.push_back(Pending{name_, bytes_, flops_, e0_, e1});
+  // bound the number of outstanding events
+  if (pending.size() > 8192) fold();
+}
+
+}  // namespace aifmm

@@ -89,6 +89,15 @@ int aifmm_ls_aca(aifmm_ls_problem* h, int tol_pow, int n1, const int64_t* rows, 
 int aifmm_ls_selftest(int op, int m, int n, int k, const double* A, const double* B, double* C, int* iout,
                       double thr);
 
+/* Kernel timing (no reference counterpart; for benchmarks): when enabled, every
+ * batched launch records a HIP event pair on its stream together with its
+ * algorithmic bytes and flops.  profile_read fills, for up to max kernels,
+ * out[5*i ..] = launches, summed milliseconds, bytes, flops, 0, and writes the
+ * kernel names '\n'-separated into names (capacity names_len); returns the number
+ * of kernels, or a negative code.  reset != 0 clears the totals afterwards. */
+int aifmm_ls_profile_enable(int on);
+int aifmm_ls_profile_read(int reset, int max, double* out, char* names, int names_len);
+
 #ifdef __cplusplus
 }
 #endif

@@ -259,6 +259,15 @@ def selftest_lu_solve(A, B, c64=False):
     return C.view(np.complex128).reshape(n, m).T
 
 
+def selftest_qr_solve(A, B, c64=False):
+    """A^{-1} B by the batched column-pivoted QR kernels; returns (X, nonzero pivots)."""
+    m, n = B.shape
+    C = np.zeros(2 * m * n)
+    nz = (ctypes.c_int * 1)()
+    _chk(lib().aifmm_ls_selftest(10 * c64 + 3, m, n, 0, _dp(_cm(A)), _dp(_cm(B)), _dp(C), nz, 0.0))
+    return C.view(np.complex128).reshape(n, m).T, nz[0]
+
+
 def selftest_cpqr(A, thr, k, c64=False):
     m, n = A.shape
     C = np.zeros(2 * m * k)

@@ -178,4 +178,94 @@ struct SolveBatch {
   }
 };
 
+// A square matrix factored in place by column-pivoted QR: the Householder
+// vectors and R overwrite the block; tau, perm and the nonzero-pivot count live
+// beside it.
+template <class T>
+struct QRFact {
+  Blk<T> A;
+  T* tau = nullptr;
+  int* perm = nullptr;
+  int* nz = nullptr;
+  int* rank = nullptr;
+};
+
+template <class T>
+QRFact<T> new_qr(Arena& a, const Blk<T>& A) {
+  QRFact<T> f;
+  f.A = A;
+  int n = A.c > 0 ? A.c : 1;
+  f.tau = a.get<T>(n);
+  f.perm = a.get<int>(n);
+  f.nz = a.get<int>(2);
+  f.rank = f.nz + 1;
+  return f;
+}
+
+template <class T>
+struct QRBatch {
+  std::vector<QRDesc<T>> d;
+  void add(const QRFact<T>& f, Arena& scratch) {
+    if (f.A.r == 0 || f.A.c == 0) return;
+    QRDesc<T> q{};
+    q.A = f.A.p;
+    q.tau = f.tau;
+    q.perm = f.perm;
+    q.work = scratch.get<typename CplxOf<T>::real>(2 * (size_t)f.A.c);
+    q.m = f.A.r;
+    q.n = f.A.c;
+    q.lda = f.A.ld;
+    q.thr = 0;
+    q.rank_out = f.rank;
+    q.maxpiv_out = nullptr;
+    q.nz_out = f.nz;
+    d.push_back(q);
+  }
+  void run(Arena& scratch, hipStream_t s) {
+    if (d.empty()) return;
+    auto* dd = upload(scratch, d, s);
+    double fl = 0;
+    if (prof_on())
+      for (auto& q : d) fl += 8.0 * ((double)q.m * q.n * q.n - (double)q.n * q.n * q.n / 3);
+    ProfScope ps(s, KNAME(T, "k_cpqr"), 0, fl);
+    launch_cpqr<T>((int)d.size(), dd, s);
+    d.clear();
+  }
+};
+
+// B <- A^{-1} B with a QRFact; B is overwritten (through a scratch copy).
+template <class T>
+struct QRSolveBatch {
+  std::vector<QSolveDesc<T>> d;
+  int maxc = 0;
+  void add(const QRFact<T>& f, const Blk<T>& B, Arena& scratch) {
+    if (f.A.r == 0 || B.c == 0) return;
+    QSolveDesc<T> q{};
+    q.A = f.A.p;
+    q.tau = f.tau;
+    q.perm = f.perm;
+    q.nz = f.nz;
+    q.B = B.p;
+    q.scratch = scratch.get<T>((size_t)f.A.r * B.c);
+    q.n = f.A.r;
+    q.nrhs = B.c;
+    q.lda = f.A.ld;
+    q.ldb = B.ld;
+    d.push_back(q);
+    int ch = (B.c + 15) / 16;
+    if (ch > maxc) maxc = ch;
+  }
+  void run(Arena& scratch, hipStream_t s) {
+    if (d.empty()) return;
+    auto* dd = upload(scratch, d, s);
+    double by = 0, fl = 0;
+    if (prof_on())
+      for (auto& q : d) by += sizeof(T) * ((double)q.n * q.n + 2.0 * q.n * q.nrhs), fl += 16.0 * q.n * q.n * q.nrhs;
+    ProfScope ps(s, KNAME(T, "k_qr_solve"), by, fl);
+    launch_qr_solve<T>((int)d.size(), maxc, dd, s);
+    d.clear();
+    maxc = 0;
+  }
+};
+
 }  // namespace aifmm

@@ -22,6 +22,8 @@ struct aifmm_ls_problem {
   std::unique_ptr<H2Op> h2;
   int precision = -1;
   double2 *tmp = nullptr, *res = nullptr, *cor = nullptr;
+  double* part = nullptr;       // per-block partial squared norms
+  std::vector<double> bnorm;    // column norms of the staged RHS
   Arena sc{size_t(16) << 20};
   Arena io{size_t(64) << 20};
   double2* rhs = nullptr;
@@ -170,6 +172,7 @@ int aifmm_ls_set_rhs(aifmm_ls_problem* h, int nrhs, const double* b) {
     int64_t N = h->pb->N;
     h->io.reset();
     h->tmp = h->res = h->cor = nullptr;
+    h->part = nullptr;
     h->rhs = h->io.get<double2>(N * nrhs);
     h->sol = h->io.get<double2>(N * nrhs);
     h->nrhs = nrhs;
@@ -181,11 +184,26 @@ int aifmm_ls_solve_resident(aifmm_ls_problem* h, double tol, int max_refine, dou
   GUARD({
     if (h->precision < 0) throw std::runtime_error("factor first");
     if (!h->rhs) throw std::runtime_error("set_rhs first");
-    int64_t N = h->pb->N;
-    int nr = h->nrhs;
+    const int64_t N = h->pb->N;
+    const int nr = h->nrhs;
+    hipStream_t st = h->pb->stream;
     auto S = [&](const double2* b, double2* x) {
       if (h->precision == 0) h->f64->solve(b, x, nr);
       else h->f32->solve(b, x, nr);
+    };
+    const int nblk = resid_blocks(N);
+    std::vector<double> part((size_t)nblk * nr);
+    // column norms of r = b - ax from the per-block partial sums (fixed order)
+    auto norms = [&](const double2* ax, double2* r, std::vector<double>& out) {
+      launch_resid(h->rhs, ax, r, N, nr, h->part, st);
+      HIP_OK(hipMemcpyAsync(part.data(), h->part, part.size() * 8, hipMemcpyDeviceToHost, st));
+      HIP_OK(hipStreamSynchronize(st));
+      out.assign(nr, 0);
+      for (int c = 0; c < nr; ++c) {
+        double s = 0;
+        for (int b = 0; b < nblk; ++b) s += part[(size_t)c * nblk + b];
+        out[c] = std::sqrt(s);
+      }
     };
     double t0 = now_s();
     S(h->rhs, h->sol);
@@ -198,30 +216,22 @@ int aifmm_ls_solve_resident(aifmm_ls_problem* h, double tol, int max_refine, dou
         h->tmp = h->io.get<double2>(N * nr);
         h->res = h->io.get<double2>(N * nr);
         h->cor = h->io.get<double2>(N * nr);
+        h->part = h->io.get<double>((size_t)nblk * nr);
+        norms(nullptr, nullptr, h->bnorm);
       }
-      std::vector<double> bn(nr), rn(nr);
-      col_norms(h->sc, h->rhs, N, nr, bn.data(), h->pb->stream);
-      Blk<double2> Bb{h->rhs, (int)N, nr, (int)N}, Tt{h->tmp, (int)N, nr, (int)N}, Rr{h->res, (int)N, nr, (int)N},
-          Cc{h->cor, (int)N, nr, (int)N}, Xx{h->sol, (int)N, nr, (int)N};
+      std::vector<double> rn;
       for (int it = 0;; ++it) {
         h->h2->apply(h->sol, h->tmp, nr);
-        CopyBatch<double2> cp;
-        cp.add(Bb, Rr);
-        cp.run(h->sc, h->pb->stream);
-        cp.add(Tt, Rr, 0, make_double2(-1, 0), 1);
-        cp.run(h->sc, h->pb->stream);
-        col_norms(h->sc, h->res, N, nr, rn.data(), h->pb->stream);
+        norms(h->tmp, h->res, rn);
         relres = 0;
-        for (int c = 0; c < nr; ++c) relres = std::max(relres, bn[c] > 0 ? rn[c] / bn[c] : rn[c]);
+        for (int c = 0; c < nr; ++c) relres = std::max(relres, h->bnorm[c] > 0 ? rn[c] / h->bnorm[c] : rn[c]);
         if (relres <= tol || it >= max_refine) break;
         S(h->res, h->cor);
-        cp.add(Cc, Xx, 0, make_double2(1, 0), 1);
-        cp.run(h->sc, h->pb->stream);
+        launch_axpy(h->sol, h->cor, N * nr, st);
         ++steps;
       }
-      h->sc.reset();
     }
-    HIP_OK(hipDeviceSynchronize());
+    HIP_OK(hipStreamSynchronize(st));
     h->t_solve = now_s() - t0;
     if (stats) {
       stats[0] = h->t_solve;
@@ -257,6 +267,7 @@ int aifmm_ls_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y) {
     double2* dy = a.get<double2>(N * nrhs);
     HIP_OK(hipMemcpy(dx, x, N * nrhs * 16, hipMemcpyHostToDevice));
     h->h2->apply(dx, dy, nrhs);
+    HIP_OK(hipStreamSynchronize(h->pb->stream));
     HIP_OK(hipMemcpy(y, dy, N * nrhs * 16, hipMemcpyDeviceToHost));
   })
 }
@@ -290,6 +301,8 @@ int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
 // Kernel self-test hook: runs one batched dense kernel on host inputs.
 // op 0: C = A(m x k) B(k x n)          op 1: C = A(m x m)^{-1} B(m x n) (LU)
 // op 2: CPQR of A (m x n) at threshold thr -> iout[0] = rank, C = Q[:, :k]
+// op 3: C = A(m x m)^{-1} B(m x n) by CPQR + ColPivHouseholderQR::solve,
+//       iout[0] = number of nonzero pivots
 // op +10: the same in complex64.  Column-major interleaved complex.
 template <class T>
 static void selftest_t(int op, int m, int n, int k, const double* A, const double* B, double* C, int* iout,
@@ -341,7 +354,7 @@ static void selftest_t(int op, int m, int n, int k, const double* A, const doubl
     d.lda = m;
     d.tau = a.get<T>(mn);
     d.perm = a.get<int>(n);
-    d.work = a.get<Rl>(n);
+    d.work = a.get<Rl>(2 * (size_t)n);
     d.thr = thr;
     d.rank_out = a.get<int>(1);
     d.maxpiv_out = nullptr;
@@ -353,6 +366,18 @@ static void selftest_t(int op, int m, int n, int k, const double* A, const doubl
     HIP_OK(hipStreamSynchronize(s));
     HIP_OK(hipMemcpy(iout, d.rank_out, 4, hipMemcpyDeviceToHost));
     down(Q, C);
+  } else if (op == 3) {
+    Blk<T> dA = up(A, m, m), dB = up(B, m, n);
+    QRFact<T> f = new_qr(a, dA);
+    QRBatch<T> qb;
+    qb.add(f, sc);
+    qb.run(sc, s);
+    QRSolveBatch<T> qs;
+    qs.add(f, dB, sc);
+    qs.run(sc, s);
+    HIP_OK(hipStreamSynchronize(s));
+    HIP_OK(hipMemcpy(iout, f.nz, 4, hipMemcpyDeviceToHost));
+    down(dB, C);
   }
   HIP_OK(hipStreamSynchronize(s));
   (void)hipStreamDestroy(s);

@@ -115,8 +115,7 @@ void Factor<T>::eliminate_color(int j, int c) {
   std::vector<std::array<int, 8>> woff(S.size());
   std::vector<int> ncols(S.size());
   CopyBatch<T> cp;
-  LUBatch<T> lu;
-  SolveBatch<T> sv;
+  DenseOps<T> dn;
   GemmBatch<T> g;
   double fl = 0;
   // 1. W = [L2P | row blocks], LU of A, W <- A^{-1} W
@@ -147,17 +146,16 @@ void Factor<T>::eliminate_color(int j, int c) {
       else cp.add(dst, dst, 4);
     }
     K.A = K.P2P[k];
-    K.piv = ar.get<int>(m);
-    lu.add(K.A, K.piv);
-    fl += 8.0 * m * m * m / 3.0 + 8.0 * m * m * off;
+    dn.factor(ar, sc, K.fa, K.A);
+    fl += 16.0 * m * m * m / 3.0 + 16.0 * m * m * off;
   }
   cp.run(sc, pb.stream);
-  lu.run(sc, pb.stream);
+  dn.run_factor(sc, pb.stream);
   for (size_t t = 0; t < S.size(); ++t) {
     FBox& K = Bx[S[t]];
-    if (K.m) sv.add(K.A, K.piv, K.W);
+    if (K.m) dn.solve(sc, K.fa, K.W);
   }
-  sv.run(sc, pb.stream);
+  dn.run_solve(sc, pb.stream);
   // 2. G = -P2M W
   for (size_t t = 0; t < S.size(); ++t) {
     FBox& K = Bx[S[t]];
@@ -173,21 +171,20 @@ void Factor<T>::eliminate_color(int j, int c) {
   for (size_t t = 0; t < S.size(); ++t) {
     FBox& K = Bx[S[t]];
     int r = K.r;
-    K.piv2 = ar.get<int>(r > 0 ? r : 1);
     X[t] = new_blk<T>(ar, r, ncols[t]);
     if (!r) continue;
-    lu.add(K.G.sub(0, 0, r, r), K.piv2);
+    dn.factor(ar, sc, K.fg, K.G.sub(0, 0, r, r));
     cp.add(X[t], X[t].sub(0, 0, r, r), 3, MONE);
     if (ncols[t] > r) cp.add(K.G.sub(0, r, r, ncols[t] - r), X[t].sub(0, r, r, ncols[t] - r));
-    fl += 8.0 * r * r * r / 3.0 + 8.0 * r * r * ncols[t];
+    fl += 16.0 * r * r * r / 3.0 + 16.0 * r * r * ncols[t];
   }
-  lu.run(sc, pb.stream);
+  dn.run_factor(sc, pb.stream);
   cp.run(sc, pb.stream);
   for (size_t t = 0; t < S.size(); ++t) {
     FBox& K = Bx[S[t]];
-    if (K.r) sv.add(K.G.sub(0, 0, K.r, K.r), K.piv2, X[t]);
+    if (K.r) dn.solve(sc, K.fg, X[t]);
   }
-  sv.run(sc, pb.stream);
+  dn.run_solve(sc, pb.stream);
   // 5. V = [0 | W[:, r:]] - W[:, :r] X
   std::vector<Blk<T>> V(S.size());
   for (size_t t = 0; t < S.size(); ++t) {
@@ -404,7 +401,7 @@ void Factor<T>::recompress(int j) {
     taus[i] = ar.get<T>(mn > 0 ? mn : 1);
     d.tau = taus[i];
     d.perm = sc.get<int>(A.c > 0 ? A.c : 1);
-    d.work = sc.get<Rl>(A.c > 0 ? A.c : 1);
+    d.work = sc.get<Rl>(A.c > 0 ? 2 * A.c : 2);
     d.thr = pow(10.0, -1.0 * pb.P.tol_pow);
     d.rank_out = dr + i;
     d.maxpiv_out = nullptr;
@@ -575,7 +572,7 @@ void Factor<T>::base_factor() {
   int n = boff[16];
   st.base_n = n;
   base = new_blk<T>(ar, n, n);
-  bpiv = ar.get<int>(n > 0 ? n : 1);
+
   if (!n) return;
   CopyBatch<T> cp;
   cp.add(base, base, 4);
@@ -588,10 +585,10 @@ void Factor<T>::base_factor() {
         cp.add(it->second, base.sub(boff[r], boff[s], Bx[r].r, Bx[s].r));
     }
   cp.run(sc, pb.stream);
-  LUBatch<T> lu;
-  lu.add(base, bpiv);
-  lu.run(sc, pb.stream);
-  st.flops += 8.0 * n * (double)n * n / 3.0;
+  DenseOps<T> dn;
+  dn.factor(ar, sc, fb, base);
+  dn.run_factor(sc, pb.stream);
+  st.flops += 16.0 * n * (double)n * n / 3.0;
   HIP_OK(hipStreamSynchronize(pb.stream));
   sc.reset();
 }
@@ -612,6 +609,8 @@ void Factor<T>::factor() {
     }
   }
   base_factor();
+  sweep_operators();
+  build_plan(1);
   HIP_OK(hipStreamSynchronize(pb.stream));
   st.t_factor = now_s() - t0;
   st.bytes = ar.used();
@@ -619,193 +618,280 @@ void Factor<T>::factor() {
     for (auto& F : B[j]) st.max_rank = std::max(st.max_rank, F.r);
 }
 
-// Forward sweep (the reference's fused RHS updates inside eliminate_*),
-// dense base solve, and back-substitution (back_substitution_phase,
-// FMM2DTree_find_error.hpp:2487-2532) over the stored factors.
+// Sweep operators of every eliminated box (the products the forward sweep
+// and back-substitution apply per box, formed once):
+//   Z = A^{-1},  H = -P2M Z,  H <- L2M_f^{-1} H,  Z <- Z - W_r H,
+//   U = W_r L2M_f^{-1},  and the inverse of the level-2 system.
 template <class T>
-void Factor<T>::solve(const double2* b, double2* x, int nrhs) {
-  auto& st_ = pb.stream;
-  sv.reset();
-  int64_t N = pb.N;
-  Blk<T> Bt = new_blk<T>(sv, (int)N, nrhs);
-  {
-    std::vector<CvtDesc<T>> cv{CvtDesc<T>{b, Bt.p, N * nrhs}};
-    auto* d = upload(sc, cv, st_);
-    launch_cvt<T>(1, d, st_);
-  }
-  Blk<T> Xt = new_blk<T>(sv, (int)N, nrhs);
-  std::vector<std::vector<Blk<T>>> bk(L + 1), bs(L + 1), mr(L + 1), xk(L + 1), yk(L + 1);
+void Factor<T>::sweep_operators() {
   CopyBatch<T> cp;
-  SolveBatch<T> svb;
+  DenseOps<T> svb;
   GemmBatch<T> g;
+  std::vector<Blk<T>> ginv;
+  for (int j = 2; j <= L; ++j)
+    for (auto& F : B[j]) {
+      F.Z = new_blk<T>(ar, F.m, F.m);
+      F.H = new_blk<T>(ar, F.r, F.m);
+      F.U = new_blk<T>(ar, F.m, F.r);
+      if (F.m) cp.add(F.Z, F.Z, 3);
+    }
+  cp.run(sc, pb.stream);
+  for (int j = 2; j <= L; ++j)
+    for (auto& F : B[j])
+      if (F.m) svb.solve(sc, F.fa, F.Z), st.flops += 8.0 * F.m * F.m * F.m;
+  svb.run_solve(sc, pb.stream);
+  for (int j = 2; j <= L; ++j)
+    for (auto& F : B[j])
+      if (F.m && F.r) g.add(F.P2M, 0, F.Z, 0, F.H, MONE, ZERO), st.flops += 8.0 * F.r * F.m * F.m;
+  g.run(sc, pb.stream);
+  for (int j = 2; j <= L; ++j)
+    for (auto& F : B[j])
+      if (F.m && F.r) svb.solve(sc, F.fg, F.H), st.flops += 8.0 * F.r * F.r * F.m;
+  svb.run_solve(sc, pb.stream);
+  for (int j = 2; j <= L; ++j)
+    for (auto& F : B[j])
+      if (F.m && F.r) g.add(F.W.sub(0, 0, F.m, F.r), 0, F.H, 0, F.Z, MONE, ONE), st.flops += 8.0 * F.m * F.r * F.m;
+  g.run(sc, pb.stream);
+  // U = W_r L2M_f^{-1} via the r x r inverse
+  for (int j = 2; j <= L; ++j)
+    for (auto& F : B[j]) {
+      Blk<T> gi = new_blk<T>(sc, F.r, F.r);
+      ginv.push_back(gi);
+      if (F.m && F.r) cp.add(gi, gi, 3);
+    }
+  cp.run(sc, pb.stream);
+  size_t q = 0;
+  for (int j = 2; j <= L; ++j)
+    for (auto& F : B[j]) {
+      if (F.m && F.r) svb.solve(sc, F.fg, ginv[q]), st.flops += 8.0 * F.r * F.r * F.r;
+      ++q;
+    }
+  svb.run_solve(sc, pb.stream);
+  q = 0;
+  for (int j = 2; j <= L; ++j)
+    for (auto& F : B[j]) {
+      if (F.m && F.r) g.add(F.W.sub(0, 0, F.m, F.r), 0, ginv[q], 0, F.U, ONE, ZERO), st.flops += 8.0 * F.m * F.r * F.r;
+      ++q;
+    }
+  g.run(sc, pb.stream);
+  int n = boff.empty() ? 0 : boff[16];
+  binv = new_blk<T>(ar, n, n);
+  if (n) {
+    cp.add(binv, binv, 3);
+    cp.run(sc, pb.stream);
+    svb.solve(sc, fb, binv);
+    svb.run_solve(sc, pb.stream);
+    st.flops += 8.0 * n * (double)n * n;
+  }
+  HIP_OK(hipStreamSynchronize(pb.stream));
+  sc.reset();
+}
+
+// The forward sweep (the RHS updates the reference fuses into eliminate_*),
+// the base solve and the back-substitution (back_substitution_phase,
+// FMM2DTree_find_error.hpp:2487-2532) as one box program:
+//  forward, level j = L..2, colour c = 0..8, box k of colour c:
+//     y = b_k - sum_{nn earlier colour} P2P[k][nn] b_nn ;  bs_k = y
+//     b_k = Z_k y ;  mr_k = H_k y            (mr_k = rows of the parent's b)
+//  then every box nn:  mr_nn -= sum_{k later colour} P2L[nn][k] b_k
+//  base: xb = (level-2 system)^{-1} yb       (yb = the level-2 mr stacked)
+//  backward, level j = 2..L, colour c = 8..0, box k:
+//     x0 = bs_k - sum_{nn later colour} P2P[k][nn] x_nn
+//               - sum_{nn earlier colour} M2P[k][nn] y_nn
+//     x_k = Z_k x0 - U_k y_k                  (y_k = rows of the parent's x)
+// A pulled update equals the reference's pushed one: each neighbour update
+// is applied before the value it changes is next read.
+template <class T>
+void Factor<T>::build_plan(int cap) {
+  plan.reset(new Program<T>());
+  pv.reset();
+  plan_cap = cap;
+  const int64_t N = pb.N;
+  Bt = new_blk<T>(pv, (int)N, cap);
+  Xt = new_blk<T>(pv, (int)N, cap);
+  const int nbase = boff.empty() ? 0 : boff[16];
+  yb = new_blk<T>(pv, nbase, cap);
+  xb = new_blk<T>(pv, nbase, cap);
+  bk.assign(L + 1, {});
+  bs.assign(L + 1, {});
+  mr.assign(L + 1, {});
+  xk.assign(L + 1, {});
+  yk.assign(L + 1, {});
   for (int j = L; j >= 2; --j) {
     int nb = 1 << (2 * j);
     bk[j].resize(nb);
     bs[j].resize(nb);
-    mr[j].resize(nb);
     xk[j].resize(nb);
-    yk[j].resize(nb);
     for (int k = 0; k < nb; ++k) {
       FBox& F = B[j][k];
       if (j == L) {
-        bk[j][k] = Bt.sub(k * R, 0, R, nrhs);
-        xk[j][k] = Xt.sub(k * R, 0, R, nrhs);
+        if (F.m != R) throw std::runtime_error("leaf block size mismatch");
+        bk[j][k] = Bt.sub(k * R, 0, R, cap);
+        xk[j][k] = Xt.sub(k * R, 0, R, cap);
       } else {
-        bk[j][k] = new_blk<T>(sv, F.m, nrhs);
-        xk[j][k] = new_blk<T>(sv, F.m, nrhs);
-        int o = 0;
-        for (int c = 0; c < 4; ++c) {
-          int cr = B[j + 1][4 * k + c].r;
-          if (cr) cp.add(mr[j + 1][4 * k + c], bk[j][k].sub(o, 0, cr, nrhs));
-          o += cr;
-        }
+        bk[j][k] = new_blk<T>(pv, F.m, cap);
+        xk[j][k] = new_blk<T>(pv, F.m, cap);
       }
-      bs[j][k] = new_blk<T>(sv, F.m, nrhs);
-      mr[j][k] = new_blk<T>(sv, F.r, nrhs);
-      yk[j][k] = new_blk<T>(sv, F.r, nrhs);
+      bs[j][k] = new_blk<T>(pv, F.m, cap);
     }
-    cp.run(sc, st_);
+  }
+  for (int j = L; j >= 2; --j) {
+    int nb = 1 << (2 * j);
+    mr[j].resize(nb);
+    yk[j].resize(nb);
+    for (int k = 0; k < nb; ++k) {
+      int r = B[j][k].r;
+      if (j == 2) {
+        mr[j][k] = yb.sub(boff[k], 0, r, cap);
+        yk[j][k] = xb.sub(boff[k], 0, r, cap);
+      } else {
+        int o = 0;
+        for (int c = 0; c < (k & 3); ++c) o += B[j][(k & ~3) + c].r;
+        mr[j][k] = bk[j - 1][k >> 2].sub(o, 0, r, cap);
+        yk[j][k] = xk[j - 1][k >> 2].sub(o, 0, r, cap);
+      }
+    }
+  }
+  Program<T>& P = *plan;
+  auto need = [](const Blk<T>& b, int rows, int cols, const char* what) {
+    if (b.r != rows || b.c != cols) throw std::runtime_error(std::string("sweep block shape mismatch: ") + what);
+  };
+  // forward
+  for (int j = L; j >= 2; --j) {
     const auto& LS = pb.lists[j];
     for (int c = 0; c < 9; ++c) {
-      const auto& S = colors[j][c];
-      for (int k : S)
-        if (B[j][k].m) cp.add(bk[j][k], bs[j][k]);
-      cp.run(sc, st_);
-      for (int k : S)
-        if (B[j][k].m) svb.add(B[j][k].A, B[j][k].piv, bk[j][k]);
-      svb.run(sc, st_);
-      for (int k : S) {
+      P.stage();
+      for (int k : colors[j][c]) {
         FBox& F = B[j][k];
-        if (F.m && F.r) g.add(F.P2M, 0, bk[j][k], 0, mr[j][k], MONE, ZERO);
-        else if (F.r) cp.add(mr[j][k], mr[j][k], 4);
-      }
-      g.run(sc, st_);
-      cp.run(sc, st_);
-      for (int k : S)
-        if (B[j][k].r) svb.add(B[j][k].G.sub(0, 0, B[j][k].r, B[j][k].r), B[j][k].piv2, mr[j][k]);
-      svb.run(sc, st_);
-      for (int k : S) {
-        FBox& F = B[j][k];
-        if (F.m && F.r) g.add(F.W.sub(0, 0, F.m, F.r), 0, mr[j][k], 0, bk[j][k], MONE, ONE);
-      }
-      g.run(sc, st_);
-      for (int k : S) {
-        FBox& F = B[j][k];
-        if (!F.m) continue;
-        for (int n = 0; n < 8; ++n) {
+        if (!F.m && !F.r) continue;
+        PProg<T>& p = P.add(F.m);
+        p.src = bk[j][k].p;
+        p.ldsrc = bk[j][k].ld;
+        p.save = bs[j][k].p;
+        p.ldsave = bs[j][k].ld;
+        P.begin_phase1();
+        for (int n = 0; n < 8 && F.m; ++n) {
           int nn = LS[k].nb[n];
-          if (nn < 0) continue;
-          FBox& Nb = B[j][nn];
-          bool before = Nb.color < c;
-          auto& cm = before ? Nb.P2L : Nb.P2P;
-          auto it = cm.find(k);
-          if (it == cm.end() || !it->second.size()) continue;
-          g.add(it->second, 0, bk[j][k], 0, before ? mr[j][nn] : bk[j][nn], MONE, ONE);
+          if (nn < 0 || B[j][nn].color >= c) continue;
+          auto it = F.P2P.find(nn);
+          if (it == F.P2P.end() || !it->second.size()) continue;
+          need(it->second, F.m, B[j][nn].m, "P2P (forward)");
+          P.term(it->second.p, it->second.ld, it->second.c, bk[j][nn].p, bk[j][nn].ld, MONE);
+        }
+        P.end_phase1();
+        int o = 0;
+        if (F.m) {
+          P.begin_out(o, bk[j][k].p, bk[j][k].ld, F.m);
+          P.term(F.Z.p, F.Z.ld, F.m, nullptr, 0, ONE);
+          P.end_out(o++);
+        }
+        if (F.r) {
+          P.begin_out(o, mr[j][k].p, mr[j][k].ld, F.r);
+          if (F.m) P.term(F.H.p, F.H.ld, F.m, nullptr, 0, ONE);
+          P.end_out(o++);
         }
       }
-      g.run(sc, st_);
+    }
+    // deferred P2L updates of earlier-coloured boxes
+    P.stage();
+    int nb = 1 << (2 * j);
+    for (int nn = 0; nn < nb; ++nn) {
+      FBox& Fn = B[j][nn];
+      if (!Fn.r) continue;
+      std::vector<std::pair<const Blk<T>*, int>> ts;
+      for (int n = 0; n < 8; ++n) {
+        int k = LS[nn].nb[n];
+        if (k < 0 || B[j][k].color <= Fn.color || !B[j][k].m) continue;
+        auto it = Fn.P2L.find(k);
+        if (it == Fn.P2L.end() || !it->second.size()) continue;
+        need(it->second, Fn.r, B[j][k].m, "P2L");
+        ts.push_back({&it->second, k});
+      }
+      if (ts.empty()) continue;
+      PProg<T>& p = P.add(Fn.r);
+      p.src = mr[j][nn].p;
+      p.ldsrc = mr[j][nn].ld;
+      p.save = mr[j][nn].p;
+      p.ldsave = mr[j][nn].ld;
+      P.begin_phase1();
+      for (auto& t : ts) P.term(t.first->p, t.first->ld, t.first->c, bk[j][t.second].p, bk[j][t.second].ld, MONE);
+      P.end_phase1();
     }
   }
   // base
-  if (boff.size() == 17 && boff[16] > 0) {
-    Blk<T> yb = new_blk<T>(sv, boff[16], nrhs);
-    for (int k = 0; k < 16; ++k)
-      if (B[2][k].r) cp.add(mr[2][k], yb.sub(boff[k], 0, B[2][k].r, nrhs));
-    cp.run(sc, st_);
-    svb.add(base, bpiv, yb);
-    svb.run(sc, st_);
-    for (int k = 0; k < 16; ++k)
-      if (B[2][k].r) cp.add(yb.sub(boff[k], 0, B[2][k].r, nrhs), yk[2][k]);
-    cp.run(sc, st_);
+  if (nbase) {
+    P.stage();
+    const int RB = 64;
+    for (int r0 = 0; r0 < nbase; r0 += RB) {
+      int rows = std::min(RB, nbase - r0);
+      PProg<T>& p = P.add(nbase);
+      p.src = yb.p;
+      p.ldsrc = yb.ld;
+      P.begin_phase1();
+      P.end_phase1();
+      P.begin_out(0, xb.p + r0, xb.ld, rows);
+      P.term(binv.p + r0, binv.ld, nbase, nullptr, 0, ONE);
+      P.end_out(0);
+    }
   }
   // backward
   for (int j = 2; j <= L; ++j) {
-    int nb = 1 << (2 * j);
     const auto& LS = pb.lists[j];
-    if (j > 2)
-      for (int K = 0; K < nb / 4; ++K) {
-        int o = 0;
-        for (int c = 0; c < 4; ++c) {
-          int ch = 4 * K + c, cr = B[j][ch].r;
-          if (cr) cp.add(xk[j - 1][K].sub(o, 0, cr, nrhs), yk[j][ch]);
-          o += cr;
-        }
-      }
-    cp.run(sc, st_);
     for (int c = 8; c >= 0; --c) {
-      const auto& S = colors[j][c];
-      for (int k : S)
-        if (B[j][k].m) cp.add(bs[j][k], xk[j][k]);
-      cp.run(sc, st_);
-      // one launch per neighbour slot: the updates of one box accumulate into
-      // the same output block, which must not happen inside one launch
-      for (int n = 0; n < 8; ++n) {
-        for (int k : S) {
-          FBox& F = B[j][k];
-          if (!F.m) continue;
+      P.stage();
+      for (int k : colors[j][c]) {
+        FBox& F = B[j][k];
+        if (!F.m) continue;
+        PProg<T>& p = P.add(F.m);
+        p.src = bs[j][k].p;
+        p.ldsrc = bs[j][k].ld;
+        P.begin_phase1();
+        for (int n = 0; n < 8; ++n) {
           int nn = LS[k].nb[n];
           if (nn < 0) continue;
           bool after = B[j][nn].color > c;
           auto& rm = after ? F.P2P : F.M2P;
           auto it = rm.find(nn);
           if (it == rm.end() || !it->second.size()) continue;
-          g.add(it->second, 0, after ? xk[j][nn] : yk[j][nn], 0, xk[j][k], MONE, ONE);
+          const Blk<T>& v = after ? xk[j][nn] : yk[j][nn];
+          need(it->second, F.m, v.r, after ? "P2P (backward)" : "M2P");
+          P.term(it->second.p, it->second.ld, it->second.c, v.p, v.ld, MONE);
         }
-        g.run(sc, st_);
+        P.end_phase1();
+        P.begin_out(0, xk[j][k].p, xk[j][k].ld, F.m);
+        P.term(F.Z.p, F.Z.ld, F.m, nullptr, 0, ONE);
+        if (F.r) P.term(F.U.p, F.U.ld, F.r, yk[j][k].p, yk[j][k].ld, MONE);
+        P.end_out(0);
       }
-      for (int k : S)
-        if (B[j][k].m) svb.add(B[j][k].A, B[j][k].piv, xk[j][k]);
-      svb.run(sc, st_);
-      std::vector<Blk<T>> zt(S.size());
-      for (size_t t = 0; t < S.size(); ++t) {
-        FBox& F = B[j][S[t]];
-        zt[t] = new_blk<T>(sv, F.r, nrhs);
-        if (F.r) cp.add(yk[j][S[t]], zt[t]);
-      }
-      cp.run(sc, st_);
-      for (size_t t = 0; t < S.size(); ++t) {
-        FBox& F = B[j][S[t]];
-        if (F.r && F.m) g.add(F.P2M, 0, xk[j][S[t]], 0, zt[t], MONE, ONE);
-      }
-      g.run(sc, st_);
-      for (size_t t = 0; t < S.size(); ++t) {
-        FBox& F = B[j][S[t]];
-        if (F.r) svb.add(F.G.sub(0, 0, F.r, F.r), F.piv2, zt[t]);
-      }
-      svb.run(sc, st_);
-      for (size_t t = 0; t < S.size(); ++t) {
-        FBox& F = B[j][S[t]];
-        if (F.r && F.m) g.add(F.W.sub(0, 0, F.m, F.r), 0, zt[t], 0, xk[j][S[t]], MONE, ONE);
-      }
-      g.run(sc, st_);
     }
   }
-  // leaf x (already in Xt) -> FP64
+  P.finalize(pv, pb.stream);
+}
+
+template <class T>
+void Factor<T>::solve(const double2* b, double2* x, int nrhs) {
+  if (B.empty()) throw std::runtime_error("solve before factor");
+  if (nrhs > plan_cap) build_plan(std::max(nrhs, 1));
+  hipStream_t s = pb.stream;
+  const int64_t N = pb.N;
+  // b and Bt/Xt share the leading dimension N
   if (std::is_same<T, double2>::value) {
-    HIP_OK(hipMemcpyAsync(x, Xt.p, N * nrhs * 16, hipMemcpyDeviceToDevice, st_));
+    HIP_OK(hipMemcpyAsync(Bt.p, b, N * nrhs * sizeof(T), hipMemcpyDeviceToDevice, s));
   } else {
-    launch_f2d((const float2*)Xt.p, x, N * nrhs, st_);
+    launch_d2f(b, (float2*)Bt.p, N * nrhs, s);
   }
-  HIP_OK(hipStreamSynchronize(st_));
-  sc.reset();
+  plan->run(nrhs, s);
+  if (std::is_same<T, double2>::value) {
+    HIP_OK(hipMemcpyAsync(x, Xt.p, N * nrhs * sizeof(T), hipMemcpyDeviceToDevice, s));
+  } else {
+    launch_f2d((const float2*)Xt.p, x, N * nrhs, s);
+  }
 }
 
 template <class T>
 size_t Factor<T>::solve_bytes(int nrhs) const {
-  size_t e = 0;
-  for (int j = 2; j <= L; ++j)
-    for (int k = 0; k < (int)B[j].size(); ++k) {
-      const FBox& F = B[j][k];
-      size_t m = F.m, r = F.r;
-      e += 2 * (m * m + m * r + r * m + r * r);  // A, W_L, P2M, L2M_f: both sweeps
-      for (auto& x : F.P2P)
-        if (x.first != k) e += x.second.size();
-      for (auto& x : F.M2P) e += x.second.size();
-      for (auto& x : F.P2L) e += x.second.size();
-      e += 0;
-    }
-  e += base.size();
-  return e * sizeof(T) + (size_t)pb.N * nrhs * 32;
+  if (plan) return (size_t)plan->bytes(nrhs);
+  return 0;
 }
 
 template class Factor<double2>;

@@ -1,14 +1,17 @@
-// FP64 hierarchical matvec: leaf near field, upward pass (P2M), M2L on every
-// level, downward pass (L2P) — the nested H2 product of the operator built
-// by getNodes / assemble_M2L / initialise_P2P_Leaf_Level
-// (FMM2DTree_find_error.hpp:1105-1361, 1729-1741).
+// FP64 hierarchical matvec: the nested H2 product of the operator built by
+// getNodes / assemble_M2L / initialise_P2P_Leaf_Level
+// (FMM2DTree_find_error.hpp:1105-1361, 1729-1741), as one box program:
+//   upward,   level j = L..2:  mu_j[k] = P2M_k (x_k  or  stacked mu_{j+1} of the children)
+//   downward, level j = 2..L:  lo_j[k] = sum_{b in IL(k)} M2L[k][b] mu_j[b]
+//                                        + (L2P_parent lo_{j-1}[parent]) rows of k
+//   leaves:                    y_k = L2P_k lo_L[k] + sum_{nn in N(k) and k} P2P[k][nn] x_nn
 #include "h2.h"
 
 #include <vector>
 
 namespace aifmm {
 
-static const double2 ONE = {1.0, 0.0}, ZERO = {0.0, 0.0};
+static const double2 ONE = {1.0, 0.0};
 
 void col_norms(Arena& sc, const double2* v, int64_t n, int nrhs, double* out, hipStream_t s) {
   std::vector<CopyDesc<double2>> d(nrhs);
@@ -27,14 +30,15 @@ void col_norms(Arena& sc, const double2* v, int64_t n, int nrhs, double* out, hi
   sc.reset();
 }
 
-void H2Op::apply(const double2* x, double2* y, int nrhs) {
+void H2Op::build(const double2* x, double2* y, int nrhs) {
   Problem& P = pb;
-  if (!P.assembled) P.assemble();
-  hipStream_t s = P.stream;
   const int L = P.L, R = P.R;
   const int N = (int)P.N;
   ws.reset();
-  // per-level multipole / local buffers, boxes stacked in box order
+  plan.reset(new Program<double2>());
+  px = x;
+  py = y;
+  cap = nrhs;
   std::vector<std::vector<int>> off(L + 1);
   std::vector<Blk<double2>> Mu(L + 1), Lo(L + 1);
   for (int j = 2; j <= L; ++j) {
@@ -44,67 +48,92 @@ void H2Op::apply(const double2* x, double2* y, int nrhs) {
     Mu[j] = new_blk<double2>(ws, off[j][nb], nrhs);
     Lo[j] = new_blk<double2>(ws, off[j][nb], nrhs);
   }
-  Blk<double2> X{(double2*)x, N, nrhs, N}, Y{y, N, nrhs, N};
-  GemmBatch<double2> g;
-  CopyBatch<double2> cp;
-  // y = 0 ; locals = 0
-  cp.add(Y, Y, 4);
-  for (int j = 2; j <= L; ++j)
-    if (Lo[j].size()) cp.add(Lo[j], Lo[j], 4);
-  cp.run(sc, s);
-  // near field, one batch per neighbour slot (outputs written once per batch)
-  for (int slot = 0; slot < 9; ++slot) {
-    for (int k = 0; k < P.nleaf; ++k) {
-      int nn = slot == 8 ? k : P.lists[L][k].nb[slot];
-      if (nn < 0) continue;
-      const Blk<double2>& A = P.P2P[k].at(nn);
-      g.add(A, 0, X.sub(nn * R, 0, R, nrhs), 0, Y.sub(k * R, 0, R, nrhs), ONE, ONE);
-    }
-    g.run(sc, s);
-  }
+  Program<double2>& G = *plan;
+  auto need = [](bool ok, const char* what) {
+    if (!ok) throw std::runtime_error(std::string("H2 operator block shape mismatch: ") + what);
+  };
   // upward pass
   for (int j = L; j >= 2; --j) {
+    G.stage();
     int nb = 1 << (2 * j);
     for (int k = 0; k < nb; ++k) {
       const SkelBox& B = P.sk[j][k];
-      if (!B.r || !B.m) continue;
-      Blk<double2> src = (j == L) ? X.sub(k * R, 0, R, nrhs) : Mu[j + 1].sub(off[j + 1][4 * k], 0, B.m, nrhs);
-      g.add(B.P2M, 0, src, 0, Mu[j].sub(off[j][k], 0, B.r, nrhs), ONE, ZERO);
-    }
-    g.run(sc, s);
-  }
-  // M2L on each level, one batch per interaction-list slot
-  for (int j = 2; j <= L; ++j) {
-    int nb = 1 << (2 * j);
-    for (int slot = 0; slot < 40; ++slot) {
-      for (int k = 0; k < nb; ++k) {
-        const Lists& Ls = P.lists[j][k];
-        int b = slot < 16 ? Ls.inner[slot] : Ls.outer[slot - 16];
-        if (b < 0) continue;
-        const SkelBox& B = P.sk[j][k];
-        const Blk<double2>& M = B.M2L.at(b);
-        if (!M.size()) continue;
-        g.add(M, 0, Mu[j].sub(off[j][b], 0, P.sk[j][b].r, nrhs), 0, Lo[j].sub(off[j][k], 0, B.r, nrhs), ONE, ONE);
+      if (!B.r) continue;
+      G.add(B.r);
+      PProg<double2>& p = G.cur();
+      p.save = Mu[j].p + off[j][k];
+      p.ldsave = Mu[j].ld;
+      G.begin_phase1();
+      if (B.m) {
+        need(B.P2M.r == B.r && B.P2M.c == B.m, "P2M");
+        if (j == L) G.term(B.P2M.p, B.P2M.ld, B.m, x + (size_t)k * R, N, ONE);
+        else G.term(B.P2M.p, B.P2M.ld, B.m, Mu[j + 1].p + off[j + 1][4 * k], Mu[j + 1].ld, ONE);
       }
-      g.run(sc, s);
+      G.end_phase1();
     }
   }
-  // downward pass: children locals += L2P_parent * local_parent ; leaf output
+  // downward pass with M2L; near field and L2P at the leaves
   for (int j = 2; j <= L; ++j) {
+    G.stage();
     int nb = 1 << (2 * j);
     for (int k = 0; k < nb; ++k) {
       const SkelBox& B = P.sk[j][k];
-      if (!B.r || !B.m) continue;
-      Blk<double2> dst = (j == L) ? Y.sub(k * R, 0, R, nrhs) : Lo[j + 1].sub(off[j + 1][4 * k], 0, B.m, nrhs);
-      g.add(B.L2P, 0, Lo[j].sub(off[j][k], 0, B.r, nrhs), 0, dst, ONE, ONE);
+      if (!B.r && j < L) continue;
+      G.add(B.r);
+      PProg<double2>& p = G.cur();
+      if (j < L && B.r) {
+        p.save = Lo[j].p + off[j][k];
+        p.ldsave = Lo[j].ld;
+      }
+      G.begin_phase1();
+      if (B.r) {
+        const Lists& Ls = P.lists[j][k];
+        for (int s = 0; s < 40; ++s) {
+          int b = s < 16 ? Ls.inner[s] : Ls.outer[s - 16];
+          if (b < 0) continue;
+          auto it = B.M2L.find(b);
+          if (it == B.M2L.end() || !it->second.size()) continue;
+          need(it->second.r == B.r && it->second.c == P.sk[j][b].r, "M2L");
+          G.term(it->second.p, it->second.ld, it->second.c, Mu[j].p + off[j][b], Mu[j].ld, ONE);
+        }
+        if (j > 2) {
+          const SkelBox& Pa = P.sk[j - 1][k >> 2];
+          int o = 0;
+          for (int c = 0; c < (k & 3); ++c) o += P.sk[j][(k & ~3) + c].r;
+          need(Pa.L2P.r == Pa.m && o + B.r <= Pa.L2P.r, "parent L2P");
+          if (Pa.r) G.term(Pa.L2P.p + o, Pa.L2P.ld, Pa.r, Lo[j - 1].p + off[j - 1][k >> 2], Lo[j - 1].ld, ONE);
+        }
+      }
+      G.end_phase1();
+      if (j == L) {
+        G.begin_out(0, y + (size_t)k * R, N, R);
+        if (B.r) {
+          need(B.L2P.r == R && B.L2P.c == B.r, "leaf L2P");
+          G.term(B.L2P.p, B.L2P.ld, B.r, nullptr, 0, ONE);
+        }
+        for (int slot = 0; slot < 9; ++slot) {
+          int nn = slot == 8 ? k : P.lists[L][k].nb[slot];
+          if (nn < 0) continue;
+          auto it = P.P2P[k].find(nn);
+          if (it == P.P2P[k].end() || !it->second.size()) continue;
+          need(it->second.r == R && it->second.c == R, "P2P");
+          G.term(it->second.p, it->second.ld, R, x + (size_t)nn * R, N, ONE);
+        }
+        G.end_out(0);
+      }
     }
-    g.run(sc, s);
   }
-  HIP_OK(hipStreamSynchronize(s));
-  sc.reset();
+  G.finalize(ws, P.stream);
+}
+
+void H2Op::apply(const double2* x, double2* y, int nrhs) {
+  if (!pb.assembled) pb.assemble();
+  if (!plan || x != px || y != py || nrhs > cap) build(x, y, nrhs);
+  plan->run(nrhs, pb.stream);
 }
 
 size_t H2Op::bytes(int nrhs) const {
+  if (plan) return (size_t)plan->bytes(nrhs);
   size_t e = 0;
   for (auto& m : pb.P2P)
     for (auto& b : m) e += b.second.size();

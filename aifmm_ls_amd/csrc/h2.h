@@ -4,21 +4,30 @@
 // the complex64 factorisation and for residual checks.
 #pragma once
 #include "problem.h"
+#include "prog.h"
+
+#include <memory>
 
 namespace aifmm {
 
 class H2Op {
  public:
   explicit H2Op(Problem& pb) : pb(pb) {}
-  // y = A_H x ; x, y device arrays N x nrhs (column-major, complex128)
+  // y = A_H x ; x, y device arrays N x nrhs (column-major, complex128).
+  // Asynchronous on the problem's stream.  The launch program is built for
+  // one (x, y) pair and rebuilt when either pointer or a larger nrhs appears.
   void apply(const double2* x, double2* y, int nrhs);
-  // algorithmic bytes read per application (operator blocks + vectors)
+  // algorithmic bytes of one application (operator blocks + vectors)
   size_t bytes(int nrhs) const;
 
  private:
   Problem& pb;
   Arena ws{size_t(64) << 20};
-  Arena sc{size_t(16) << 20};
+  std::unique_ptr<Program<double2>> plan;
+  const double2* px = nullptr;
+  double2* py = nullptr;
+  int cap = 0;
+  void build(const double2* x, double2* y, int nrhs);
 };
 
 // column 2-norms of an N x nrhs device array (host result)

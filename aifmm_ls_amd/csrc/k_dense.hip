@@ -5,64 +5,10 @@
 // or per output tile; every batch is described by a device array of
 // descriptors built on the host.
 #include "common.h"
+#include "cplx_dev.h"
 #include "kernels.h"
 
 namespace aifmm {
-
-template <class T>
-__device__ __forceinline__ T czero();
-template <>
-__device__ __forceinline__ double2 czero<double2>() {
-  return mk(0.0, 0.0);
-}
-template <>
-__device__ __forceinline__ float2 czero<float2>() {
-  return mkf(0.f, 0.f);
-}
-template <class T>
-__device__ __forceinline__ T from2(double2 a);
-template <>
-__device__ __forceinline__ double2 from2<double2>(double2 a) {
-  return a;
-}
-template <>
-__device__ __forceinline__ float2 from2<float2>(double2 a) {
-  return mkf((float)a.x, (float)a.y);
-}
-// a / b by Smith's algorithm (no |b|^2, so no underflow for small b)
-template <class T>
-__device__ __forceinline__ T cdiv(T a, T b) {
-  using R = typename CplxOf<T>::real;
-  T r;
-  if (fabs(b.x) >= fabs(b.y)) {
-    R q = b.y / b.x, den = b.x + b.y * q;
-    r.x = (a.x + a.y * q) / den;
-    r.y = (a.y - a.x * q) / den;
-  } else {
-    R q = b.x / b.y, den = b.y + b.x * q;
-    r.x = (a.x * q + a.y) / den;
-    r.y = (a.y * q - a.x) / den;
-  }
-  return r;
-}
-template <class T>
-__device__ __forceinline__ T cfma(T a, T b, T c) {  // c + a*b
-  T r;
-  r.x = c.x + a.x * b.x - a.y * b.y;
-  r.y = c.y + a.x * b.y + a.y * b.x;
-  return r;
-}
-template <class T>
-__device__ __forceinline__ T cfms(T a, T b, T c) {  // c - a*b
-  T r;
-  r.x = c.x - a.x * b.x + a.y * b.y;
-  r.y = c.y - a.x * b.y - a.y * b.x;
-  return r;
-}
-template <class T>
-__device__ __forceinline__ typename CplxOf<T>::real cabsT(T a) {
-  return hypot(a.x, a.y);
-}
 
 constexpr int NT = 256;
 
@@ -261,7 +207,12 @@ __global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
   __shared__ double shs[4];
   const int m = d.m, n = d.n, ld = d.lda;
   T* A = d.A;
+  // column norms as LAPACK/Eigen keep them: nrm = running (downdated) norm,
+  // dir = last exactly computed norm; a downdate that loses more than
+  // sqrt(eps) triggers an exact recomputation (ColPivHouseholderQR)
   R* nrm = d.work;
+  R* dir = d.work + n;
+  const double dd_thr = sqrt((double)(sizeof(R) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-07));
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int j = threadIdx.x; j < n; j += NT) d.perm[j] = j;
   for (int j = w; j < n; j += NT / 64) {
@@ -269,11 +220,20 @@ __global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
     for (int i = lane; i < m; i += 64) s += (double)A[i + (size_t)ld * j].x * A[i + (size_t)ld * j].x +
                                           (double)A[i + (size_t)ld * j].y * A[i + (size_t)ld * j].y;
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (lane == 0) nrm[j] = (R)sqrt(s);
+    if (lane == 0) nrm[j] = dir[j] = (R)sqrt(s);
   }
   __syncthreads();
   int size = m < n ? m : n;
   double maxpiv = 0;
+  // Eigen's "nonzero pivots": the first k whose pivot norm^2 falls below
+  // (max initial norm * eps)^2 / m * (m - k), with the double epsilon for both
+  // element types (a complex64 factor that dropped pivots below the float
+  // epsilon would lose the badly scaled rows of low-contrast boxes)
+  double maxnorm = 0;
+  for (int j = 0; j < n; ++j) maxnorm = fmax(maxnorm, (double)nrm[j]);
+  const double eps_r = 2.220446049250313e-16;
+  const double th_helper = (maxnorm * eps_r) * (maxnorm * eps_r) / (m > 0 ? m : 1);
+  int nonzero = size;
   for (int k = 0; k < size; ++k) {
     double bv = -1;
     int bi = -1;
@@ -285,6 +245,7 @@ __global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
     int big;
     block_argmax_first(bv, bi, shv, shi, pv, big);
     if (big < 0) big = k;
+    if (nonzero == size && pv * pv < th_helper * (m - k)) nonzero = k;
     if (big != k) {
       for (int i = threadIdx.x; i < m; i += NT) {
         T t = A[i + (size_t)ld * k];
@@ -295,6 +256,9 @@ __global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
         R t = nrm[k];
         nrm[k] = nrm[big];
         nrm[big] = t;
+        t = dir[k];
+        dir[k] = dir[big];
+        dir[big] = t;
         int q = d.perm[k];
         d.perm[k] = d.perm[big];
         d.perm[big] = q;
@@ -359,7 +323,17 @@ __global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
         nn += (double)x.x * x.x + (double)x.y * x.y;
       }
       for (int o = 32; o > 0; o >>= 1) nn += __shfl_xor(nn, o);
-      if (lane == 0) nrm[j] = (R)sqrt(nn);
+      if (lane == 0) {
+        double u = nrm[j];
+        if (u != 0) {
+          double tq = hypot((double)col[k].x, (double)col[k].y) / u;
+          tq = (1 + tq) * (1 - tq);
+          if (tq < 0) tq = 0;
+          double rr = u / (double)dir[j];
+          if (tq * rr * rr <= dd_thr) nrm[j] = dir[j] = (R)sqrt(nn);
+          else nrm[j] = (R)(u * sqrt(tq));
+        }
+      }
     }
     __syncthreads();
   }
@@ -370,6 +344,50 @@ __global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
       if (hypot((double)A[i + (size_t)ld * i].x, (double)A[i + (size_t)ld * i].y) > pre) ++r;
     *d.rank_out = r;
     if (d.maxpiv_out) *d.maxpiv_out = maxpiv;
+    if (d.nz_out) *d.nz_out = nonzero;
+  }
+}
+
+// ColPivHouseholderQR::solve for square A, one wave per right-hand side.
+// Lane l owns rows l, l + 64, ... of its column in the scratch copy, so every
+// value a lane reads it wrote itself; sums and pivots move between lanes by
+// shuffles only.
+template <class T>
+__global__ __launch_bounds__(NT) void k_qr_solve(const QSolveDesc<T>* __restrict__ ds) {
+  const QSolveDesc<T> d = ds[blockIdx.x];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int n = d.n, la = d.lda;
+  const int nz = *d.nz;
+  const T* A = d.A;
+  for (int c = blockIdx.y * 16 + w; c < d.nrhs && c < blockIdx.y * 16 + 16; c += NT / 64) {
+    T* b = d.B + (size_t)d.ldb * c;
+    T* x = d.scratch + (size_t)n * c;
+    for (int i = lane; i < n; i += 64) x[i] = b[i];
+    // x <- Q^H x
+    for (int k = 0; k < nz; ++k) {
+      const T* v = A + (size_t)la * k;
+      T s = czero<T>();
+      for (int i = k + lane; i < n; i += 64) s = (i == k) ? cadd(s, x[i]) : cfma(cconj(v[i]), x[i], s);
+      for (int o = 32; o > 0; o >>= 1) {
+        s.x += __shfl_xor(s.x, o);
+        s.y += __shfl_xor(s.y, o);
+      }
+      s = cmul(d.tau[k], s);
+      for (int i = k + lane; i < n; i += 64) x[i] = (i == k) ? csub(x[i], s) : cfms(s, v[i], x[i]);
+    }
+    // R[:nz, :nz] y = x
+    for (int i = nz - 1; i >= 0; --i) {
+      T yi = czero<T>();
+      if (lane == (i & 63)) {
+        yi = cdiv(x[i], A[i + (size_t)la * i]);
+        x[i] = yi;
+      }
+      yi.x = __shfl(yi.x, i & 63);
+      yi.y = __shfl(yi.y, i & 63);
+      for (int l = lane; l < i; l += 64) x[l] = cfms(A[l + (size_t)la * i], yi, x[l]);
+    }
+    // b[perm[i]] = y[i] (i < nz), 0 for the rest
+    for (int i = lane; i < n; i += 64) b[d.perm[i]] = i < nz ? x[i] : czero<T>();
   }
 }
 
@@ -476,6 +494,12 @@ void launch_lu_solve(int nb, int max_chunks, const SolveDesc<T>* d, hipStream_t 
   HIP_OK(hipGetLastError());
 }
 template <class T>
+void launch_qr_solve(int nb, int max_chunks, const QSolveDesc<T>* d, hipStream_t s) {
+  if (!nb || !max_chunks) return;
+  hipLaunchKernelGGL(k_qr_solve<T>, dim3(nb, max_chunks), dim3(NT), 0, s, d);
+  HIP_OK(hipGetLastError());
+}
+template <class T>
 void launch_cpqr(int nb, const QRDesc<T>* d, hipStream_t s) {
   if (!nb) return;
   hipLaunchKernelGGL(k_cpqr<T>, dim3(nb), dim3(NT), 0, s, d);
@@ -521,6 +545,7 @@ void launch_f2d(const float2* src, double2* dst, int64_t n, hipStream_t s) {
   template void launch_lu<T>(int, const LUDesc<T>*, hipStream_t);                       \
   template void launch_lu_solve<T>(int, int, const SolveDesc<T>*, hipStream_t);         \
   template void launch_cpqr<T>(int, const QRDesc<T>*, hipStream_t);                     \
+  template void launch_qr_solve<T>(int, int, const QSolveDesc<T>*, hipStream_t);         \
   template void launch_qform<T>(int, const QFormDesc<T>*, hipStream_t);                 \
   template void launch_copy<T>(int, const CopyDesc<T>*, hipStream_t);                   \
   template void launch_fro<T>(int, const CopyDesc<T>*, double*, hipStream_t);            \

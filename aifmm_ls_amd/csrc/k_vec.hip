@@ -1,0 +1,67 @@
+// Vector kernels of the refinement loop: residual with per-block partial
+// squared norms (summed on the host in a fixed order, so the norm does not
+// depend on scheduling), and y += x.
+#include "cplx_dev.h"
+#include "kernels.h"
+
+namespace aifmm {
+
+constexpr int VNT = 256;
+
+// r = b - ax (ax may be null: r = b); r may be null (norms only).
+// partial[c * gridDim.x + blockIdx.x] = sum over the block's rows of |r|^2.
+__global__ __launch_bounds__(VNT) void k_resid(const double2* __restrict__ b, const double2* __restrict__ ax,
+                                               double2* __restrict__ r, int64_t n, double* __restrict__ partial) {
+  const int c = blockIdx.y;
+  const size_t base = (size_t)n * c;
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * VNT + threadIdx.x; i < n; i += (int64_t)gridDim.x * VNT) {
+    double2 v = b[base + i];
+    if (ax) {
+      double2 a = ax[base + i];
+      v.x -= a.x;
+      v.y -= a.y;
+    }
+    if (r) r[base + i] = v;
+    s += v.x * v.x + v.y * v.y;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  __shared__ double sh[VNT / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < VNT / 64; ++w) t += sh[w];
+    partial[(size_t)c * gridDim.x + blockIdx.x] = t;
+  }
+}
+
+__global__ __launch_bounds__(VNT) void k_axpy(double2* __restrict__ y, const double2* __restrict__ x, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * VNT + threadIdx.x; i < n; i += (int64_t)gridDim.x * VNT) {
+    double2 a = y[i], b = x[i];
+    a.x += b.x;
+    a.y += b.y;
+    y[i] = a;
+  }
+}
+
+int resid_blocks(int64_t n) {
+  int64_t b = (n + VNT - 1) / VNT;
+  return (int)(b < 1024 ? (b < 1 ? 1 : b) : 1024);
+}
+
+void launch_resid(const double2* b, const double2* ax, double2* r, int64_t n, int nrhs, double* partial,
+                  hipStream_t s) {
+  if (!n || !nrhs) return;
+  hipLaunchKernelGGL(k_resid, dim3(resid_blocks(n), nrhs), dim3(VNT), 0, s, b, ax, r, n, partial);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_axpy(double2* y, const double2* x, int64_t n, hipStream_t s) {
+  if (!n) return;
+  int64_t b = (n + VNT - 1) / VNT;
+  hipLaunchKernelGGL(k_axpy, dim3((unsigned)(b < 2048 ? b : 2048)), dim3(VNT), 0, s, y, x, n);
+  HIP_OK(hipGetLastError());
+}
+
+}  // namespace aifmm

@@ -92,12 +92,29 @@ struct QRDesc {
   T* A;         // m x n, overwritten with R and Householder vectors
   T* tau;       // min(m,n)
   int* perm;    // n
-  typename CplxOf<T>::real* work;  // n column norms
+  typename CplxOf<T>::real* work;  // 2n: running and last exact column norms
   int m, n, lda;
   double thr;   // relative rank threshold
   int* rank_out;
   double* maxpiv_out;
+  int* nz_out;  // optional: number of pivots above Eigen's zero threshold
 };
+
+// B <- A^{-1} B from the column-pivoted QR of a square A (ColPivHouseholderQR
+// ::solve): Q^H by the first nz reflectors, R[:nz, :nz] back-substitution,
+// un-permute; unknowns past nz are 0.  scratch: n x nrhs, leading dim n.
+template <class T>
+struct QSolveDesc {
+  const T* A;
+  const T* tau;
+  const int* perm;
+  const int* nz;
+  T* B;
+  T* scratch;
+  int n, nrhs, lda, ldb;
+};
+template <class T>
+void launch_qr_solve(int nb, int max_chunks, const QSolveDesc<T>* d, hipStream_t s);
 
 template <class T>
 struct QFormDesc {
@@ -144,6 +161,12 @@ template <class T>
 void launch_copy(int nb, const CopyDesc<T>* d, hipStream_t s);
 template <class T>
 void launch_fro(int nb, const CopyDesc<T>* d, double* out, hipStream_t s);
+// refinement vectors: r = b - ax with per-block partial squared norms
+// (resid_blocks(n) blocks per column), and y += x over n entries
+int resid_blocks(int64_t n);
+void launch_resid(const double2* b, const double2* ax, double2* r, int64_t n, int nrhs, double* partial,
+                  hipStream_t s);
+void launch_axpy(double2* y, const double2* x, int64_t n, hipStream_t s);
 void launch_d2f(const double2* src, float2* dst, int64_t n, hipStream_t s);
 void launch_f2d(const float2* src, double2* dst, int64_t n, hipStream_t s);
 

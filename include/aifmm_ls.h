@@ -84,7 +84,8 @@ int aifmm_ls_aca(aifmm_ls_problem* h, int tol_pow, int n1, const int64_t* rows, 
 /* Kernel self-test (no reference counterpart; for the test suite): runs one
  * batched dense kernel on host data.  op 0: C = A B (m x k, k x n); op 1:
  * C = A^{-1} B by LU (A m x m, B m x n); op 2: column-pivoted QR of A (m x n)
- * with relative threshold thr, iout[0] = rank, C = first k columns of Q.
+ * with relative threshold thr, iout[0] = rank, C = first k columns of Q; op 3:
+ * C = A^{-1} B by column-pivoted QR (A m x m), iout[0] = nonzero pivots.
  * op + 10 runs the complex64 kernels. */
 int aifmm_ls_selftest(int op, int m, int n, int k, const double* A, const double* B, double* C, int* iout,
                       double thr);

@@ -45,6 +45,13 @@ void* orc_create(const int* ip, double kappa, double L, const double* bumps, int
 
 void orc_destroy(void* p) { delete (Handle*)p; }
 
+// Sensitivity probe: perturb the far-field entries (see Disc::perturb).
+void orc_set_perturb(void* p, double eps, unsigned long long seed) {
+  Handle* h = (Handle*)p;
+  h->D.perturb = eps;
+  h->D.perturb_seed = seed;
+}
+
 long long orc_n(void* p) { return ((Handle*)p)->D.N; }
 
 void orc_points(void* p, double* xy, double* q) {

@@ -401,6 +401,13 @@ cd Disc::entry(int64_t i, int64_t j) const {
     return e;
   }
   cd s = far_sum(i, lj, jj);
+  if (perturb != 0) {
+    uint64_t h = ((uint64_t)i * 0x9E3779B97F4A7C15ull) ^ ((uint64_t)j * 0xC2B2AE3D27D4EB4Full) ^ perturb_seed;
+    h ^= h >> 31;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 29;
+    s *= 1.0 + perturb * ((double)(h % 2000001ull) / 1000000.0 - 1.0);
+  }
   s *= cd(0.0, 0.25);
   s *= P.kappa * P.kappa * q[i];
   if (i == j) s += 1.0;

@@ -106,6 +106,12 @@ struct Disc {
   Mat M;                             // 121 x rank (level L only)
   Mat T[9];                          // colleague operators at level L
   int series = 60;
+  // Sensitivity probe (tests only): far-field entries are multiplied by
+  // 1 + perturb * u(i, j), u a fixed hash of (i, j, perturb_seed) in [-1, 1].
+  // Measures how far the algorithm's discrete choices (pivots, ranks) move
+  // the solution under last-bit changes of its input.
+  double perturb = 0;
+  uint64_t perturb_seed = 0;
   void build(const Params& P);
   cd entry(int64_t i, int64_t j) const;
   cd far_sum(int64_t i, int leaf_j, int idx_j) const;

@@ -35,6 +35,7 @@ def lib():
         L.orc_create.restype = P
         L.orc_create.argtypes = [ip, ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_int]
         L.orc_destroy.argtypes = [P]
+        L.orc_set_perturb.argtypes = [P, ctypes.c_double, ctypes.c_ulonglong]
         L.orc_n.restype = ctypes.c_longlong
         L.orc_n.argtypes = [P]
         L.orc_points.argtypes = [P, dp, dp]
@@ -69,7 +70,7 @@ class Problem:
     """One discretised LS problem (tables, points, contrast) in the oracle."""
 
     def __init__(self, n_levels, kappa, tol_pow=8, qchoice=0, n_cheb=6, degree=6, L=1.0,
-                 bumps=None, quantize=True, exact_bessel=False):
+                 bumps=None, quantize=True, exact_bessel=False, perturb=0.0, perturb_seed=0):
         ip = np.array([n_cheb, degree, n_levels, tol_pow, qchoice, int(quantize), int(exact_bessel)], np.int32)
         nb = 0
         self._bumps = None
@@ -82,6 +83,9 @@ class Problem:
         self.rank = n_cheb * n_cheb
         self.nbasis = degree * (degree + 1) // 2
         self.N = int(lib().orc_n(self.h))
+        if perturb:
+            # sensitivity probe: far-field entries scaled by 1 + perturb * u(i, j)
+            lib().orc_set_perturb(self.h, float(perturb), int(perturb_seed))
 
     def __del__(self):
         if getattr(self, "h", None):

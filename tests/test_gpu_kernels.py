@@ -33,6 +33,33 @@ def test_lu_solve(m, n):
     assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-3
 
 
+@pytest.mark.parametrize("m,n", [(1, 1), (36, 1), (36, 290), (77, 65), (200, 3), (130, 17)])
+def test_qr_solve(m, n):
+    rng = np.random.default_rng(3 * m + n)
+    a, b = rnd(rng, m, m), rnd(rng, m, n)
+    ref = np.linalg.solve(a, b)
+    x, nz = A.selftest_qr_solve(a, b)
+    assert nz == m
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-11
+    x, nz = A.selftest_qr_solve(a, b, c64=True)
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-3
+
+
+def test_qr_solve_rank_deficient_matches_oracle():
+    """Singular A: the solve keeps the nonzero pivots only (unknowns past them
+    are 0), as ColPivHouseholderQR::solve; compared with the oracle's CPQR."""
+    from oracle import oracle as O
+    rng = np.random.default_rng(11)
+    m, r = 24, 9
+    # exactly singular: m - r zero columns in shuffled positions
+    a = np.concatenate([rnd(rng, m, r), np.zeros((m, m - r))], axis=1)[:, rng.permutation(m)]
+    b = rnd(rng, m, 3)
+    x, nz = A.selftest_qr_solve(a, b)
+    xo = O.qr_solve(a, b)
+    assert nz == r
+    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-10
+
+
 @pytest.mark.parametrize("m,n,r", [(36, 20, 8), (40, 130, 12), (100, 60, 31)])
 def test_cpqr_rank_and_basis(m, n, r):
     rng = np.random.default_rng(m + n + r)

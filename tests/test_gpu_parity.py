@@ -1,0 +1,233 @@
+"""GPU solve path against the oracle (FP64 CPU restatement of the reference)
+on identical inputs, through the C ABI.
+
+Bars: tables and matrix entries to rounding; NCA ranks and skeleton index
+lists exactly; the FP64 direct solve within 10x the compression tolerance of
+the oracle run with the same (colour) elimination schedule; the refined
+solves against the dense FP64 solution of the oracle's matrix; and, at the
+N = 147,456 workload, size-independent properties (linearity, residual,
+agreement of the complex64 and complex128 factorisations after refinement).
+"""
+import numpy as np
+import pytest
+
+import aifmm_ls_amd as A
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+BUMPS2 = [[1.0, 0.15, 0.1, 0.2], [0.7, 0.1, -0.3, -0.2]]
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def c2_bumps(seed=2024, n=8):
+    rng = np.random.default_rng(seed)
+    return np.c_[rng.uniform(0.2, 1.5, n), rng.uniform(0.05, 0.2, n), rng.uniform(-0.7, 0.7, (n, 2))]
+
+
+CASES = [  # (levels, kappa, tol_pow, qchoice, bumps)
+    (2, 10.0, 8, 0, None),
+    (3, 10.0, 8, 0, None),
+    (3, 20.0, 8, 6, BUMPS2),
+    (3, 40.0, 7, 6, c2_bumps()),
+    (4, 10.0, 8, 0, None),
+]
+IDS = [f"L{c[0]}-k{int(c[1])}-t{c[2]}-q{c[3]}" for c in CASES]
+
+
+@pytest.fixture(scope="module", params=CASES, ids=IDS)
+def pair(request):
+    L, kap, tp, qc, bumps = request.param
+    print(f"\n[case] levels={L} kappa={kap} tol=1e-{tp} q={qc}", flush=True)
+    g = A.LSProblem(L, kap, tp, qc, bumps=bumps)
+    o = O.Problem(L, kap, tp, qc, bumps=bumps)
+    yield request.param, g, o
+    g.close()
+
+
+def test_tables(pair):
+    _, g, o = pair
+    Mg, Tg, _ = g.tables()
+    Mo, To, _ = o.tables()
+    assert rel(Mg, Mo) < 1e-14
+    assert rel(Tg, To) < 1e-14
+
+
+def test_entries(pair):
+    _, g, o = pair
+    rng = np.random.default_rng(0)
+    I = rng.integers(0, g.N, 20000)
+    J = rng.integers(0, g.N, 20000)
+    eg, eo = g.entries(I, J), o.entries(I, J)
+    scale = np.maximum(np.abs(eo), 1e-300)
+    big = np.abs(eo) > 1e-14
+    assert np.max(np.abs(eg - eo)[big] / scale[big]) < 1e-11
+    assert np.max(np.abs(eg - eo)) < 1e-13 * max(np.abs(eo).max(), 1.0)
+
+
+def test_nca_ranks_and_skeletons(pair):
+    (L, *_), g, o = pair
+    g.assemble()
+    o.nca()
+    ro = o.ranks()
+    for j in range(2, L + 1):
+        np.testing.assert_array_equal(g.ranks(j), ro[j][: 4 ** j])
+    rng = np.random.default_rng(1)
+    for j in range(2, L + 1):
+        for k in rng.integers(0, 4 ** j, 6):
+            for which in range(4):
+                np.testing.assert_array_equal(g.skeleton(j, int(k), which), o.skeleton(j, int(k), which))
+
+
+def test_aca(pair):
+    (L, kap, tp, *_), g, o = pair
+    # leaf box 0 against the points of its interaction list (inner + outer rings)
+    _, inner, outer = o.lists(L, 0)
+    il = [int(b) for b in np.concatenate([inner, outer]) if b >= 0]
+    rows = np.arange(36)
+    cols = np.concatenate([np.arange(36 * b, 36 * b + 36) for b in il]) if il else np.arange(36, 72)
+    rbg, cbg, Acg, Arg = g.aca(rows, cols, tp)
+    rbo, cbo, Aco, Aro = o.aca(rows, cols, tp)
+    np.testing.assert_array_equal(rbg, rbo)
+    np.testing.assert_array_equal(cbg, cbo)
+    if rbg.size:
+        assert rel(Acg, Aco) < 1e-11 and rel(Arg, Aro) < 1e-11
+
+
+def oracle_sensitivity(param, b, schedule):
+    """How far the oracle's own solution moves when its far-field entries
+    change in the last bit (two draws of a 1e-15 relative perturbation): the
+    algorithm's discrete choices (ACA pivots and stops, CPQR pivots and
+    ranks) make it discontinuous, so no second implementation can agree with
+    it more closely than this on the same input."""
+    L, kap, tp, qc, bumps = param
+    x0, _ = O.Problem(L, kap, tp, qc, bumps=bumps).solve(b, schedule)
+    d = 0.0
+    for seed in (1, 2):
+        xp, _ = O.Problem(L, kap, tp, qc, bumps=bumps, perturb=1e-15, perturb_seed=seed).solve(b, schedule)
+        d = max(d, rel(xp, x0))
+    return d
+
+
+def test_fp64_solve_vs_oracle_colour_schedule(pair):
+    """Same (colour) schedule on both sides.  Bar: 10x the compression
+    tolerance, or 20x the oracle's own last-bit sensitivity where that is
+    larger (strong multi-bump contrasts)."""
+    param, g, o = pair
+    tp = param[2]
+    b = g.rhs(0.0)
+    bo = o.rhs(0.0)
+    assert rel(b, bo) < 1e-15
+    g.factor(A.PREC_C128)
+    x, _ = g.solve(b)
+    xo, _ = o.solve(bo, O.SCHED_COLORED)
+    sens = oracle_sensitivity(param, bo, O.SCHED_COLORED)
+    bar = max(10 * 10.0 ** -tp, 20 * sens)
+    print(f"  fp64 direct vs oracle colour schedule {rel(x, xo):.3e} (oracle sensitivity {sens:.2e}, bar {bar:.2e})",
+          flush=True)
+    assert rel(x, xo) < bar
+
+
+def test_c64_refined_vs_fp64_refined(pair):
+    (L, kap, tp, *_), g, o = pair
+    b = g.rhs(0.4)
+    g.factor(A.PREC_C128)
+    x64, st64 = g.solve(b, tol=1e-13, max_refine=40)
+    g.factor(A.PREC_C64)
+    x32, st32 = g.solve(b, tol=1e-13, max_refine=40)
+    print(f"  refined relres fp64 {st64[2]:.2e} ({int(st64[1])} steps) c64 {st32[2]:.2e} ({int(st32[1])} steps) "
+          f"c64 vs fp64 {rel(x32, x64):.3e}", flush=True)
+    assert st64[2] < 1e-11 and st32[2] < 1e-11
+    assert rel(x32, x64) < 1e-9
+
+
+def test_multi_rhs_and_determinism(pair):
+    _, g, o = pair
+    g.factor(A.PREC_C128)
+    B = np.stack([g.rhs(t) for t in (0.0, 1.0, 2.5)], axis=1)
+    X, _ = g.solve(B)
+    for c in range(3):
+        x1, _ = g.solve(B[:, c])
+        x2, _ = g.solve(B[:, c])
+        assert np.array_equal(x1, x2)
+        assert rel(X[:, c], x1) < 1e-13
+
+
+@pytest.mark.parametrize("qc,kap,bumps", [(0, 10.0, None), (6, 20.0, BUMPS2)])
+def test_against_dense_solution(qc, kap, bumps):
+    """L=3: the hierarchical operator applied to the dense solution, and the
+    direct / refined solutions, against the dense FP64 solve of the oracle's
+    matrix (numpy LU)."""
+    L, tp = 3, 8
+    g = A.LSProblem(L, kap, tp, qc, bumps=bumps)
+    o = O.Problem(L, kap, tp, qc, bumps=bumps)
+    Ad = o.dense()
+    b = o.rhs(0.0)
+    xd = np.linalg.solve(Ad, b)
+    g.assemble()
+    res_op = rel(g.matvec(xd), b)
+    g.factor(A.PREC_C128)
+    xdir, _ = g.solve(b)
+    xe, _ = o.solve(b, O.SCHED_EFFICIENT)
+    g.factor(A.PREC_C64)
+    xr, st = g.solve(b, tol=1e-13, max_refine=40)
+    print(f"\n  |A_H xd - b|/|b| {res_op:.2e}; vs dense: direct {rel(xdir, xd):.2e} "
+          f"reference schedule {rel(xe, xd):.2e} refined c64 {rel(xr, xd):.2e}", flush=True)
+    # the reference's compressed operator (ACA at tol, skeleton ranks cut to
+    # min(incoming, outgoing)) is itself only this close to the dense matrix:
+    # the bars are the reference's own direct-solve error
+    ref_err = max(rel(xe, xd), 10.0 ** -tp)
+    assert res_op < 2 * ref_err
+    assert rel(xr, xd) < 1.5 * ref_err
+    assert rel(xdir, xd) < 1.5 * ref_err
+    g.close()
+
+
+def test_zero_contrast():
+    """q = 0 everywhere: A = I and b = 0, so the solution is 0 (rank-0 boxes,
+    an empty base system)."""
+    g = A.LSProblem(3, 10.0, 8, 6, bumps=[[0.0, 0.1, 0.0, 0.0]])
+    b = g.rhs(0.0)
+    assert np.all(b == 0)
+    g.factor(A.PREC_C64)
+    x, _ = g.solve(b, tol=1e-10, max_refine=3)
+    assert np.all(np.isfinite(x)) and np.abs(x).max() == 0
+    g.close()
+
+
+def test_errors_are_reported():
+    with pytest.raises(A.AifmmError):
+        A.LSProblem(1, 10.0, 8, 0)
+    g = A.LSProblem(2, 10.0, 8, 0)
+    with pytest.raises(A.AifmmError):
+        g.set_rhs(g.rhs(0.0))
+        g.solve_resident(1e-6, 0)
+    with pytest.raises(A.AifmmError):
+        g.factor(7)
+    g.close()
+
+
+def test_c2_workload_properties():
+    """N = 147,456, kappa = 40, tol 1e-7, 8 bumps: linearity of the direct
+    solve, refinement to the FP64 operator, and c64 vs c128 after refinement."""
+    g = A.LSProblem(6, 40.0, 7, 6, bumps=c2_bumps())
+    g.assemble()
+    b1, b2 = g.rhs(0.3), g.rhs(1.7)
+    g.factor(A.PREC_C128)
+    X, _ = g.solve(np.stack([b1, b2, b1 + 2j * b2], axis=1))
+    # rounding only, amplified by the self blocks (condition up to ~1e10)
+    assert rel(X[:, 2], X[:, 0] + 2j * X[:, 1]) < 1e-10
+    x64, st64 = g.solve(b1, tol=1e-12, max_refine=40)
+    assert st64[2] < 1e-12
+    g.factor(A.PREC_C64)
+    X32, _ = g.solve(np.stack([b1, b2, b1 + 2j * b2], axis=1))
+    # complex64 rounding through the same self blocks; refinement removes it
+    assert rel(X32[:, 2], X32[:, 0] + 2j * X32[:, 1]) < 1e-2
+    x32, st32 = g.solve(b1, tol=1e-12, max_refine=40)
+    print(f"\n  C2: fp64 {int(st64[1])} steps, c64 {int(st32[1])} steps, c64 vs fp64 {rel(x32, x64):.2e}", flush=True)
+    assert st32[2] < 1e-12
+    assert rel(x32, x64) < 1e-9
+    g.close()

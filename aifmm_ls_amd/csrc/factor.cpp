@@ -752,6 +752,13 @@ void Factor<T>::build_plan(int cap) {
       }
     }
   }
+  // phase-1 vectors of tall boxes in the back-substitution (no save target)
+  std::vector<std::vector<Blk<T>>> yt(L + 1);
+  for (int j = 2; j <= L; ++j) {
+    yt[j].resize(1 << (2 * j));
+    for (int k = 0; k < (1 << (2 * j)); ++k)
+      if (B[j][k].m > Program<T>::TILE_M) yt[j][k] = new_blk<T>(pv, B[j][k].m, cap);
+  }
   Program<T>& P = *plan;
   auto need = [](const Blk<T>& b, int rows, int cols, const char* what) {
     if (b.r != rows || b.c != cols) throw std::runtime_error(std::string("sweep block shape mismatch: ") + what);
@@ -764,32 +771,26 @@ void Factor<T>::build_plan(int cap) {
       for (int k : colors[j][c]) {
         FBox& F = B[j][k];
         if (!F.m && !F.r) continue;
-        PProg<T>& p = P.add(F.m);
-        p.src = bk[j][k].p;
-        p.ldsrc = bk[j][k].ld;
-        p.save = bs[j][k].p;
-        p.ldsave = bs[j][k].ld;
-        P.begin_phase1();
+        BoxOp<T> op;
+        op.m = F.m;
+        op.src = bk[j][k].p;
+        op.ldsrc = bk[j][k].ld;
+        op.save = bs[j][k].p;
+        op.ldsave = bs[j][k].ld;
         for (int n = 0; n < 8 && F.m; ++n) {
           int nn = LS[k].nb[n];
           if (nn < 0 || B[j][nn].color >= c) continue;
           auto it = F.P2P.find(nn);
           if (it == F.P2P.end() || !it->second.size()) continue;
           need(it->second, F.m, B[j][nn].m, "P2P (forward)");
-          P.term(it->second.p, it->second.ld, it->second.c, bk[j][nn].p, bk[j][nn].ld, MONE);
+          op.term(it->second.p, it->second.ld, it->second.c, bk[j][nn].p, bk[j][nn].ld, MONE);
         }
-        P.end_phase1();
-        int o = 0;
-        if (F.m) {
-          P.begin_out(o, bk[j][k].p, bk[j][k].ld, F.m);
-          P.term(F.Z.p, F.Z.ld, F.m, nullptr, 0, ONE);
-          P.end_out(o++);
-        }
+        if (F.m) out_term<T>(op.out(bk[j][k].p, bk[j][k].ld, F.m), F.Z.p, F.Z.ld, F.m, nullptr, 0, ONE);
         if (F.r) {
-          P.begin_out(o, mr[j][k].p, mr[j][k].ld, F.r);
-          if (F.m) P.term(F.H.p, F.H.ld, F.m, nullptr, 0, ONE);
-          P.end_out(o++);
+          auto& o = op.out(mr[j][k].p, mr[j][k].ld, F.r);
+          if (F.m) out_term<T>(o, F.H.p, F.H.ld, F.m, nullptr, 0, ONE);
         }
+        P.add(op);
       }
     }
     // deferred P2L updates of earlier-coloured boxes
@@ -798,41 +799,30 @@ void Factor<T>::build_plan(int cap) {
     for (int nn = 0; nn < nb; ++nn) {
       FBox& Fn = B[j][nn];
       if (!Fn.r) continue;
-      std::vector<std::pair<const Blk<T>*, int>> ts;
+      BoxOp<T> op;
+      op.m = Fn.r;
+      op.src = mr[j][nn].p;
+      op.ldsrc = mr[j][nn].ld;
+      op.save = mr[j][nn].p;
+      op.ldsave = mr[j][nn].ld;
       for (int n = 0; n < 8; ++n) {
         int k = LS[nn].nb[n];
         if (k < 0 || B[j][k].color <= Fn.color || !B[j][k].m) continue;
         auto it = Fn.P2L.find(k);
         if (it == Fn.P2L.end() || !it->second.size()) continue;
         need(it->second, Fn.r, B[j][k].m, "P2L");
-        ts.push_back({&it->second, k});
+        op.term(it->second.p, it->second.ld, it->second.c, bk[j][k].p, bk[j][k].ld, MONE);
       }
-      if (ts.empty()) continue;
-      PProg<T>& p = P.add(Fn.r);
-      p.src = mr[j][nn].p;
-      p.ldsrc = mr[j][nn].ld;
-      p.save = mr[j][nn].p;
-      p.ldsave = mr[j][nn].ld;
-      P.begin_phase1();
-      for (auto& t : ts) P.term(t.first->p, t.first->ld, t.first->c, bk[j][t.second].p, bk[j][t.second].ld, MONE);
-      P.end_phase1();
+      if (!op.terms.empty()) P.add(op);
     }
   }
   // base
   if (nbase) {
     P.stage();
-    const int RB = 64;
-    for (int r0 = 0; r0 < nbase; r0 += RB) {
-      int rows = std::min(RB, nbase - r0);
-      PProg<T>& p = P.add(nbase);
-      p.src = yb.p;
-      p.ldsrc = yb.ld;
-      P.begin_phase1();
-      P.end_phase1();
-      P.begin_out(0, xb.p + r0, xb.ld, rows);
-      P.term(binv.p + r0, binv.ld, nbase, nullptr, 0, ONE);
-      P.end_out(0);
-    }
+    BoxOp<T> op;
+    op.m = 0;
+    out_term<T>(op.out(xb.p, xb.ld, nbase), binv.p, binv.ld, nbase, yb.p, yb.ld, ONE);
+    P.add(op);
   }
   // backward
   for (int j = 2; j <= L; ++j) {
@@ -842,10 +832,12 @@ void Factor<T>::build_plan(int cap) {
       for (int k : colors[j][c]) {
         FBox& F = B[j][k];
         if (!F.m) continue;
-        PProg<T>& p = P.add(F.m);
-        p.src = bs[j][k].p;
-        p.ldsrc = bs[j][k].ld;
-        P.begin_phase1();
+        BoxOp<T> op;
+        op.m = F.m;
+        op.src = bs[j][k].p;
+        op.ldsrc = bs[j][k].ld;
+        op.ybuf = yt[j][k].p;
+        op.ldybuf = yt[j][k].ld;
         for (int n = 0; n < 8; ++n) {
           int nn = LS[k].nb[n];
           if (nn < 0) continue;
@@ -855,13 +847,12 @@ void Factor<T>::build_plan(int cap) {
           if (it == rm.end() || !it->second.size()) continue;
           const Blk<T>& v = after ? xk[j][nn] : yk[j][nn];
           need(it->second, F.m, v.r, after ? "P2P (backward)" : "M2P");
-          P.term(it->second.p, it->second.ld, it->second.c, v.p, v.ld, MONE);
+          op.term(it->second.p, it->second.ld, it->second.c, v.p, v.ld, MONE);
         }
-        P.end_phase1();
-        P.begin_out(0, xk[j][k].p, xk[j][k].ld, F.m);
-        P.term(F.Z.p, F.Z.ld, F.m, nullptr, 0, ONE);
-        if (F.r) P.term(F.U.p, F.U.ld, F.r, yk[j][k].p, yk[j][k].ld, MONE);
-        P.end_out(0);
+        auto& o = op.out(xk[j][k].p, xk[j][k].ld, F.m);
+        out_term<T>(o, F.Z.p, F.Z.ld, F.m, nullptr, 0, ONE);
+        if (F.r) out_term<T>(o, F.U.p, F.U.ld, F.r, yk[j][k].p, yk[j][k].ld, MONE);
+        P.add(op);
       }
     }
   }

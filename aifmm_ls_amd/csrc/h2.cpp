@@ -59,17 +59,16 @@ void H2Op::build(const double2* x, double2* y, int nrhs) {
     for (int k = 0; k < nb; ++k) {
       const SkelBox& B = P.sk[j][k];
       if (!B.r) continue;
-      G.add(B.r);
-      PProg<double2>& p = G.cur();
-      p.save = Mu[j].p + off[j][k];
-      p.ldsave = Mu[j].ld;
-      G.begin_phase1();
+      BoxOp<double2> op;
+      op.m = B.r;
+      op.save = Mu[j].p + off[j][k];
+      op.ldsave = Mu[j].ld;
       if (B.m) {
         need(B.P2M.r == B.r && B.P2M.c == B.m, "P2M");
-        if (j == L) G.term(B.P2M.p, B.P2M.ld, B.m, x + (size_t)k * R, N, ONE);
-        else G.term(B.P2M.p, B.P2M.ld, B.m, Mu[j + 1].p + off[j + 1][4 * k], Mu[j + 1].ld, ONE);
+        if (j == L) op.term(B.P2M.p, B.P2M.ld, B.m, x + (size_t)k * R, N, ONE);
+        else op.term(B.P2M.p, B.P2M.ld, B.m, Mu[j + 1].p + off[j + 1][4 * k], Mu[j + 1].ld, ONE);
       }
-      G.end_phase1();
+      G.add(op);
     }
   }
   // downward pass with M2L; near field and L2P at the leaves
@@ -79,14 +78,12 @@ void H2Op::build(const double2* x, double2* y, int nrhs) {
     for (int k = 0; k < nb; ++k) {
       const SkelBox& B = P.sk[j][k];
       if (!B.r && j < L) continue;
-      G.add(B.r);
-      PProg<double2>& p = G.cur();
-      if (j < L && B.r) {
-        p.save = Lo[j].p + off[j][k];
-        p.ldsave = Lo[j].ld;
-      }
-      G.begin_phase1();
+      BoxOp<double2> op;
+      op.m = B.r;
       if (B.r) {
+        // the local vector is kept for the children (and is the tiled phase-1 target)
+        op.save = Lo[j].p + off[j][k];
+        op.ldsave = Lo[j].ld;
         const Lists& Ls = P.lists[j][k];
         for (int s = 0; s < 40; ++s) {
           int b = s < 16 ? Ls.inner[s] : Ls.outer[s - 16];
@@ -94,22 +91,21 @@ void H2Op::build(const double2* x, double2* y, int nrhs) {
           auto it = B.M2L.find(b);
           if (it == B.M2L.end() || !it->second.size()) continue;
           need(it->second.r == B.r && it->second.c == P.sk[j][b].r, "M2L");
-          G.term(it->second.p, it->second.ld, it->second.c, Mu[j].p + off[j][b], Mu[j].ld, ONE);
+          op.term(it->second.p, it->second.ld, it->second.c, Mu[j].p + off[j][b], Mu[j].ld, ONE);
         }
         if (j > 2) {
           const SkelBox& Pa = P.sk[j - 1][k >> 2];
           int o = 0;
           for (int c = 0; c < (k & 3); ++c) o += P.sk[j][(k & ~3) + c].r;
           need(Pa.L2P.r == Pa.m && o + B.r <= Pa.L2P.r, "parent L2P");
-          if (Pa.r) G.term(Pa.L2P.p + o, Pa.L2P.ld, Pa.r, Lo[j - 1].p + off[j - 1][k >> 2], Lo[j - 1].ld, ONE);
+          if (Pa.r) op.term(Pa.L2P.p + o, Pa.L2P.ld, Pa.r, Lo[j - 1].p + off[j - 1][k >> 2], Lo[j - 1].ld, ONE);
         }
       }
-      G.end_phase1();
       if (j == L) {
-        G.begin_out(0, y + (size_t)k * R, N, R);
+        auto& out = op.out(y + (size_t)k * R, N, R);
         if (B.r) {
           need(B.L2P.r == R && B.L2P.c == B.r, "leaf L2P");
-          G.term(B.L2P.p, B.L2P.ld, B.r, nullptr, 0, ONE);
+          out_term<double2>(out, B.L2P.p, B.L2P.ld, B.r, nullptr, 0, ONE);
         }
         for (int slot = 0; slot < 9; ++slot) {
           int nn = slot == 8 ? k : P.lists[L][k].nb[slot];
@@ -117,10 +113,10 @@ void H2Op::build(const double2* x, double2* y, int nrhs) {
           auto it = P.P2P[k].find(nn);
           if (it == P.P2P[k].end() || !it->second.size()) continue;
           need(it->second.r == R && it->second.c == R, "P2P");
-          G.term(it->second.p, it->second.ld, R, x + (size_t)nn * R, N, ONE);
+          out_term<double2>(out, it->second.p, it->second.ld, R, x + (size_t)nn * R, N, ONE);
         }
-        G.end_out(0);
       }
+      G.add(op);
     }
   }
   G.finalize(ws, P.stream);

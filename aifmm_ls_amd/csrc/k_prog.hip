@@ -1,7 +1,7 @@
-// Box-program kernel (descriptors in prog.h): one workgroup (4 wave64) per
-// program.  Each output row block of up to 256 rows is spread over the
+// Box-program kernel (descriptors in prog.h): one workgroup (4 or 16 wave64)
+// per program.  Each output row block of up to 256 rows is spread over the
 // workgroup as rp rows x ns column slices (rp = rows rounded up to 16, 32 or
-// a multiple of 64; ns = 256 / rp), so a wave reads a contiguous column
+// a multiple of 64; ns = threads / rp), so a wave reads a contiguous column
 // segment of the matrix per step (coalesced), and slice partial sums are
 // combined through LDS.  The phase-1 vector lives in LDS; right-hand sides
 // are processed CH columns at a time.
@@ -10,20 +10,45 @@
 
 namespace aifmm {
 
-constexpr int PNT = 256;
 constexpr int PROG_LDS = 64 * 1024;
 
-template <class T, int CH>
+// Sum over terms of alpha_t A_t[rb + i, :] x_t for this thread's row i and
+// column slice s (columns s, s + ns, ...), four independent loads in flight.
+template <class T, int CH, int NTH>
 __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __restrict__ terms, int t0, int t1,
                                               int c0, int nc, const T* ylds, int ldy, T* red, T (&acc)[CH],
                                               int& row) {
   const int tid = threadIdx.x;
   const int rp = rr <= 16 ? 16 : rr <= 32 ? 32 : ((rr + 63) >> 6) << 6;
-  const int ns = PNT / rp;
+  const int ns = NTH / rp;
   const int i = tid % rp, s = tid / rp;
 #pragma unroll
   for (int c = 0; c < CH; ++c) acc[c] = czero<T>();
-  if (s < ns && i < rr) {
+  const int nt = t1 - t0;
+  if (CH == 1 && ns >= nt && nt > 0) {
+    // single right-hand side, no more terms than slices: the slices are shared
+    // out among the terms (spt each), so all terms stream at once
+    const int spt = ns / nt;
+    if (s < spt * nt && i < rr) {
+      const PTerm<T> tm = terms[t0 + s % nt];
+      const int q = s / nt;
+      const T* A = tm.A + rb + i;
+      const T* x = tm.x ? tm.x + (size_t)tm.ldx * c0 : ylds;
+      const size_t la = tm.lda;
+      T pa = czero<T>();
+      int j = q;
+      for (; j + 3 * spt < tm.cols; j += 4 * spt) {
+        const T a0 = A[la * j], a1 = A[la * (j + spt)], a2 = A[la * (j + 2 * spt)], a3 = A[la * (j + 3 * spt)];
+        const T x0 = x[j], x1 = x[j + spt], x2 = x[j + 2 * spt], x3 = x[j + 3 * spt];
+        pa = cfma(a0, x0, pa);
+        pa = cfma(a1, x1, pa);
+        pa = cfma(a2, x2, pa);
+        pa = cfma(a3, x3, pa);
+      }
+      for (; j < tm.cols; j += spt) pa = cfma(A[la * j], x[j], pa);
+      acc[0] = cmul(from2<T>(tm.alpha), pa);
+    }
+  } else if (s < ns && i < rr) {
     for (int t = t0; t < t1; ++t) {
       const PTerm<T> tm = terms[t];
       const T* A = tm.A + rb + i;
@@ -36,11 +61,25 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
         x = ylds;
         ldx = ldy;
       }
+      const size_t la = tm.lda;
       T pa[CH];
 #pragma unroll
       for (int c = 0; c < CH; ++c) pa[c] = czero<T>();
-      for (int j = s; j < tm.cols; j += ns) {
-        const T a = A[(size_t)tm.lda * j];
+      int j = s;
+      for (; j + 3 * ns < tm.cols; j += 4 * ns) {
+        const T a0 = A[la * j], a1 = A[la * (j + ns)], a2 = A[la * (j + 2 * ns)], a3 = A[la * (j + 3 * ns)];
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+          if (c < nc) {
+            const T* xc = x + ldx * c;
+            pa[c] = cfma(a0, xc[j], pa[c]);
+            pa[c] = cfma(a1, xc[j + ns], pa[c]);
+            pa[c] = cfma(a2, xc[j + 2 * ns], pa[c]);
+            pa[c] = cfma(a3, xc[j + 3 * ns], pa[c]);
+          }
+      }
+      for (; j < tm.cols; j += ns) {
+        const T a = A[la * j];
 #pragma unroll
         for (int c = 0; c < CH; ++c)
           if (c < nc) pa[c] = cfma(a, x[j + ldx * c], pa[c]);
@@ -52,20 +91,20 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
   }
   if (ns > 1) {
 #pragma unroll
-    for (int c = 0; c < CH; ++c) red[tid + PNT * c] = acc[c];
+    for (int c = 0; c < CH; ++c) red[tid + NTH * c] = acc[c];
     __syncthreads();
     if (s == 0 && i < rr)
       for (int q = 1; q < ns; ++q)
 #pragma unroll
-        for (int c = 0; c < CH; ++c) acc[c] = cadd(acc[c], red[i + q * rp + PNT * c]);
+        for (int c = 0; c < CH; ++c) acc[c] = cadd(acc[c], red[i + q * rp + NTH * c]);
     __syncthreads();
   }
   row = rb + i;
   return s == 0 && i < rr;
 }
 
-template <class T, int CH>
-__global__ __launch_bounds__(PNT) void k_prog(const PProg<T>* __restrict__ progs, const PTerm<T>* __restrict__ terms,
+template <class T, int CH, int NTH>
+__global__ __launch_bounds__(NTH) void k_prog(const PProg<T>* __restrict__ progs, const PTerm<T>* __restrict__ terms,
                                               int nrhs, int ldy) {
   extern __shared__ __align__(16) unsigned char smem[];
   T* ylds = (T*)smem;
@@ -74,11 +113,11 @@ __global__ __launch_bounds__(PNT) void k_prog(const PProg<T>* __restrict__ progs
   const int m = P->m;
   for (int c0 = 0; c0 < nrhs; c0 += CH) {
     const int nc = min(CH, nrhs - c0);
-    for (int rb = 0; rb < m; rb += PNT) {
-      const int rr = min(PNT, m - rb);
+    for (int rb = 0; rb < m; rb += 256) {
+      const int rr = min(256, m - rb);
       T acc[CH];
       int row;
-      if (row_block_sum<T, CH>(rb, rr, terms, P->t0, P->t1, c0, nc, ylds, ldy, red, acc, row)) {
+      if (row_block_sum<T, CH, NTH>(rb, rr, terms, P->t0, P->t1, c0, nc, ylds, ldy, red, acc, row)) {
         const T sa = from2<T>(P->src_alpha);
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
@@ -93,11 +132,11 @@ __global__ __launch_bounds__(PNT) void k_prog(const PProg<T>* __restrict__ progs
     __syncthreads();
     for (int o = 0; o < P->nout; ++o) {
       const int rows = P->rows[o];
-      for (int rb = 0; rb < rows; rb += PNT) {
-        const int rr = min(PNT, rows - rb);
+      for (int rb = 0; rb < rows; rb += 256) {
+        const int rr = min(256, rows - rb);
         T acc[CH];
         int row;
-        if (row_block_sum<T, CH>(rb, rr, terms, P->u0[o], P->u1[o], c0, nc, ylds, ldy, red, acc, row)) {
+        if (row_block_sum<T, CH, NTH>(rb, rr, terms, P->u0[o], P->u1[o], c0, nc, ylds, ldy, red, acc, row)) {
           T* out = P->out[o];
           const size_t ldo = P->ldo[o];
 #pragma unroll
@@ -112,25 +151,37 @@ __global__ __launch_bounds__(PNT) void k_prog(const PProg<T>* __restrict__ progs
   }
 }
 
-static int pick_ch(int ldy, int eb) {
+// right-hand-side columns per pass: 1 for a single RHS, else the most that
+// fit the LDS with the reduction buffer of NTH threads
+static int pick_ch(int nrhs, int ldy, int eb, int nth) {
+  if (nrhs == 1) return (size_t)(ldy + nth) * eb <= (size_t)PROG_LDS ? 1 : 0;
   for (int ch : {4, 2, 1})
-    if ((size_t)(ldy + PNT) * ch * eb <= (size_t)PROG_LDS) return ch;
+    if ((size_t)(ldy + nth) * ch * eb <= (size_t)PROG_LDS) return ch;
   return 0;
 }
 
-int prog_max_m(int elem_bytes) { return PROG_LDS / elem_bytes - PNT; }
+int prog_max_m(int elem_bytes) { return PROG_LDS / elem_bytes - 1024; }
 
+template <class T, int NTH>
+static void launch_nth(int nprog, const PProg<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, hipStream_t s) {
+  const int ch = pick_ch(nrhs, ldy, sizeof(T), NTH);
+  if (!ch) throw std::runtime_error("box program vector of " + std::to_string(ldy) + " rows exceeds LDS");
+  const size_t lds = (size_t)(ldy + NTH) * ch * sizeof(T);
+  if (ch == 4) hipLaunchKernelGGL((k_prog<T, 4, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy);
+  else if (ch == 2) hipLaunchKernelGGL((k_prog<T, 2, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy);
+  else hipLaunchKernelGGL((k_prog<T, 1, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy);
+  HIP_OK(hipGetLastError());
+}
+
+// 1024-thread workgroups: with one right-hand side each term of a program
+// gets its own column slices, so a program's loads are all in flight at once.
+// Stages of more than 1024 programs already fill the chip with 256 threads.
 template <class T>
 void launch_prog(int nprog, const PProg<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, hipStream_t s) {
   if (!nprog || !nrhs) return;
   if (ldy < 1) ldy = 1;
-  const int ch = pick_ch(ldy, sizeof(T));
-  if (!ch) throw std::runtime_error("box program vector of " + std::to_string(ldy) + " rows exceeds LDS");
-  const size_t lds = (size_t)(ldy + PNT) * ch * sizeof(T);
-  if (ch == 4) hipLaunchKernelGGL((k_prog<T, 4>), dim3(nprog), dim3(PNT), lds, s, progs, terms, nrhs, ldy);
-  else if (ch == 2) hipLaunchKernelGGL((k_prog<T, 2>), dim3(nprog), dim3(PNT), lds, s, progs, terms, nrhs, ldy);
-  else hipLaunchKernelGGL((k_prog<T, 1>), dim3(nprog), dim3(PNT), lds, s, progs, terms, nrhs, ldy);
-  HIP_OK(hipGetLastError());
+  if (nprog > 1024) launch_nth<T, 256>(nprog, progs, terms, nrhs, ldy, s);
+  else launch_nth<T, 1024>(nprog, progs, terms, nrhs, ldy, s);
 }
 
 template void launch_prog<double2>(int, const PProg<double2>*, const PTerm<double2>*, int, int, hipStream_t);

@@ -2,6 +2,7 @@
 // algorithmic byte / flop count of one run.
 #include "prog.h"
 
+#include <algorithm>
 #include <string>
 #include <type_traits>
 
@@ -10,7 +11,84 @@
 namespace aifmm {
 
 template <class T>
+void Program<T>::add(const BoxOp<T>& op) {
+  if (op.outs.size() > 2) throw std::runtime_error("box program with more than two outputs");
+  int maxrows = 0;
+  bool uses_y = false;
+  for (const auto& o : op.outs) {
+    maxrows = std::max(maxrows, o.rows);
+    for (const auto& t : o.terms) uses_y |= t.x == nullptr;
+  }
+  if (op.m <= TILE_M && maxrows <= TILE_M) {
+    PProg<T> p{};
+    p.m = op.m;
+    p.src = op.src;
+    p.ldsrc = op.ldsrc;
+    p.src_alpha = op.src_alpha;
+    p.save = op.save;
+    p.ldsave = op.ldsave;
+    p.t0 = (int)terms_.size();
+    terms_.insert(terms_.end(), op.terms.begin(), op.terms.end());
+    p.t1 = (int)terms_.size();
+    p.nout = (int)op.outs.size();
+    for (int o = 0; o < p.nout; ++o) {
+      p.out[o] = op.outs[o].out;
+      p.ldo[o] = op.outs[o].ldo;
+      p.rows[o] = op.outs[o].rows;
+      p.u0[o] = (int)terms_.size();
+      terms_.insert(terms_.end(), op.outs[o].terms.begin(), op.outs[o].terms.end());
+      p.u1[o] = (int)terms_.size();
+    }
+    push(pa_, p);
+    return;
+  }
+  // tiled: phase 1 in row tiles into y (save or ybuf), then the outputs in
+  // row tiles reading y from memory
+  T* y = op.save ? op.save : op.ybuf;
+  const int ldy = op.save ? op.ldsave : op.ldybuf;
+  if (op.m > 0 && !y) throw std::runtime_error("tiled box program without a vector for phase 1");
+  if (uses_y && op.m == 0) throw std::runtime_error("box program term reads an empty phase-1 vector");
+  for (int r0 = 0; r0 < op.m; r0 += TILE) {
+    PProg<T> p{};
+    p.m = std::min(TILE, op.m - r0);
+    p.src = op.src ? op.src + r0 : nullptr;
+    p.ldsrc = op.ldsrc;
+    p.src_alpha = op.src_alpha;
+    p.save = y + r0;
+    p.ldsave = ldy;
+    p.t0 = (int)terms_.size();
+    for (auto t : op.terms) {
+      t.A += r0;
+      terms_.push_back(t);
+    }
+    p.t1 = (int)terms_.size();
+    push(pa_, p);
+  }
+  for (const auto& o : op.outs)
+    for (int r0 = 0; r0 < o.rows; r0 += TILE) {
+      PProg<T> p{};
+      p.m = 0;
+      p.nout = 1;
+      p.out[0] = o.out + r0;
+      p.ldo[0] = o.ldo;
+      p.rows[0] = std::min(TILE, o.rows - r0);
+      p.u0[0] = (int)terms_.size();
+      for (auto t : o.terms) {
+        t.A += r0;
+        if (!t.x) {
+          t.x = y;
+          t.ldx = ldy;
+        }
+        terms_.push_back(t);
+      }
+      p.u1[0] = (int)terms_.size();
+      push(pb_, p);
+    }
+}
+
+template <class T>
 void Program<T>::finalize(Arena& a, hipStream_t s) {
+  close();
   const int lim = prog_max_m(sizeof(T));
   if (!terms_.empty()) {
     dterms_ = a.get<PTerm<T>>(terms_.size());
@@ -60,14 +138,44 @@ static double stage_flops(const std::vector<PProg<T>>& ps, const std::vector<PTe
 }
 
 template <class T>
-void Program<T>::run(int nrhs, hipStream_t s) const {
-  const bool prof = prof_on();
+void Program<T>::launch_all(int nrhs, hipStream_t s, bool prof) const {
   for (auto& g : st_) {
     if (g.progs.empty()) continue;
     ProfScope ps(s, std::is_same<T, float2>::value ? "k_prog<c64>" : "k_prog<c128>",
                  prof ? stage_bytes(g.progs, terms_, nrhs) : 0, prof ? stage_flops(g.progs, terms_, nrhs) : 0);
     launch_prog<T>((int)g.progs.size(), g.dprogs, dterms_, nrhs, g.maxm, s);
   }
+}
+
+template <class T>
+void Program<T>::run(int nrhs, hipStream_t s) {
+  if (prof_on()) {
+    launch_all(nrhs, s, true);
+    return;
+  }
+  if (!gexec_ || gnrhs_ != nrhs) {
+    if (gexec_) HIP_OK(hipGraphExecDestroy(gexec_));
+    gexec_ = nullptr;
+    hipGraph_t g = nullptr;
+    HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+      launch_all(nrhs, s, false);
+    } catch (...) {
+      (void)hipStreamEndCapture(s, &g);
+      if (g) (void)hipGraphDestroy(g);
+      throw;
+    }
+    HIP_OK(hipStreamEndCapture(s, &g));
+    HIP_OK(hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0));
+    HIP_OK(hipGraphDestroy(g));
+    gnrhs_ = nrhs;
+  }
+  HIP_OK(hipGraphLaunch(gexec_, s));
+}
+
+template <class T>
+Program<T>::~Program() {
+  if (gexec_) (void)hipGraphExecDestroy(gexec_);
 }
 
 template <class T>

@@ -9,6 +9,12 @@
 // are column-major with nrhs columns.  Every descriptor of one stage writes
 // its own output blocks, and reads only blocks no descriptor of the same stage
 // writes, so the workgroups of a stage are independent.
+//
+// The builder takes whole box operations (BoxOp) and lowers them: a box with
+// few rows becomes one program; a tall box is cut into row tiles run by many
+// workgroups over two launches (phase 1 into a vector in memory, then the
+// outputs from it), because one workgroup alone cannot keep enough loads in
+// flight for it.
 #pragma once
 #include <vector>
 
@@ -45,43 +51,62 @@ void launch_prog(int nprog, const PProg<T>* progs, const PTerm<T>* terms, int nr
 // largest phase-1 length a stage may use
 int prog_max_m(int elem_bytes);
 
+// One box operation before lowering.
+template <class T>
+struct BoxOp {
+  struct Out {
+    T* out;
+    int ldo, rows;
+    std::vector<PTerm<T>> terms;
+  };
+  int m = 0;
+  const T* src = nullptr;
+  int ldsrc = 0;
+  double2 src_alpha = {1, 0};
+  std::vector<PTerm<T>> terms;  // phase 1
+  T* save = nullptr;            // phase-1 result (optional)
+  int ldsave = 0;
+  T* ybuf = nullptr;            // where a tiled box keeps y when save is null
+  int ldybuf = 0;
+  std::vector<Out> outs;        // phase 2 (at most 2)
+  void term(const T* A, int lda, int cols, const T* x, int ldx, double2 alpha) {
+    if (cols > 0) terms.push_back(PTerm<T>{A, x, lda, ldx, cols, alpha});
+  }
+  Out& out(T* o, int ldo, int rows) {
+    outs.push_back(Out{o, ldo, rows, {}});
+    return outs.back();
+  }
+};
+
+template <class T>
+inline void out_term(typename BoxOp<T>::Out& o, const T* A, int lda, int cols, const T* x, int ldx, double2 alpha) {
+  if (cols > 0) o.terms.push_back(PTerm<T>{A, x, lda, ldx, cols, alpha});
+}
+
 template <class T>
 class Program {
  public:
-  // Start a new stage (a launch boundary).
-  void stage() { st_.push_back(Stage{}); }
-  // Append a program to the current stage; fill its phase-1 terms with term()
-  // between begin_phase1() and end_phase1(), and each output's between
-  // begin_out(o) and end_out(o).
-  PProg<T>& add(int m) {
-    PProg<T> p{};
-    p.m = m;
-    p.src_alpha = make_double2(1, 0);
-    st_.back().progs.push_back(p);
-    if (m > st_.back().maxm) st_.back().maxm = m;
-    return st_.back().progs.back();
-  }
-  PProg<T>& cur() { return st_.back().progs.back(); }
-  void term(const T* A, int lda, int cols, const T* x, int ldx, double2 alpha) {
-    if (cols <= 0) return;
-    terms_.push_back(PTerm<T>{A, x, lda, ldx, cols, alpha});
-  }
-  void begin_phase1() { cur().t0 = (int)terms_.size(); }
-  void end_phase1() { cur().t1 = (int)terms_.size(); }
-  void begin_out(int o, T* out, int ldo, int rows) {
-    PProg<T>& p = cur();
-    p.out[o] = out;
-    p.ldo[o] = ldo;
-    p.rows[o] = rows;
-    p.u0[o] = (int)terms_.size();
-    if (o + 1 > p.nout) p.nout = o + 1;
-  }
-  void end_out(int o) { cur().u1[o] = (int)terms_.size(); }
+  // boxes taller than this are tiled
+  static constexpr int TILE_M = 64;
+  static constexpr int TILE = 16;
+
+  // Start a new stage (a launch boundary; a stage with tiled boxes becomes
+  // two launches).
+  void stage() { close(); }
+  // Add one box operation to the current stage.  A tiled box without `save`
+  // needs `ybuf` (m x nrhs).
+  void add(const BoxOp<T>& op);
 
   // Copy descriptors to the device (arena-owned memory).
   void finalize(Arena& a, hipStream_t s);
-  // Launch every stage in order on s (asynchronous).
-  void run(int nrhs, hipStream_t s) const;
+  // Launch every stage in order on s (asynchronous).  The launch sequence is
+  // captured once per nrhs into a HIP graph and replayed; with kernel
+  // profiling on, the launches are issued one by one instead.
+  void run(int nrhs, hipStream_t s);
+  Program() = default;
+  Program(const Program&) = delete;
+  Program& operator=(const Program&) = delete;
+  ~Program();
   size_t stages() const { return st_.size(); }
   size_t nterms() const { return terms_.size(); }
   // algorithmic bytes of one run at nrhs (matrices once, vectors per column)
@@ -97,6 +122,22 @@ class Program {
   std::vector<Stage> st_;
   std::vector<PTerm<T>> terms_;
   PTerm<T>* dterms_ = nullptr;
+  hipGraphExec_t gexec_ = nullptr;
+  int gnrhs_ = -1;
+  void launch_all(int nrhs, hipStream_t s, bool prof) const;
+  // pending launches of the open stage: A = fused programs and tile phase 1,
+  // B = tile phase 2
+  Stage pa_, pb_;
+  void push(Stage& g, const PProg<T>& p) {
+    g.progs.push_back(p);
+    if (p.m > g.maxm) g.maxm = p.m;
+  }
+  void close() {
+    if (!pa_.progs.empty()) st_.push_back(pa_);
+    if (!pb_.progs.empty()) st_.push_back(pb_);
+    pa_ = Stage{};
+    pb_ = Stage{};
+  }
 };
 
 }  // namespace aifmm

@@ -153,7 +153,7 @@ def test_multi_rhs_and_determinism(pair):
         x1, _ = g.solve(B[:, c])
         x2, _ = g.solve(B[:, c])
         assert np.array_equal(x1, x2)
-        assert rel(X[:, c], x1) < 1e-13
+        assert rel(X[:, c], x1) < 1e-11  # summation order differs between the paths
 
 
 @pytest.mark.parametrize("qc,kap,bumps", [(0, 10.0, None), (6, 20.0, BUMPS2)])

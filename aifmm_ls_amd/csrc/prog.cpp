@@ -3,6 +3,7 @@
 #include "prog.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <string>
 #include <type_traits>
 
@@ -147,10 +148,20 @@ void Program<T>::launch_all(int nrhs, hipStream_t s, bool prof) const {
   }
 }
 
+// AIFMM_GRAPH=0 issues the launches one by one instead of replaying a graph
+// (for tools that do not follow stream capture)
+static bool graphs_enabled() {
+  static const bool on = [] {
+    const char* e = getenv("AIFMM_GRAPH");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
 template <class T>
 void Program<T>::run(int nrhs, hipStream_t s) {
-  if (prof_on()) {
-    launch_all(nrhs, s, true);
+  if (prof_on() || !graphs_enabled()) {
+    launch_all(nrhs, s, prof_on());
     return;
   }
   if (!gexec_ || gnrhs_ != nrhs) {

@@ -64,6 +64,7 @@ def lib():
         L.aifmm_ls_selftest.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp,
                                         ctypes.POINTER(ctypes.c_int), ctypes.c_double]
         L.aifmm_ls_profile_enable.argtypes = [ctypes.c_int]
+        L.aifmm_ls_set_refinement.argtypes = [P, ctypes.c_int]
         L.aifmm_ls_profile_read.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_char_p, ctypes.c_int]
         _lib = L
     return _lib
@@ -201,6 +202,11 @@ class LSProblem:
         st = np.zeros(8)
         _chk(lib().aifmm_ls_solve_resident(self.h, tol, max_refine, _dp(st)))
         return st
+
+    def set_refinement(self, method):
+        """0: flexible GMRES with the factorisation as preconditioner (default);
+        1: plain iterative refinement."""
+        _chk(lib().aifmm_ls_set_refinement(self.h, int(method)))
 
     def get_solution(self):
         x = np.zeros(2 * self.N * self._nrhs)

@@ -11,6 +11,7 @@
 #include "../../include/aifmm_ls.h"
 #include "factor.h"
 #include "prof.h"
+#include "krylov.h"
 #include "h2.h"
 
 using namespace aifmm;
@@ -20,10 +21,9 @@ struct aifmm_ls_problem {
   std::unique_ptr<Factor<double2>> f64;
   std::unique_ptr<Factor<float2>> f32;
   std::unique_ptr<H2Op> h2;
+  std::unique_ptr<Refiner> ref;
+  int refine_method = 0;
   int precision = -1;
-  double2 *tmp = nullptr, *res = nullptr, *cor = nullptr;
-  double* part = nullptr;       // per-block partial squared norms
-  std::vector<double> bnorm;    // column norms of the staged RHS
   Arena sc{size_t(16) << 20};
   Arena io{size_t(64) << 20};
   double2* rhs = nullptr;
@@ -171,8 +171,6 @@ int aifmm_ls_set_rhs(aifmm_ls_problem* h, int nrhs, const double* b) {
     if (nrhs < 1) throw std::runtime_error("nrhs must be >= 1");
     int64_t N = h->pb->N;
     h->io.reset();
-    h->tmp = h->res = h->cor = nullptr;
-    h->part = nullptr;
     h->rhs = h->io.get<double2>(N * nrhs);
     h->sol = h->io.get<double2>(N * nrhs);
     h->nrhs = nrhs;
@@ -184,60 +182,34 @@ int aifmm_ls_solve_resident(aifmm_ls_problem* h, double tol, int max_refine, dou
   GUARD({
     if (h->precision < 0) throw std::runtime_error("factor first");
     if (!h->rhs) throw std::runtime_error("set_rhs first");
-    const int64_t N = h->pb->N;
     const int nr = h->nrhs;
-    hipStream_t st = h->pb->stream;
-    auto S = [&](const double2* b, double2* x) {
+    auto M = [&](const double2* b, double2* x) {
       if (h->precision == 0) h->f64->solve(b, x, nr);
       else h->f32->solve(b, x, nr);
     };
-    const int nblk = resid_blocks(N);
-    std::vector<double> part((size_t)nblk * nr);
-    // column norms of r = b - ax from the per-block partial sums (fixed order)
-    auto norms = [&](const double2* ax, double2* r, std::vector<double>& out) {
-      launch_resid(h->rhs, ax, r, N, nr, h->part, st);
-      HIP_OK(hipMemcpyAsync(part.data(), h->part, part.size() * 8, hipMemcpyDeviceToHost, st));
-      HIP_OK(hipStreamSynchronize(st));
-      out.assign(nr, 0);
-      for (int c = 0; c < nr; ++c) {
-        double s = 0;
-        for (int b = 0; b < nblk; ++b) s += part[(size_t)c * nblk + b];
-        out[c] = std::sqrt(s);
-      }
-    };
     double t0 = now_s();
-    S(h->rhs, h->sol);
-    int steps = 0;
-    double relres = -1;
+    RefineResult R;
     if (max_refine > 0) {
-      // iterative refinement against the FP64 hierarchical operator
       if (!h->h2) h->h2.reset(new H2Op(*h->pb));
-      if (!h->tmp) {
-        h->tmp = h->io.get<double2>(N * nr);
-        h->res = h->io.get<double2>(N * nr);
-        h->cor = h->io.get<double2>(N * nr);
-        h->part = h->io.get<double>((size_t)nblk * nr);
-        norms(nullptr, nullptr, h->bnorm);
-      }
-      std::vector<double> rn;
-      for (int it = 0;; ++it) {
-        h->h2->apply(h->sol, h->tmp, nr);
-        norms(h->tmp, h->res, rn);
-        relres = 0;
-        for (int c = 0; c < nr; ++c) relres = std::max(relres, h->bnorm[c] > 0 ? rn[c] / h->bnorm[c] : rn[c]);
-        if (relres <= tol || it >= max_refine) break;
-        S(h->res, h->cor);
-        launch_axpy(h->sol, h->cor, N * nr, st);
-        ++steps;
-      }
+      if (!h->ref) h->ref.reset(new Refiner(*h->pb, *h->h2));
+      R = h->ref->run(M, h->rhs, h->sol, nr, tol, max_refine, h->refine_method);
+    } else {
+      M(h->rhs, h->sol);
     }
-    HIP_OK(hipStreamSynchronize(st));
+    HIP_OK(hipStreamSynchronize(h->pb->stream));
     h->t_solve = now_s() - t0;
     if (stats) {
       stats[0] = h->t_solve;
-      stats[1] = steps;
-      stats[2] = relres;
+      stats[1] = R.steps;
+      stats[2] = R.relres;
     }
+  })
+}
+
+int aifmm_ls_set_refinement(aifmm_ls_problem* h, int method) {
+  GUARD({
+    if (method != 0 && method != 1) throw std::runtime_error("refinement method must be 0 (FGMRES) or 1 (Richardson)");
+    h->refine_method = method;
   })
 }
 

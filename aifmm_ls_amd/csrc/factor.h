@@ -20,45 +20,73 @@ struct FactorStats {
   int recompressions = 0;  // boxes re-based after fill-in
 };
 
-// Factorisation of a square block for repeated solves: column-pivoted QR in
-// complex128 (ColPivHouseholderQR, as the reference), LU with partial
-// pivoting in complex64 (row operations keep the relative accuracy of the
-// badly scaled rows that low-contrast boxes produce, which Householder
-// reflections in float do not).
+// Factorisation of a square block for repeated solves.  complex128: column-
+// pivoted QR, as the reference (ColPivHouseholderQR).  complex64: the block is
+// widened to an FP64 copy and LU-factored in FP64 (the pivot blocks of the
+// complex64 factorisation are the one place its rounding would be amplified
+// by their conditioning); solves widen the right-hand sides, solve in FP64
+// and round the result to complex64.
 template <class T>
 struct DFact {
-  QRFact<T> q;  // q.A is the factored block in both cases
+  QRFact<T> q;                  // complex128: q.A is the factored block
+  Blk<double2> a64;             // complex64: FP64 LU factors
   int* piv = nullptr;
 };
 
 template <class T>
-struct DenseOps {
-  static constexpr bool LU = std::is_same<T, float2>::value;
-  QRBatch<T> qf;
-  QRSolveBatch<T> qs;
-  LUBatch<T> lf;
-  SolveBatch<T> ls;
-  void factor(Arena& ar, Arena& sc, DFact<T>& f, const Blk<T>& A) {
-    if (LU) {
-      f.q.A = A;
-      f.piv = ar.get<int>(A.r > 0 ? A.r : 1);
-      lf.add(A, f.piv);
-    } else {
-      f.q = new_qr(ar, A);
-      qf.add(f.q, sc);
-    }
+struct DenseOps;
+
+template <>
+struct DenseOps<double2> {
+  QRBatch<double2> qf;
+  QRSolveBatch<double2> qs;
+  void factor(Arena& ar, Arena& sc, DFact<double2>& f, const Blk<double2>& A) {
+    f.q = new_qr(ar, A);
+    qf.add(f.q, sc);
   }
-  void solve(Arena& sc, const DFact<T>& f, const Blk<T>& B) {
-    if (LU) ls.add(f.q.A, f.piv, B);
-    else qs.add(f.q, B, sc);
+  void solve(Arena& sc, const DFact<double2>& f, const Blk<double2>& B) { qs.add(f.q, B, sc); }
+  void run_factor(Arena& sc, hipStream_t s) { qf.run(sc, s); }
+  void run_solve(Arena& sc, hipStream_t s) { qs.run(sc, s); }
+};
+
+template <>
+struct DenseOps<float2> {
+  std::vector<WidenDesc> wf, ws;   // widen for factor / for solve
+  std::vector<CvtDesc<float2>> nb; // narrow solve results
+  LUBatch<double2> lf;
+  SolveBatch<double2> ls;
+  static Blk<double2> wide(Arena& a, const Blk<float2>& b) { return new_blk<double2>(a, b.r, b.c); }
+  static void need_contig(const Blk<float2>& b) {
+    if (b.size() && b.ld != b.r) throw std::runtime_error("FP64 pivot block path needs contiguous blocks");
+  }
+  void factor(Arena& ar, Arena& sc, DFact<float2>& f, const Blk<float2>& A) {
+    need_contig(A);
+    f.q.A = A;
+    f.a64 = wide(ar, A);
+    f.piv = ar.get<int>(A.r > 0 ? A.r : 1);
+    if (!A.size()) return;
+    wf.push_back(WidenDesc{A.p, f.a64.p, (int64_t)A.size()});
+    lf.add(f.a64, f.piv);
+  }
+  void solve(Arena& sc, const DFact<float2>& f, const Blk<float2>& B) {
+    need_contig(B);
+    if (!f.a64.r || !B.c) return;
+    Blk<double2> b64 = wide(sc, B);
+    ws.push_back(WidenDesc{B.p, b64.p, (int64_t)B.size()});
+    ls.add(f.a64, f.piv, b64);
+    nb.push_back(CvtDesc<float2>{b64.p, B.p, (int64_t)B.size()});
   }
   void run_factor(Arena& sc, hipStream_t s) {
-    qf.run(sc, s);
+    if (!wf.empty()) launch_widen((int)wf.size(), upload(sc, wf, s), s);
+    wf.clear();
     lf.run(sc, s);
   }
   void run_solve(Arena& sc, hipStream_t s) {
-    qs.run(sc, s);
+    if (!ws.empty()) launch_widen((int)ws.size(), upload(sc, ws, s), s);
+    ws.clear();
     ls.run(sc, s);
+    if (!nb.empty()) launch_cvt<float2>((int)nb.size(), upload(sc, nb, s), s);
+    nb.clear();
   }
 };
 

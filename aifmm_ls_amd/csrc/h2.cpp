@@ -13,32 +13,17 @@ namespace aifmm {
 
 static const double2 ONE = {1.0, 0.0};
 
-void col_norms(Arena& sc, const double2* v, int64_t n, int nrhs, double* out, hipStream_t s) {
-  std::vector<CopyDesc<double2>> d(nrhs);
-  for (int c = 0; c < nrhs; ++c) {
-    d[c] = CopyDesc<double2>{};
-    d[c].src = v + (size_t)n * c;
-    d[c].nr = (int)n;
-    d[c].nc = 1;
-    d[c].lds = (int)n;
-  }
-  auto* dd = upload(sc, d, s);
-  double* dn = sc.get<double>(nrhs);
-  launch_fro<double2>(nrhs, dd, dn, s);
-  HIP_OK(hipMemcpyAsync(out, dn, 8 * nrhs, hipMemcpyDeviceToHost, s));
-  HIP_OK(hipStreamSynchronize(s));
-  sc.reset();
-}
-
-void H2Op::build(const double2* x, double2* y, int nrhs) {
+void H2Op::build(int nrhs) {
   Problem& P = pb;
   const int L = P.L, R = P.R;
   const int N = (int)P.N;
   ws.reset();
   plan.reset(new Program<double2>());
-  px = x;
-  py = y;
   cap = nrhs;
+  xin = ws.get<double2>((size_t)N * nrhs);
+  yout = ws.get<double2>((size_t)N * nrhs);
+  const double2* x = xin;
+  double2* y = yout;
   std::vector<std::vector<int>> off(L + 1);
   std::vector<Blk<double2>> Mu(L + 1), Lo(L + 1);
   for (int j = 2; j <= L; ++j) {
@@ -124,8 +109,11 @@ void H2Op::build(const double2* x, double2* y, int nrhs) {
 
 void H2Op::apply(const double2* x, double2* y, int nrhs) {
   if (!pb.assembled) pb.assemble();
-  if (!plan || x != px || y != py || nrhs > cap) build(x, y, nrhs);
+  if (!plan || nrhs > cap) build(nrhs);
+  const size_t bytes = (size_t)pb.N * nrhs * sizeof(double2);
+  HIP_OK(hipMemcpyAsync(xin, x, bytes, hipMemcpyDeviceToDevice, pb.stream));
   plan->run(nrhs, pb.stream);
+  HIP_OK(hipMemcpyAsync(y, yout, bytes, hipMemcpyDeviceToDevice, pb.stream));
 }
 
 size_t H2Op::bytes(int nrhs) const {

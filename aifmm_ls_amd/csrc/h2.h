@@ -14,8 +14,9 @@ class H2Op {
  public:
   explicit H2Op(Problem& pb) : pb(pb) {}
   // y = A_H x ; x, y device arrays N x nrhs (column-major, complex128).
-  // Asynchronous on the problem's stream.  The launch program is built for
-  // one (x, y) pair and rebuilt when either pointer or a larger nrhs appears.
+  // Asynchronous on the problem's stream.  The launch program works on its
+  // own input and output buffers (x and y are copied in and out), built once
+  // per right-hand-side capacity.
   void apply(const double2* x, double2* y, int nrhs);
   // algorithmic bytes of one application (operator blocks + vectors)
   size_t bytes(int nrhs) const;
@@ -24,13 +25,10 @@ class H2Op {
   Problem& pb;
   Arena ws{size_t(64) << 20};
   std::unique_ptr<Program<double2>> plan;
-  const double2* px = nullptr;
-  double2* py = nullptr;
+  double2* xin = nullptr;
+  double2* yout = nullptr;
   int cap = 0;
-  void build(const double2* x, double2* y, int nrhs);
+  void build(int nrhs);
 };
-
-// column 2-norms of an N x nrhs device array (host result)
-void col_norms(Arena& sc, const double2* v, int64_t n, int nrhs, double* out, hipStream_t s);
 
 }  // namespace aifmm

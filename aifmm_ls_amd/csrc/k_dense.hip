@@ -529,6 +529,15 @@ void launch_cvt(int nb, const CvtDesc<T>* d, hipStream_t s) {
   hipLaunchKernelGGL(k_cvt<T>, dim3(nb), dim3(NT), 0, s, d);
   HIP_OK(hipGetLastError());
 }
+__global__ __launch_bounds__(NT) void k_widen(const WidenDesc* __restrict__ ds) {
+  const WidenDesc d = ds[blockIdx.x];
+  for (int64_t i = threadIdx.x; i < d.n; i += NT) d.dst[i] = mk((double)d.src[i].x, (double)d.src[i].y);
+}
+void launch_widen(int nb, const WidenDesc* d, hipStream_t s) {
+  if (!nb) return;
+  hipLaunchKernelGGL(k_widen, dim3(nb), dim3(NT), 0, s, d);
+  HIP_OK(hipGetLastError());
+}
 void launch_d2f(const double2* src, float2* dst, int64_t n, hipStream_t s) {
   if (!n) return;
   hipLaunchKernelGGL(k_d2f, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, src, dst, n);

@@ -45,6 +45,72 @@ __global__ __launch_bounds__(VNT) void k_axpy(double2* __restrict__ y, const dou
   }
 }
 
+// Dots of w against nv basis vectors, per right-hand side:
+// partial[((c * nv) + i) * gridDim.x + blockIdx.x] = sum over the block's rows
+// of conj(V_i[n, c]) w[n, c];  V_i = V + i * vstride (each N x nrhs, ld N).
+__global__ __launch_bounds__(VNT) void k_multidot(const double2* __restrict__ V, int64_t vstride, int nv,
+                                                  const double2* __restrict__ w, int64_t n,
+                                                  double2* __restrict__ partial) {
+  const int c = blockIdx.y;
+  const size_t base = (size_t)n * c;
+  __shared__ double2 sh[VNT / 64];
+  for (int i = 0; i < nv; ++i) {
+    const double2* v = V + (size_t)i * vstride + base;
+    double2 s = mk(0.0, 0.0);
+    for (int64_t r = (int64_t)blockIdx.x * VNT + threadIdx.x; r < n; r += (int64_t)gridDim.x * VNT) {
+      double2 a = v[r], b = w[base + r];
+      s.x += a.x * b.x + a.y * b.y;
+      s.y += a.x * b.y - a.y * b.x;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      s.x += __shfl_xor(s.x, o);
+      s.y += __shfl_xor(s.y, o);
+    }
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double2 t = mk(0.0, 0.0);
+      for (int q = 0; q < VNT / 64; ++q) t.x += sh[q].x, t.y += sh[q].y;
+      partial[((size_t)c * nv + i) * gridDim.x + blockIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// w[n, c] = beta_c * w[n, c] + sum_i coef[c * nv + i] * V_i[n, c]
+__global__ __launch_bounds__(VNT) void k_lincomb(double2* __restrict__ w, const double2* __restrict__ V,
+                                                 int64_t vstride, int nv, const double2* __restrict__ coef,
+                                                 const double2* __restrict__ beta, int64_t n) {
+  const int c = blockIdx.y;
+  const size_t base = (size_t)n * c;
+  const double2 bt = beta[c];
+  for (int64_t r = (int64_t)blockIdx.x * VNT + threadIdx.x; r < n; r += (int64_t)gridDim.x * VNT) {
+    double2 acc = cmul(bt, w[base + r]);
+    for (int i = 0; i < nv; ++i) {
+      const double2 a = coef[(size_t)c * nv + i], v = V[(size_t)i * vstride + base + r];
+      acc.x += a.x * v.x - a.y * v.y;
+      acc.y += a.x * v.y + a.y * v.x;
+    }
+    w[base + r] = acc;
+  }
+}
+
+void launch_multidot(const double2* V, int64_t vstride, int nv, const double2* w, int64_t n, int nrhs,
+                     double2* partial, hipStream_t s) {
+  if (!n || !nrhs || !nv) return;
+  hipLaunchKernelGGL(k_multidot, dim3(resid_blocks(n), nrhs), dim3(VNT), 0, s, V, vstride, nv, w, n, partial);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_lincomb(double2* w, const double2* V, int64_t vstride, int nv, const double2* coef, const double2* beta,
+                    int64_t n, int nrhs, hipStream_t s) {
+  if (!n || !nrhs) return;
+  int64_t b = (n + VNT - 1) / VNT;
+  hipLaunchKernelGGL(k_lincomb, dim3((unsigned)(b < 1024 ? b : 1024), nrhs), dim3(VNT), 0, s, w, V, vstride, nv, coef,
+                     beta, n);
+  HIP_OK(hipGetLastError());
+}
+
 int resid_blocks(int64_t n) {
   int64_t b = (n + VNT - 1) / VNT;
   return (int)(b < 1024 ? (b < 1 ? 1 : b) : 1024);

@@ -146,6 +146,13 @@ struct CvtDesc {
 };
 template <class T>
 void launch_cvt(int nb, const CvtDesc<T>* d, hipStream_t s);
+// dst[i] = (double2) src[i], i < n (complex64 blocks -> FP64 work copies)
+struct WidenDesc {
+  const float2* src;
+  double2* dst;
+  int64_t n;
+};
+void launch_widen(int nb, const WidenDesc* d, hipStream_t s);
 
 template <class T>
 void launch_gemm(int ntiles, const TileTask* tiles, const GemmDesc<T>* d, hipStream_t s);
@@ -167,6 +174,13 @@ int resid_blocks(int64_t n);
 void launch_resid(const double2* b, const double2* ax, double2* r, int64_t n, int nrhs, double* partial,
                   hipStream_t s);
 void launch_axpy(double2* y, const double2* x, int64_t n, hipStream_t s);
+// Krylov vectors (N x nrhs each, ld N; basis vector i at V + i * vstride):
+// per-block partial dots conj(V_i) . w (resid_blocks(n) blocks per (i, column)),
+// and w = beta_c w + sum_i coef[c * nv + i] V_i
+void launch_multidot(const double2* V, int64_t vstride, int nv, const double2* w, int64_t n, int nrhs,
+                     double2* partial, hipStream_t s);
+void launch_lincomb(double2* w, const double2* V, int64_t vstride, int nv, const double2* coef, const double2* beta,
+                    int64_t n, int nrhs, hipStream_t s);
 void launch_d2f(const double2* src, float2* dst, int64_t n, hipStream_t s);
 void launch_f2d(const float2* src, double2* dst, int64_t n, hipStream_t s);
 

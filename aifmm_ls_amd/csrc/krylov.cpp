@@ -1,0 +1,189 @@
+#include "krylov.h"
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+
+namespace aifmm {
+
+using cplx = std::complex<double>;
+
+void Refiner::norms(const double2* v, int nrhs, std::vector<double>& out) {
+  const int64_t N = pb_.N;
+  const int nblk = resid_blocks(N);
+  std::vector<double> part((size_t)nblk * nrhs);
+  double* dp = (double*)dpart_;
+  launch_resid(v, nullptr, nullptr, N, nrhs, dp, pb_.stream);
+  HIP_OK(hipMemcpyAsync(part.data(), dp, part.size() * 8, hipMemcpyDeviceToHost, pb_.stream));
+  HIP_OK(hipStreamSynchronize(pb_.stream));
+  out.assign(nrhs, 0);
+  for (int c = 0; c < nrhs; ++c) {
+    double s = 0;
+    for (int b = 0; b < nblk; ++b) s += part[(size_t)c * nblk + b];
+    out[c] = std::sqrt(s);
+  }
+}
+
+void Refiner::dots(const double2* V, int64_t vstride, int nv, const double2* w, int nrhs, std::vector<double2>& out) {
+  const int64_t N = pb_.N;
+  const int nblk = resid_blocks(N);
+  std::vector<double2> part((size_t)nblk * nv * nrhs);
+  launch_multidot(V, vstride, nv, w, N, nrhs, dpart_, pb_.stream);
+  HIP_OK(hipMemcpyAsync(part.data(), dpart_, part.size() * 16, hipMemcpyDeviceToHost, pb_.stream));
+  HIP_OK(hipStreamSynchronize(pb_.stream));
+  out.assign((size_t)nv * nrhs, make_double2(0, 0));
+  for (int c = 0; c < nrhs; ++c)
+    for (int i = 0; i < nv; ++i) {
+      double2 s = make_double2(0, 0);
+      for (int b = 0; b < nblk; ++b) {
+        const double2 p = part[((size_t)c * nv + i) * nblk + b];
+        s.x += p.x;
+        s.y += p.y;
+      }
+      out[(size_t)c * nv + i] = s;
+    }
+}
+
+void Refiner::lincomb(double2* w, const double2* V, int64_t vstride, int nv, const std::vector<double2>& coef,
+                      const std::vector<double2>& beta, int nrhs) {
+  // the host vectors are copied before this returns (pageable source)
+  if (nv) HIP_OK(hipMemcpyAsync(dcoef_, coef.data(), (size_t)nv * nrhs * 16, hipMemcpyHostToDevice, pb_.stream));
+  HIP_OK(hipMemcpyAsync(dbeta_, beta.data(), (size_t)nrhs * 16, hipMemcpyHostToDevice, pb_.stream));
+  launch_lincomb(w, V, vstride, nv, dcoef_, dbeta_, pb_.N, nrhs, pb_.stream);
+  HIP_OK(hipStreamSynchronize(pb_.stream));
+}
+
+RefineResult Refiner::run(const Precond& M, const double2* b, double2* x, int nr, double tol, int maxit, int method) {
+  const int64_t N = pb_.N;
+  hipStream_t s = pb_.stream;
+  const size_t vb = (size_t)N * nr;
+  RefineResult R;
+  M(b, x);
+  if (maxit <= 0) return R;
+  // restart length: up to 20, the Krylov vectors within 4 GiB
+  int m = std::min(maxit, 20);
+  const size_t lim = (size_t(4) << 30) / (vb * 32);
+  m = std::max(1, std::min<int>(m, (int)std::max<size_t>(lim, 2) - 1));
+  ws_.reset();
+  double2* V = ws_.get<double2>((size_t)(m + 1) * vb);
+  double2* Z = ws_.get<double2>((size_t)m * vb);
+  double2* r = ws_.get<double2>(vb);
+  dpart_ = ws_.get<double2>((size_t)resid_blocks(N) * (m + 1) * nr);
+  dcoef_ = ws_.get<double2>((size_t)(m + 1) * nr);
+  dbeta_ = ws_.get<double2>(nr);
+
+  std::vector<double> bn, rn;
+  norms(b, nr, bn);
+  auto relmax = [&](const std::vector<double>& v) {
+    double q = 0;
+    for (int c = 0; c < nr; ++c) q = std::max(q, bn[c] > 0 ? v[c] / bn[c] : v[c]);
+    return q;
+  };
+  const double2 one = make_double2(1, 0);
+  for (;;) {
+    // true residual r = b - A x
+    A_.apply(x, r, nr);
+    {
+      const int nblk = resid_blocks(N);
+      std::vector<double> part((size_t)nblk * nr);
+      launch_resid(b, r, r, N, nr, (double*)dpart_, s);
+      HIP_OK(hipMemcpyAsync(part.data(), dpart_, part.size() * 8, hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
+      rn.assign(nr, 0);
+      for (int c = 0; c < nr; ++c) {
+        double q = 0;
+        for (int k = 0; k < nblk; ++k) q += part[(size_t)c * nblk + k];
+        rn[c] = std::sqrt(q);
+      }
+    }
+    R.relres = relmax(rn);
+    if (R.relres <= tol || R.steps >= maxit) break;
+    if (method == 1) {  // Richardson: x += M r
+      M(r, Z);
+      launch_axpy(x, Z, (int64_t)vb, s);
+      ++R.steps;
+      continue;
+    }
+    // FGMRES cycle
+    HIP_OK(hipMemcpyAsync(V, r, vb * 16, hipMemcpyDeviceToDevice, s));
+    std::vector<double2> sc(nr);
+    for (int c = 0; c < nr; ++c) sc[c] = make_double2(rn[c] > 0 ? 1.0 / rn[c] : 0.0, 0);
+    lincomb(V, nullptr, 0, 0, {}, sc, nr);
+    std::vector<cplx> H((size_t)(m + 1) * m * nr, 0.0), g((size_t)(m + 1) * nr, 0.0), cs((size_t)m * nr, 0.0),
+        sn((size_t)m * nr, 0.0);
+    auto Hc = [&](int c, int i, int j) -> cplx& { return H[((size_t)c * (m + 1) + i) * m + j]; };
+    for (int c = 0; c < nr; ++c) g[(size_t)c * (m + 1)] = rn[c];
+    int k = 0;
+    for (int j = 0; j < m && R.steps < maxit; ++j) {
+      double2* vj = V + (size_t)j * vb;
+      double2* zj = Z + (size_t)j * vb;
+      double2* w = V + (size_t)(j + 1) * vb;
+      M(vj, zj);
+      A_.apply(zj, w, nr);
+      ++R.steps;
+      std::vector<double2> h, h2, neg((size_t)(j + 1) * nr);
+      std::vector<double2> keep(nr, one);
+      dots(V, (int64_t)vb, j + 1, w, nr, h);
+      for (size_t q = 0; q < neg.size(); ++q) neg[q] = make_double2(-h[q].x, -h[q].y);
+      lincomb(w, V, (int64_t)vb, j + 1, neg, keep, nr);
+      dots(V, (int64_t)vb, j + 1, w, nr, h2);
+      for (size_t q = 0; q < neg.size(); ++q) neg[q] = make_double2(-h2[q].x, -h2[q].y);
+      lincomb(w, V, (int64_t)vb, j + 1, neg, keep, nr);
+      std::vector<double> hn;
+      norms(w, nr, hn);
+      for (int c = 0; c < nr; ++c) sc[c] = make_double2(hn[c] > 0 ? 1.0 / hn[c] : 0.0, 0);
+      lincomb(w, nullptr, 0, 0, {}, sc, nr);
+      double worst = 0;
+      for (int c = 0; c < nr; ++c) {
+        for (int i = 0; i <= j; ++i) {
+          const double2 a = h[(size_t)c * (j + 1) + i], e = h2[(size_t)c * (j + 1) + i];
+          Hc(c, i, j) = cplx(a.x + e.x, a.y + e.y);
+        }
+        Hc(c, j + 1, j) = hn[c];
+        // previous rotations
+        for (int i = 0; i < j; ++i) {
+          const cplx ci = cs[(size_t)c * m + i], si = sn[(size_t)c * m + i];
+          const cplx a = Hc(c, i, j), bb = Hc(c, i + 1, j);
+          Hc(c, i, j) = ci * a + si * bb;
+          Hc(c, i + 1, j) = -std::conj(si) * a + ci * bb;
+        }
+        // new rotation annihilating H(j+1, j)
+        const cplx h1 = Hc(c, j, j), h2v = Hc(c, j + 1, j);
+        const double t = std::sqrt(std::norm(h1) + std::norm(h2v));
+        cplx cj = 0, sj = 1;
+        if (t > 0 && std::abs(h1) > 0) {
+          cj = std::abs(h1) / t;
+          sj = (h1 / std::abs(h1)) * std::conj(h2v) / t;
+        }
+        cs[(size_t)c * m + j] = cj;
+        sn[(size_t)c * m + j] = sj;
+        Hc(c, j, j) = cj * h1 + sj * h2v;
+        Hc(c, j + 1, j) = 0;
+        cplx& gj = g[(size_t)c * (m + 1) + j];
+        g[(size_t)c * (m + 1) + j + 1] = -std::conj(sj) * gj;
+        gj = cj * gj;
+        const double est = std::abs(g[(size_t)c * (m + 1) + j + 1]);
+        worst = std::max(worst, bn[c] > 0 ? est / bn[c] : est);
+      }
+      k = j + 1;
+      if (worst <= 0.5 * tol) break;
+    }
+    // y = H[:k, :k]^{-1} g[:k] per column; x += Z y
+    std::vector<double2> y((size_t)k * nr);
+    for (int c = 0; c < nr; ++c) {
+      std::vector<cplx> yy(k, 0.0);
+      for (int i = k - 1; i >= 0; --i) {
+        cplx acc = g[(size_t)c * (m + 1) + i];
+        for (int l = i + 1; l < k; ++l) acc -= Hc(c, i, l) * yy[l];
+        const cplx d = Hc(c, i, i);
+        yy[i] = std::abs(d) > 0 ? acc / d : 0.0;
+      }
+      for (int i = 0; i < k; ++i) y[(size_t)c * k + i] = make_double2(yy[i].real(), yy[i].imag());
+    }
+    lincomb(x, Z, (int64_t)vb, k, y, std::vector<double2>(nr, one), nr);
+  }
+  HIP_OK(hipStreamSynchronize(s));
+  return R;
+}
+
+}  // namespace aifmm

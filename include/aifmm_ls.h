@@ -68,6 +68,11 @@ int aifmm_ls_solve(aifmm_ls_problem* h, int nrhs, const double* b, double* x, do
 int aifmm_ls_set_rhs(aifmm_ls_problem* h, int nrhs, const double* b);
 int aifmm_ls_solve_resident(aifmm_ls_problem* h, double tol, int max_refine, double* stats);
 int aifmm_ls_get_solution(aifmm_ls_problem* h, double* x);
+/* Refinement method for max_refine > 0 (no reference counterpart: the reference
+ * solves once, FMM2DTree_find_error.hpp:2407-2532): 0 = flexible GMRES with the
+ * factorisation as right preconditioner (default), 1 = plain iterative
+ * refinement x += M (b - A_H x).  Both use the FP64 operator of aifmm_ls_matvec. */
+int aifmm_ls_set_refinement(aifmm_ls_problem* h, int method);
 /* FP64 product y = A_H x with the assembled hierarchical operator (leaf P2P plus the
  * nested L2P M2L P2M far field), the operator the factorisation inverts; N x nrhs. */
 int aifmm_ls_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y);

@@ -40,9 +40,28 @@ def build(verbose=True):
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stderr[-4000:])
     os.replace(tmp, LIB)
+    build_drivers()
     if verbose:
         print("built", LIB)
     return LIB
+
+
+DRIVERS = ["testFMM2D_find_error"]
+ROOT = os.path.dirname(HERE)
+
+
+def build_drivers():
+    """Host C++ drivers over the source-compatible headers in include/aifmm,
+    linked against the in-tree library (rpath $ORIGIN)."""
+    for name in DRIVERS:
+        out = os.path.join(OUT, name)
+        cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"),
+               os.path.join(HERE, "drivers", name + ".cpp"), "-o", out + ".tmp", "-L", OUT, "-laifmm_ls",
+               "-Wl,-rpath,$ORIGIN", "-Wl,-rpath-link,/opt/rocm/lib"]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"driver {name} failed to build:\n" + r.stderr[-4000:])
+        os.replace(out + ".tmp", out)
 
 
 if __name__ == "__main__":

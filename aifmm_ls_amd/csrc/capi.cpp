@@ -127,6 +127,14 @@ int aifmm_ls_rhs_planewave(aifmm_ls_problem* h, double theta, double* b) {
 
 int aifmm_ls_assemble(aifmm_ls_problem* h) { GUARD({ h->pb->assemble(); }) }
 
+int aifmm_ls_set_tolerance(aifmm_ls_problem* h, int tol_pow) {
+  GUARD({
+    if (h->pb->assembled) throw std::runtime_error("the tolerance is fixed once the operator is assembled");
+    if (tol_pow < 1 || tol_pow > 15) throw std::runtime_error("tol_pow must be in [1, 15]");
+    h->pb->P.tol_pow = tol_pow;
+  })
+}
+
 int aifmm_ls_ranks(aifmm_ls_problem* h, int level, int* ranks) {
   GUARD({
     Problem& P = *h->pb;

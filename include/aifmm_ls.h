@@ -48,6 +48,9 @@ int aifmm_ls_entries(aifmm_ls_problem* h, int64_t n, const int64_t* I, const int
 /* userKernel::RHSFunction with incidence direction (cos th, sin th) (LS_functions.hpp:113-120) */
 int aifmm_ls_rhs_planewave(aifmm_ls_problem* h, double theta, double* b);
 
+/* TOL_POW of the FMM2DTree ctor (FMM2DTree_find_error.hpp:98-112): the ACA / RRQR
+ * tolerance 10^-tol_pow, settable until assemble (create takes an initial value). */
+int aifmm_ls_set_tolerance(aifmm_ls_problem* h, int tol_pow);
 /* FMM2DTree::getNodes + assemble_M2L + initialise_P2P_Leaf_Level
  * (FMM2DTree_find_error.hpp:1137-1361, 1729-1741) */
 int aifmm_ls_assemble(aifmm_ls_problem* h);

@@ -1,0 +1,89 @@
+"""The find_error driver built on the source-compatible FMM2DTree / userkernel
+headers (include/aifmm), and the rhs_x/*.bin vector format it shares with the
+reference's generator (cereal PortableBinary of a std::vector<double>:
+endianness byte, uint64 count, doubles)."""
+import os
+import struct
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "aifmm_ls_amd", "_lib", "testFMM2D_find_error")
+
+
+def write_vec(path, v):
+    v = np.ascontiguousarray(v, np.float64)
+    with open(path, "wb") as f:
+        f.write(struct.pack("<B", 1))
+        f.write(struct.pack("<Q", v.size))
+        f.write(v.astype("<f8").tobytes())
+
+
+def read_vec(path):
+    with open(path, "rb") as f:
+        little = struct.unpack("<B", f.read(1))[0]
+        e = "<" if little else ">"
+        n = struct.unpack(e + "Q", f.read(8))[0]
+        return np.frombuffer(f.read(8 * n), dtype=e + "f8").astype(np.float64)
+
+
+def test_vector_file_layout(tmp_path):
+    v = np.array([1.5, -2.0, 3.25e-300, np.pi])
+    p = tmp_path / "v.bin"
+    write_vec(p, v)
+    raw = p.read_bytes()
+    assert raw[0] == 1 and struct.unpack("<Q", raw[1:9])[0] == 4 and len(raw) == 9 + 32
+    np.testing.assert_array_equal(read_vec(p), v)
+    # big-endian writer, as a PortableBinary archive from such a host would be
+    q = tmp_path / "b.bin"
+    with open(q, "wb") as f:
+        f.write(struct.pack("B", 0) + struct.pack(">Q", 4) + v.astype(">f8").tobytes())
+    np.testing.assert_array_equal(read_vec(q), v)
+
+
+def test_driver_built_and_usage():
+    if not os.path.exists(DRIVER):
+        pytest.skip("driver not built")
+    r = subprocess.run([DRIVER], capture_output=True, text=True)
+    assert r.returncode == 2 and "usage" in r.stderr
+
+
+@pytest.mark.gpu
+def test_find_error_protocol(tmp_path):
+    """The reference's two-run protocol at L=3 (kappa 10, Qchoice 0, TOL_POW 8):
+    x_true = the dense FP64 solution of the oracle's matrix, b = A x_true written
+    to rhs_x/ in the archive layout; the driver reads b, solves, and reports the
+    forward error against x_true read back from rhs_x/."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    o = O.Problem(3, 10.0, 8, 0)
+    A = o.dense()
+    x_true = np.linalg.solve(A, o.rhs(0.0))
+    b = A @ x_true
+    N = o.N
+    d = tmp_path / "rhs_x"
+    d.mkdir()
+    tag = f"0_10_{N}.bin"
+    write_vec(d / f"rhs_real_{tag}", b.real)
+    write_vec(d / f"rhs_imag_{tag}", b.imag)
+    write_vec(d / f"x_real_{tag}", x_true.real)
+    write_vec(d / f"x_imag_{tag}", x_true.imag)
+    print(f"\n[driver] L=3 N={N}", flush=True)
+    r = subprocess.run([DRIVER, "6", "10", "6", "1", "0", "8", "3"], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "rhs from rhs_x/rhs_real_" in r.stdout
+    err = float(r.stdout.strip().splitlines()[-1].split(":")[1])
+    xe, _ = o.solve(b, O.SCHED_EFFICIENT)
+    ref_err = np.linalg.norm(xe - x_true) / np.linalg.norm(x_true)
+    print(f"  driver forward error {err:.3e}, reference schedule {ref_err:.3e}", flush=True)
+    assert err < 1.5 * ref_err + 1e-9
+    # with complex64 factors and refinement
+    r = subprocess.run([DRIVER, "6", "10", "6", "1", "0", "8", "3", "1", "20", "1e-13"], capture_output=True,
+                       text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    err2 = float(r.stdout.strip().splitlines()[-1].split(":")[1])
+    assert err2 < 1.5 * ref_err + 1e-9

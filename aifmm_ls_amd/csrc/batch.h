@@ -41,14 +41,21 @@ Blk<T> new_blk(Arena& a, int r, int c) {
   return b;
 }
 
-// Upload a host vector into a transient device buffer (synchronous copy;
-// the buffer lives in the given scratch arena).
+// Pinned staging for descriptor uploads: the host copies a batch's
+// descriptors into a pinned ring and queues an asynchronous copy to the
+// device, so it can build the next batch while the GPU runs.  A region of the
+// ring is reused only after the device has been synchronised (on wrap-around).
+void* stage_pinned(const void* src, size_t bytes, hipStream_t s);
+
+// Upload a host vector into a transient device buffer in the given scratch
+// arena (stream-ordered; the host vector may be released on return).
 template <class D>
 D* upload(Arena& a, const std::vector<D>& v, hipStream_t s) {
   if (v.empty()) return nullptr;
-  D* d = (D*)a.alloc(v.size() * sizeof(D));
-  HIP_OK(hipMemcpyAsync(d, v.data(), v.size() * sizeof(D), hipMemcpyHostToDevice, s));
-  HIP_OK(hipStreamSynchronize(s));
+  const size_t bytes = v.size() * sizeof(D);
+  D* d = (D*)a.alloc(bytes);
+  void* h = stage_pinned(v.data(), bytes, s);
+  HIP_OK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
   return d;
 }
 

@@ -61,8 +61,7 @@ void Factor<T>::init_level(int j) {
   }
   auto* d = upload(sc, cv, pb.stream);
   launch_cvt<T>((int)cv.size(), d, pb.stream);
-  HIP_OK(hipStreamSynchronize(pb.stream));
-  sc.reset();
+  sc.reset();  // scratch reuse is ordered on the stream
 }
 
 // Parent P2P blocks from the children's M2L blocks
@@ -103,8 +102,7 @@ void Factor<T>::nonleaf_p2p(int j) {
   }
   zero.run(sc, pb.stream);
   cp.run(sc, pb.stream);
-  HIP_OK(hipStreamSynchronize(pb.stream));
-  sc.reset();
+  sc.reset();  // scratch reuse is ordered on the stream
 }
 
 template <class T>
@@ -268,8 +266,7 @@ void Factor<T>::eliminate_color(int j, int c) {
   zero.run(sc, pb.stream);
   g.run(sc, pb.stream);
   for (int k : S) Bx[k].elim = true;
-  HIP_OK(hipStreamSynchronize(pb.stream));
-  sc.reset();
+  sc.reset();  // scratch reuse is ordered on the stream
   st.flops += fl;
 }
 
@@ -325,6 +322,7 @@ void Factor<T>::recompress(int j) {
   {
     auto* dd = upload(sc, nd, pb.stream);
     double* dn = sc.get<double>(cand.size());
+    ProfScope ps(pb.stream, KNAME(T, "k_fro"), 0, 0);
     launch_fro<T>((int)nd.size(), dd, dn, pb.stream);
     HIP_OK(hipMemcpyAsync(norms.data(), dn, 8 * cand.size(), hipMemcpyDeviceToHost, pb.stream));
     HIP_OK(hipStreamSynchronize(pb.stream));
@@ -412,6 +410,7 @@ void Factor<T>::recompress(int j) {
   }
   {
     auto* dq = upload(sc, qd, pb.stream);
+    ProfScope ps(pb.stream, KNAME(T, "k_cpqr"), 0, 0);
     launch_cpqr<T>((int)qd.size(), dq, pb.stream);
   }
   std::vector<int> rk(2 * nt);
@@ -445,6 +444,7 @@ void Factor<T>::recompress(int j) {
   }
   {
     auto* d = upload(sc, qf, pb.stream);
+    ProfScope ps(pb.stream, KNAME(T, "k_qform"), 0, 0);
     launch_qform<T>((int)qf.size(), d, pb.stream);
   }
   for (int i = 0; i < nt; ++i) {
@@ -558,8 +558,7 @@ void Factor<T>::recompress(int j) {
     FBox& U = Bx[f.u];
     (f.kind == 0 ? U.P2P : f.kind == 1 ? U.M2P : U.P2L).erase(f.p);
   }
-  HIP_OK(hipStreamSynchronize(pb.stream));
-  sc.reset();
+  sc.reset();  // scratch reuse is ordered on the stream
 }
 
 // solve_particles_at_base_level (FMM2DTree_find_error.hpp:2453-2485): one
@@ -589,8 +588,7 @@ void Factor<T>::base_factor() {
   dn.factor(ar, sc, fb, base);
   dn.run_factor(sc, pb.stream);
   st.flops += 16.0 * n * (double)n * n / 3.0;
-  HIP_OK(hipStreamSynchronize(pb.stream));
-  sc.reset();
+  sc.reset();  // scratch reuse is ordered on the stream
 }
 
 template <class T>
@@ -683,8 +681,7 @@ void Factor<T>::sweep_operators() {
     svb.run_solve(sc, pb.stream);
     st.flops += 8.0 * n * (double)n * n;
   }
-  HIP_OK(hipStreamSynchronize(pb.stream));
-  sc.reset();
+  sc.reset();  // scratch reuse is ordered on the stream
 }
 
 // The forward sweep (the RHS updates the reference fuses into eliminate_*),

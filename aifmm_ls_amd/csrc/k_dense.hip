@@ -12,7 +12,7 @@ namespace aifmm {
 
 constexpr int NT = 256;
 
-template <class R>
+template <class R, int NB = NT>
 __device__ void block_argmax_first(R v, int i, R* shv, int* shi, R& outv, int& outi) {
   // larger wins, ties -> lower index; i < 0 marks "none"
   for (int o = 32; o > 0; o >>= 1) {
@@ -25,7 +25,7 @@ __device__ void block_argmax_first(R v, int i, R* shv, int* shi, R& outv, int& o
   __syncthreads();
   R bv = shv[0];
   int bi = shi[0];
-  for (int k = 1; k < NT / 64; ++k) {
+  for (int k = 1; k < NB / 64; ++k) {
     R ov = shv[k];
     int oi = shi[k];
     if (oi >= 0 && (bi < 0 || ov > bv || (ov == bv && oi < bi))) bv = ov, bi = oi;
@@ -35,13 +35,13 @@ __device__ void block_argmax_first(R v, int i, R* shv, int* shi, R& outv, int& o
   outi = bi;
 }
 
-template <class R>
+template <class R, int NB = NT>
 __device__ R block_sumR(R v, R* sh) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o);
   if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
   __syncthreads();
   R r = 0;
-  for (int k = 0; k < NT / 64; ++k) r += sh[k];
+  for (int k = 0; k < NB / 64; ++k) r += sh[k];
   __syncthreads();
   return r;
 }
@@ -196,15 +196,15 @@ __global__ __launch_bounds__(NT) void k_lu_solve(const SolveDesc<T>* __restrict_
 // Column-pivoted Householder QR.  Column norms are recomputed exactly after
 // each reflection.  Reflector: I - t v v^H with v_0 = 1, beta real,
 // t = conj((beta - c0)/beta), beta = -sign(Re c0) ||x||.
-template <class T>
-__global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
+template <class T, int NQ>
+__global__ __launch_bounds__(NQ) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
   // norms, reflector scalars and the rank test in double for both element
   // types: squared moduli of complex64 entries underflow below ~1e-19
   using R = typename CplxOf<T>::real;
   const QRDesc<T> d = ds[blockIdx.x];
-  __shared__ double shv[4];
-  __shared__ int shi[4];
-  __shared__ double shs[4];
+  __shared__ double shv[NQ / 64];
+  __shared__ int shi[NQ / 64];
+  __shared__ double shs[NQ / 64];
   const int m = d.m, n = d.n, ld = d.lda;
   T* A = d.A;
   // column norms as LAPACK/Eigen keep them: nrm = running (downdated) norm,
@@ -214,8 +214,8 @@ __global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
   R* dir = d.work + n;
   const double dd_thr = sqrt((double)(sizeof(R) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-07));
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int j = threadIdx.x; j < n; j += NT) d.perm[j] = j;
-  for (int j = w; j < n; j += NT / 64) {
+  for (int j = threadIdx.x; j < n; j += NQ) d.perm[j] = j;
+  for (int j = w; j < n; j += NQ / 64) {
     double s = 0;
     for (int i = lane; i < m; i += 64) s += (double)A[i + (size_t)ld * j].x * A[i + (size_t)ld * j].x +
                                           (double)A[i + (size_t)ld * j].y * A[i + (size_t)ld * j].y;
@@ -237,17 +237,17 @@ __global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
   for (int k = 0; k < size; ++k) {
     double bv = -1;
     int bi = -1;
-    for (int j = k + threadIdx.x; j < n; j += NT) {
+    for (int j = k + threadIdx.x; j < n; j += NQ) {
       double v = nrm[j];
       if (bi < 0 || v > bv) bv = v, bi = j;
     }
     double pv;
     int big;
-    block_argmax_first(bv, bi, shv, shi, pv, big);
+    block_argmax_first<double, NQ>(bv, bi, shv, shi, pv, big);
     if (big < 0) big = k;
     if (nonzero == size && pv * pv < th_helper * (m - k)) nonzero = k;
     if (big != k) {
-      for (int i = threadIdx.x; i < m; i += NT) {
+      for (int i = threadIdx.x; i < m; i += NQ) {
         T t = A[i + (size_t)ld * k];
         A[i + (size_t)ld * k] = A[i + (size_t)ld * big];
         A[i + (size_t)ld * big] = t;
@@ -266,10 +266,10 @@ __global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
     }
     __syncthreads();
     double tl = 0;
-    for (int i = k + 1 + threadIdx.x; i < m; i += NT)
+    for (int i = k + 1 + threadIdx.x; i < m; i += NQ)
       tl += (double)A[i + (size_t)ld * k].x * A[i + (size_t)ld * k].x +
             (double)A[i + (size_t)ld * k].y * A[i + (size_t)ld * k].y;
-    tl = block_sumR(tl, shs);
+    tl = block_sumR<double, NQ>(tl, shs);
     double c0x = A[k + (size_t)ld * k].x, c0y = A[k + (size_t)ld * k].y;
     double beta = sqrt(c0x * c0x + c0y * c0y + tl), tx, ty;
     if ((tl == 0 && c0y == 0) || beta == 0) {
@@ -279,7 +279,7 @@ __global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
     } else {
       if (c0x >= 0) beta = -beta;
       double2 den = mk(c0x - beta, c0y);
-      for (int i = k + 1 + threadIdx.x; i < m; i += NT) {
+      for (int i = k + 1 + threadIdx.x; i < m; i += NQ) {
         T a = A[i + (size_t)ld * k];
         double2 q = cdiv(mk((double)a.x, (double)a.y), den);
         T o;
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(NT) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
     if (fabs(beta) > maxpiv) maxpiv = fabs(beta);
     __syncthreads();
     // apply to trailing columns, one wave per column; refresh the norms
-    for (int j = k + 1 + w; j < n; j += NT / 64) {
+    for (int j = k + 1 + w; j < n; j += NQ / 64) {
       T* col = A + (size_t)ld * j;
       const T* v = A + (size_t)ld * k;
       T s = czero<T>();
@@ -499,10 +499,12 @@ void launch_qr_solve(int nb, int max_chunks, const QSolveDesc<T>* d, hipStream_t
   hipLaunchKernelGGL(k_qr_solve<T>, dim3(nb, max_chunks), dim3(NT), 0, s, d);
   HIP_OK(hipGetLastError());
 }
+// 1024 threads: the trailing update of a wide stack (hundreds of columns)
+// runs one wave per column, so more waves per matrix shorten every pivot step
 template <class T>
 void launch_cpqr(int nb, const QRDesc<T>* d, hipStream_t s) {
   if (!nb) return;
-  hipLaunchKernelGGL(k_cpqr<T>, dim3(nb), dim3(NT), 0, s, d);
+  hipLaunchKernelGGL((k_cpqr<T, 1024>), dim3(nb), dim3(1024), 0, s, d);
   HIP_OK(hipGetLastError());
 }
 template <class T>

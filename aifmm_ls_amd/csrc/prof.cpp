@@ -1,5 +1,8 @@
 #include "prof.h"
+#include "batch.h"
 
+#include <algorithm>
+#include <cstring>
 #include <map>
 #include <mutex>
 
@@ -78,6 +81,36 @@ ProfScope::~ProfScope() {
   pending.push_back(Pending{name_, bytes_, flops_, e0_, e1});
   // bound the number of outstanding events
   if (pending.size() > 8192) fold();
+}
+
+namespace {
+std::mutex ring_mu;
+char* ring_base = nullptr;
+size_t ring_cap = 0, ring_off = 0;
+}  // namespace
+
+void* stage_pinned(const void* src, size_t bytes, hipStream_t s) {
+  (void)s;
+  std::lock_guard<std::mutex> g(ring_mu);
+  const size_t need = (bytes + 255) & ~size_t(255);
+  if (!ring_base || need > ring_cap) {
+    if (ring_base) {
+      HIP_OK(hipDeviceSynchronize());
+      HIP_OK(hipHostFree(ring_base));
+    }
+    ring_cap = std::max<size_t>(size_t(64) << 20, need);
+    HIP_OK(hipHostMalloc((void**)&ring_base, ring_cap, hipHostMallocDefault));
+    ring_off = 0;
+  }
+  if (ring_off + need > ring_cap) {
+    // every copy queued from the ring so far (any stream) must have finished
+    HIP_OK(hipDeviceSynchronize());
+    ring_off = 0;
+  }
+  void* p = ring_base + ring_off;
+  std::memcpy(p, src, bytes);
+  ring_off += need;
+  return p;
 }
 
 }  // namespace aifmm

@@ -97,6 +97,7 @@ def main():
     ap.add_argument("--max-refine", type=int, default=30)
     ap.add_argument("--fp64-steps", type=int, default=5)
     ap.add_argument("--cpu-levels", type=int, default=5, help="tree depth of the CPU baseline sample (0: skip)")
+    ap.add_argument("--batch-rhs", type=int, default=16, help="RHS count of the extra batched-throughput line (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} refused: this solver runs on one GPU; "
@@ -142,6 +143,19 @@ def main():
     total = time.perf_counter() - t0
     x = g.get_solution()[:, 0]
     rel_err = float(np.linalg.norm(x - x_ref) / xn)
+
+    # many right-hand sides in one refined solve (plane waves at equally
+    # spaced angles): throughput per RHS; the headline stays the single solve
+    batched = None
+    if args.batch_rhs > 1:
+        thetas = np.linspace(0, 2 * np.pi, args.batch_rhs, endpoint=False)
+        Bm = np.stack([g.rhs(t) for t in thetas], axis=1)
+        g.set_rhs(Bm)
+        timed_solves(g, 1, args.refine_tol, args.max_refine)
+        latb, stepsb, _ = timed_solves(g, 3, args.refine_tol, args.max_refine)
+        batched = {"nrhs": args.batch_rhs, "solves_per_s": round(3 * args.batch_rhs / float(latb.sum()), 3),
+                   "ms_per_batch": round(1e3 * float(np.median(latb)), 3), "refine_steps": int(np.median(stepsb))}
+        g.set_rhs(b)
 
     # the same steps again with a HIP event pair around every launch (issued
     # one by one instead of as a graph), for per-kernel durations
@@ -196,6 +210,7 @@ def main():
         "fp64_method": fp64,
         "assemble_s": round(t_assemble, 4),
         "factor_flops": st["factor_flops"],
+        "batched_rhs": batched,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

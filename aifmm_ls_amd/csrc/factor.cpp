@@ -403,6 +403,7 @@ void Factor<T>::recompress(int j) {
     d.thr = pow(10.0, -1.0 * pb.P.tol_pow);
     d.rank_out = dr + i;
     d.maxpiv_out = nullptr;
+    d.early = 1;
     if (A.r && A.c) qd.push_back(d);
     else HIP_OK(hipMemsetAsync(dr + i, 0, 4, pb.stream));
     double mm = A.r, nn = A.c;

@@ -234,6 +234,7 @@ __global__ __launch_bounds__(NQ) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
   const double eps_r = 2.220446049250313e-16;
   const double th_helper = (maxnorm * eps_r) * (maxnorm * eps_r) / (m > 0 ? m : 1);
   int nonzero = size;
+  int stop = size;
   for (int k = 0; k < size; ++k) {
     double bv = -1;
     int bi = -1;
@@ -272,6 +273,13 @@ __global__ __launch_bounds__(NQ) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
     tl = block_sumR<double, NQ>(tl, shs);
     double c0x = A[k + (size_t)ld * k].x, c0y = A[k + (size_t)ld * k].y;
     double beta = sqrt(c0x * c0x + c0y * c0y + tl), tx, ty;
+    if (d.early && k > 0 && beta <= maxpiv * d.thr) {
+      // column norms do not grow under the remaining reflections: every later
+      // |R_ii| is at most this one, so the rank is k
+      for (int i = k + threadIdx.x; i < size; i += NQ) d.tau[i] = czero<T>();
+      stop = k;
+      break;
+    }
     if ((tl == 0 && c0y == 0) || beta == 0) {
       // nothing to annihilate (or a column below the double range)
       tx = ty = 0;
@@ -340,7 +348,7 @@ __global__ __launch_bounds__(NQ) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
   if (threadIdx.x == 0) {
     int r = 0;
     double pre = maxpiv * d.thr;
-    for (int i = 0; i < size; ++i)
+    for (int i = 0; i < stop; ++i)
       if (hypot((double)A[i + (size_t)ld * i].x, (double)A[i + (size_t)ld * i].y) > pre) ++r;
     *d.rank_out = r;
     if (d.maxpiv_out) *d.maxpiv_out = maxpiv;

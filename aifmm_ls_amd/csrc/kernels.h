@@ -98,6 +98,8 @@ struct QRDesc {
   int* rank_out;
   double* maxpiv_out;
   int* nz_out;  // optional: number of pivots above Eigen's zero threshold
+  int early;    // 1: stop at the first pivot |R_kk| <= thr * |R_00| (rank and
+                //    the first rank columns of Q only; later tau are zeroed)
 };
 
 // B <- A^{-1} B from the column-pivoted QR of a square A (ColPivHouseholderQR

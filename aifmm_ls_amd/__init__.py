@@ -98,7 +98,7 @@ def profile_read(reset=True, max_kernels=64):
 
 
 STAT_NAMES = ["t_tables", "t_assemble", "t_factor", "factor_flops", "factor_bytes", "base_n", "max_rank",
-              "solve_bytes_1rhs", "recompressed_boxes", "operator_bytes", "t_solve", "t_nca"]
+              "solve_bytes_1rhs", "recompressed_boxes", "operator_bytes", "t_solve", "t_nca", "ill_boxes"]
 
 
 class LSProblem:

@@ -212,9 +212,10 @@ QRFact<T> new_qr(Arena& a, const Blk<T>& A) {
 template <class T>
 struct QRBatch {
   std::vector<QRDesc<T>> d;
-  void add(const QRFact<T>& f, Arena& scratch) {
+  void add(const QRFact<T>& f, Arena& scratch, double nz_eps = 0) {
     if (f.A.r == 0 || f.A.c == 0) return;
     QRDesc<T> q{};
+    q.nz_eps = nz_eps;
     q.A = f.A.p;
     q.tau = f.tau;
     q.perm = f.perm;

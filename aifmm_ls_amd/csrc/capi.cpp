@@ -254,7 +254,7 @@ int aifmm_ls_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y) {
 
 int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
   GUARD({
-    double v[12] = {0};
+    double v[13] = {0};
     Problem& P = *h->pb;
     v[0] = P.t_tables;
     v[1] = P.t_assemble;
@@ -266,13 +266,14 @@ int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
       v[5] = st->base_n;
       v[6] = st->max_rank;
       v[8] = st->recompressions;
+      v[12] = st->ill_boxes;
     }
     if (h->f64) v[7] = (double)h->f64->solve_bytes(1);
     if (h->f32) v[7] = (double)h->f32->solve_bytes(1);
     v[9] = (double)P.operator_bytes();
     v[10] = h->t_solve;
     v[11] = P.t_nca;
-    for (int i = 0; i < n && i < 12; ++i) out[i] = v[i];
+    for (int i = 0; i < n && i < 13; ++i) out[i] = v[i];
   })
 }
 

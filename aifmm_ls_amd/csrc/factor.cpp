@@ -30,6 +30,15 @@ static double now_s() {
 }
 static const double2 ONE = {1.0, 0.0}, MONE = {-1.0, 0.0}, ZERO = {0.0, 0.0};
 
+// Zero-pivot epsilon of the L2M_f factorisations (complex128).  Eigen's
+// ColPivHouseholderQR counts a pivot as nonzero down to 2.2e-16 of the largest
+// column norm; an L2M_f whose trailing pivot sits below 1e-12 of its leading
+// one (a low-contrast box whose two skeleton directions coincide to rounding)
+// has no correct digit left in that pivot, and its inverse would carry that
+// rounding into every fill-in by up to 1e15.  Such pivots are treated as zero:
+// the solve is Eigen's basic solution, as for an exactly singular block.
+static constexpr double kPivEps = 1e-12;
+
 template <class T>
 Factor<T>::Factor(Problem& p) : pb(p), L(p.L), R(p.R) {}
 
@@ -171,7 +180,7 @@ void Factor<T>::eliminate_color(int j, int c) {
     int r = K.r;
     X[t] = new_blk<T>(ar, r, ncols[t]);
     if (!r) continue;
-    dn.factor(ar, sc, K.fg, K.G.sub(0, 0, r, r));
+    dn.factor(ar, sc, K.fg, K.G.sub(0, 0, r, r), kPivEps);
     cp.add(X[t], X[t].sub(0, 0, r, r), 3, MONE);
     if (ncols[t] > r) cp.add(K.G.sub(0, r, r, ncols[t] - r), X[t].sub(0, r, r, ncols[t] - r));
     fl += 16.0 * r * r * r / 3.0 + 16.0 * r * r * ncols[t];
@@ -627,13 +636,60 @@ void Factor<T>::sweep_operators() {
   DenseOps<T> svb;
   GemmBatch<T> g;
   std::vector<Blk<T>> ginv;
+  // condition estimate of every L2M_f from the diagonal of its factor
+  // (max |d_i| / min |d_i|): one strided copy of the diagonals, one readback
+  {
+    CopyBatch<double2> dg;
+    size_t nd = 0;
+    for (int j = 2; j <= L; ++j)
+      for (auto& F : B[j])
+        if (F.m && F.r) nd += F.r;
+    Blk<double2> dd = new_blk<double2>(sc, (int)nd, 1);
+    size_t o = 0;
+    for (int j = 2; j <= L; ++j)
+      for (auto& F : B[j]) {
+        if (!F.m || !F.r) continue;
+        Blk<double2> f = DenseOps<T>::fp64_factor(F.fg);
+        Blk<double2> diag{f.p, 1, F.r, f.ld + 1};  // element (0, i) is f(i, i)
+        dg.add(diag, dd.sub((int)o, 0, F.r, 1), 2);
+        o += F.r;
+      }
+    dg.run(sc, pb.stream);
+    std::vector<double2> hd(nd);
+    if (nd) HIP_OK(hipMemcpyAsync(hd.data(), dd.p, nd * sizeof(double2), hipMemcpyDeviceToHost, pb.stream));
+    HIP_OK(hipStreamSynchronize(pb.stream));
+    o = 0;
+    st.ill_boxes = 0;
+    for (int j = 2; j <= L; ++j)
+      for (auto& F : B[j]) {
+        F.ill = false;
+        if (!F.m || !F.r) continue;
+        double mx = 0, mn = HUGE_VAL;
+        for (int i = 0; i < F.r; ++i) {
+          const double a = std::hypot(hd[o + i].x, hd[o + i].y);
+          mx = std::max(mx, a);
+          mn = std::min(mn, a);
+        }
+        o += F.r;
+        F.ill = !(mx <= kIllCond * mn);
+        st.ill_boxes += F.ill;
+      }
+  }
+  int rmax = 0;
   for (int j = 2; j <= L; ++j)
     for (auto& F : B[j]) {
       F.Z = new_blk<T>(ar, F.m, F.m);
       F.H = new_blk<T>(ar, F.r, F.m);
       F.U = new_blk<T>(ar, F.m, F.r);
       if (F.m) cp.add(F.Z, F.Z, 3);
+      if (F.ill) {
+        F.Ai = new_blk<T>(ar, F.m, F.m);
+        F.PA = new_blk<T>(ar, F.r, F.m);
+        rmax = std::max(rmax, F.r);
+      }
     }
+  eye = new_blk<T>(ar, rmax, rmax);
+  if (rmax) cp.add(eye, eye, 3);
   cp.run(sc, pb.stream);
   for (int j = 2; j <= L; ++j)
     for (auto& F : B[j])
@@ -643,6 +699,10 @@ void Factor<T>::sweep_operators() {
     for (auto& F : B[j])
       if (F.m && F.r) g.add(F.P2M, 0, F.Z, 0, F.H, MONE, ZERO), st.flops += 8.0 * F.r * F.m * F.m;
   g.run(sc, pb.stream);
+  for (int j = 2; j <= L; ++j)
+    for (auto& F : B[j])
+      if (F.ill) cp.add(F.Z, F.Ai), cp.add(F.H, F.PA);
+  cp.run(sc, pb.stream);
   for (int j = 2; j <= L; ++j)
     for (auto& F : B[j])
       if (F.m && F.r) svb.solve(sc, F.fg, F.H), st.flops += 8.0 * F.r * F.r * F.m;
@@ -752,10 +812,14 @@ void Factor<T>::build_plan(int cap) {
   }
   // phase-1 vectors of tall boxes in the back-substitution (no save target)
   std::vector<std::vector<Blk<T>>> yt(L + 1);
+  vk.assign(L + 1, {});
   for (int j = 2; j <= L; ++j) {
     yt[j].resize(1 << (2 * j));
-    for (int k = 0; k < (1 << (2 * j)); ++k)
+    vk[j].resize(1 << (2 * j));
+    for (int k = 0; k < (1 << (2 * j)); ++k) {
       if (B[j][k].m > Program<T>::TILE_M) yt[j][k] = new_blk<T>(pv, B[j][k].m, cap);
+      if (B[j][k].ill) vk[j][k] = new_blk<T>(pv, B[j][k].r, cap);
+    }
   }
   Program<T>& P = *plan;
   auto need = [](const Blk<T>& b, int rows, int cols, const char* what) {
@@ -783,11 +847,27 @@ void Factor<T>::build_plan(int cap) {
           need(it->second, F.m, B[j][nn].m, "P2P (forward)");
           op.term(it->second.p, it->second.ld, it->second.c, bk[j][nn].p, bk[j][nn].ld, MONE);
         }
-        if (F.m) out_term<T>(op.out(bk[j][k].p, bk[j][k].ld, F.m), F.Z.p, F.Z.ld, F.m, nullptr, 0, ONE);
+        const Blk<T>& Zf = F.ill ? F.Ai : F.Z;
+        if (F.m) out_term<T>(op.out(bk[j][k].p, bk[j][k].ld, F.m), Zf.p, Zf.ld, F.m, nullptr, 0, ONE);
         if (F.r) {
           auto& o = op.out(mr[j][k].p, mr[j][k].ld, F.r);
           if (F.m) out_term<T>(o, F.H.p, F.H.ld, F.m, nullptr, 0, ONE);
         }
+        P.add(op);
+      }
+      // ill boxes of this colour: b_k = A^{-1} y - W_r mr_k
+      bool staged = false;
+      for (int k : colors[j][c]) {
+        FBox& F = B[j][k];
+        if (!F.ill) continue;
+        if (!staged) P.stage(), staged = true;
+        BoxOp<T> op;
+        op.m = F.m;
+        op.src = bk[j][k].p;
+        op.ldsrc = bk[j][k].ld;
+        op.save = bk[j][k].p;
+        op.ldsave = bk[j][k].ld;
+        op.term(F.W.p, F.W.ld, F.r, mr[j][k].p, mr[j][k].ld, MONE);
         P.add(op);
       }
     }
@@ -848,8 +928,30 @@ void Factor<T>::build_plan(int cap) {
           op.term(it->second.p, it->second.ld, it->second.c, v.p, v.ld, MONE);
         }
         auto& o = op.out(xk[j][k].p, xk[j][k].ld, F.m);
-        out_term<T>(o, F.Z.p, F.Z.ld, F.m, nullptr, 0, ONE);
-        if (F.r) out_term<T>(o, F.U.p, F.U.ld, F.r, yk[j][k].p, yk[j][k].ld, MONE);
+        if (F.ill) {
+          out_term<T>(o, F.Ai.p, F.Ai.ld, F.m, nullptr, 0, ONE);
+          auto& ov = op.out(vk[j][k].p, vk[j][k].ld, F.r);
+          out_term<T>(ov, eye.p, eye.ld, F.r, yk[j][k].p, yk[j][k].ld, ONE);
+          out_term<T>(ov, F.PA.p, F.PA.ld, F.m, nullptr, 0, ONE);
+        } else {
+          out_term<T>(o, F.Z.p, F.Z.ld, F.m, nullptr, 0, ONE);
+          if (F.r) out_term<T>(o, F.U.p, F.U.ld, F.r, yk[j][k].p, yk[j][k].ld, MONE);
+        }
+        P.add(op);
+      }
+      // ill boxes of this colour: x_k = A^{-1} x0 - U (y_k - P2M A^{-1} x0)
+      bool staged = false;
+      for (int k : colors[j][c]) {
+        FBox& F = B[j][k];
+        if (!F.ill) continue;
+        if (!staged) P.stage(), staged = true;
+        BoxOp<T> op;
+        op.m = F.m;
+        op.src = xk[j][k].p;
+        op.ldsrc = xk[j][k].ld;
+        op.save = xk[j][k].p;
+        op.ldsave = xk[j][k].ld;
+        op.term(F.U.p, F.U.ld, F.r, vk[j][k].p, vk[j][k].ld, MONE);
         P.add(op);
       }
     }

@@ -18,6 +18,7 @@ struct FactorStats {
   int max_rank = 0;
   size_t bytes = 0;        // factor storage
   int recompressions = 0;  // boxes re-based after fill-in
+  int ill_boxes = 0;       // boxes whose sweeps keep the reference's association
 };
 
 // Factorisation of a square block for repeated solves.  complex128: column-
@@ -40,11 +41,13 @@ template <>
 struct DenseOps<double2> {
   QRBatch<double2> qf;
   QRSolveBatch<double2> qs;
-  void factor(Arena& ar, Arena& sc, DFact<double2>& f, const Blk<double2>& A) {
+  void factor(Arena& ar, Arena& sc, DFact<double2>& f, const Blk<double2>& A, double nz_eps = 0) {
     f.q = new_qr(ar, A);
-    qf.add(f.q, sc);
+    qf.add(f.q, sc, nz_eps);
   }
   void solve(Arena& sc, const DFact<double2>& f, const Blk<double2>& B) { qs.add(f.q, B, sc); }
+  // the FP64 array holding the factor (R on and above the diagonal)
+  static Blk<double2> fp64_factor(const DFact<double2>& f) { return f.q.A; }
   void run_factor(Arena& sc, hipStream_t s) { qf.run(sc, s); }
   void run_solve(Arena& sc, hipStream_t s) { qs.run(sc, s); }
 };
@@ -59,7 +62,9 @@ struct DenseOps<float2> {
   static void need_contig(const Blk<float2>& b) {
     if (b.size() && b.ld != b.r) throw std::runtime_error("FP64 pivot block path needs contiguous blocks");
   }
-  void factor(Arena& ar, Arena& sc, DFact<float2>& f, const Blk<float2>& A) {
+  // nz_eps applies to the QR path only: LU has no zero-pivot test
+  void factor(Arena& ar, Arena& sc, DFact<float2>& f, const Blk<float2>& A, double nz_eps = 0) {
+    (void)nz_eps;
     need_contig(A);
     f.q.A = A;
     f.a64 = wide(ar, A);
@@ -76,6 +81,8 @@ struct DenseOps<float2> {
     ls.add(f.a64, f.piv, b64);
     nb.push_back(CvtDesc<float2>{b64.p, B.p, (int64_t)B.size()});
   }
+  // the FP64 array holding the factor (U on and above the diagonal)
+  static Blk<double2> fp64_factor(const DFact<float2>& f) { return f.a64; }
   void run_factor(Arena& sc, hipStream_t s) {
     if (!wf.empty()) launch_widen((int)wf.size(), upload(sc, wf, s), s);
     wf.clear();
@@ -116,7 +123,17 @@ class Factor {
     //   Z = A^{-1} - W_r H               (m x m)
     //   U = W_r L2M_f^{-1}               (m x r)
     Blk<T> Z, H, U;
+    // A badly conditioned L2M_f (condition estimate above kIllCond) makes
+    // Z and U large, and Z x0 - U y_k would cancel in rounding.  Such boxes
+    // run the sweeps in the reference's order instead:
+    //   forward   b_k = A^{-1} y - W_r (H y)
+    //   backward  x_k = A^{-1} x0 - U (y_k - P2M A^{-1} x0)
+    // so L2M_f^{-1} only meets the vectors the reference applies it to.
+    bool ill = false;
+    Blk<T> Ai;  // A^{-1}                (m x m)
+    Blk<T> PA;  // -P2M A^{-1}           (r x m)
   };
+  static constexpr double kIllCond = 1e4;
   std::vector<std::vector<FBox>> B;
   std::vector<std::vector<std::vector<int>>> colors;
 
@@ -137,6 +154,8 @@ class Factor {
   int plan_cap = 0;
   Blk<T> Bt, Xt, yb, xb;
   std::vector<std::vector<Blk<T>>> bk, bs, mr, xk, yk;
+  std::vector<std::vector<Blk<T>>> vk;  // y_k - P2M A^{-1} x0 of the ill boxes
+  Blk<T> eye;                           // identity, rank x rank of the largest box
   void sweep_operators();
   void build_plan(int cap);
 

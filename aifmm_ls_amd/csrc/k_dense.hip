@@ -231,7 +231,7 @@ __global__ __launch_bounds__(NQ) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
   // epsilon would lose the badly scaled rows of low-contrast boxes)
   double maxnorm = 0;
   for (int j = 0; j < n; ++j) maxnorm = fmax(maxnorm, (double)nrm[j]);
-  const double eps_r = 2.220446049250313e-16;
+  const double eps_r = d.nz_eps > 0 ? d.nz_eps : 2.220446049250313e-16;
   const double th_helper = (maxnorm * eps_r) * (maxnorm * eps_r) / (m > 0 ? m : 1);
   int nonzero = size;
   int stop = size;

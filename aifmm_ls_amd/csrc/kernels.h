@@ -100,6 +100,7 @@ struct QRDesc {
   int* nz_out;  // optional: number of pivots above Eigen's zero threshold
   int early;    // 1: stop at the first pivot |R_kk| <= thr * |R_00| (rank and
                 //    the first rank columns of Q only; later tau are zeroed)
+  double nz_eps;  // epsilon of the zero-pivot test (0: the FP64 epsilon)
 };
 
 // B <- A^{-1} B from the column-pivoted QR of a square A (ColPivHouseholderQR

@@ -80,7 +80,9 @@ int aifmm_ls_set_refinement(aifmm_ls_problem* h, int method);
  * nested L2P M2L P2M far field), the operator the factorisation inverts; N x nrhs. */
 int aifmm_ls_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y);
 /* statistics: out[0..] = t_tables, t_assemble, t_factor, factor flops, factor bytes,
- * base size, max rank, solve bytes (nrhs=1), recompressed boxes, operator bytes */
+ * base size, max rank, solve bytes (nrhs=1), recompressed boxes, operator bytes,
+ * last solve time, NCA time, boxes whose sweeps keep the reference's order
+ * (badly conditioned L2M_f) */
 int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n);
 
 /* LowRank::ACA_only_nodes (ACA.hpp:368-577) on explicit point lists: pivot rows rb

@@ -231,3 +231,32 @@ def test_c2_workload_properties():
     assert st32[2] < 1e-12
     assert rel(x32, x64) < 1e-9
     g.close()
+
+
+def test_reference_config_c1():
+    """The reference's own run `6 10 6 1 0 8 5` (N = 36,864, kappa 10, Qchoice 0,
+    TOL_POW 8).  GPU colour schedule vs the oracle's, bar from the oracle's own
+    last-bit sensitivity; the FP64-refined GPU solution vs the reference
+    schedule (eliminate_phase_efficient), which differs from it by the
+    reference method's own error."""
+    param = (5, 10.0, 8, 0, None)
+    g = A.LSProblem(5, 10.0, 8, 0)
+    o = O.Problem(5, 10.0, 8, 0)
+    b = g.rhs(0.0)
+    g.factor(A.PREC_C128)
+    x, _ = g.solve(b)
+    ill = g.stats()["ill_boxes"]
+    xo, _ = o.solve(b, O.SCHED_COLORED)
+    sens = oracle_sensitivity(param, b, O.SCHED_COLORED)
+    d = rel(x, xo)
+    xe, _ = o.solve(b, O.SCHED_EFFICIENT)
+    g.factor(A.PREC_C64)
+    xr, st = g.solve(b, tol=1e-12, max_refine=30)
+    print(f"\n  C1: GPU vs oracle colour {d:.2e} (sensitivity {sens:.2e}, {ill:.0f} boxes in reference order); "
+          f"refined (relres {st[2]:.1e}, {st[1]:.0f} steps) vs reference schedule {rel(xr, xe):.2e}; "
+          f"colour vs reference schedule {rel(xo, xe):.2e}", flush=True)
+    assert d < max(10 * 1e-8, 20 * sens)
+    assert st[2] < 1e-11
+    # the reference schedule's direct solve is within its method error of the converged solution
+    assert rel(xr, xe) < 1e-3
+    g.close()

@@ -210,6 +210,7 @@ def main():
         "fp64_method": fp64,
         "assemble_s": round(t_assemble, 4),
         "factor_flops": st["factor_flops"],
+        "reference_order_boxes": int(st["ill_boxes"]),
         "batched_rhs": batched,
         "roofline": roof,
         "cpu_baseline": cpu,

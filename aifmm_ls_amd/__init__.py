@@ -66,6 +66,15 @@ def lib():
         L.aifmm_ls_profile_enable.argtypes = [ctypes.c_int]
         L.aifmm_ls_set_refinement.argtypes = [P, ctypes.c_int]
         L.aifmm_ls_profile_read.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_char_p, ctypes.c_int]
+        L.aifmm_ls_device_bytes.argtypes = [P, ip64]
+        L.aifmm_ls_server_create.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.aifmm_ls_server_factor.argtypes = [P, ctypes.POINTER(_Params), ctypes.c_int, ip64, ip64]
+        L.aifmm_ls_server_solve.argtypes = [P, ctypes.c_int64, ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_int,
+                                            ip64]
+        L.aifmm_ls_server_wait.argtypes = [P, ctypes.c_int64, dp]
+        L.aifmm_ls_server_release.argtypes = [P, ctypes.c_int64]
+        L.aifmm_ls_server_stats.argtypes = [P, dp, ctypes.c_int]
+        L.aifmm_ls_server_destroy.argtypes = [P]
         _lib = L
     return _lib
 
@@ -240,6 +249,12 @@ class LSProblem:
         Ar = Ar[: 2 * cols.size * r].view(np.complex128).reshape(cols.size, r).T
         return rb[:r].copy(), cb[:r].copy(), Ac, Ar
 
+    def device_bytes(self):
+        """Device memory the problem holds (operator, factors, vectors)."""
+        b = ctypes.c_int64()
+        _chk(lib().aifmm_ls_device_bytes(self.h, ctypes.byref(b)))
+        return b.value
+
     def stats(self):
         out = np.zeros(len(STAT_NAMES))
         _chk(lib().aifmm_ls_stats(self.h, _dp(out), len(STAT_NAMES)))
@@ -281,3 +296,97 @@ def selftest_cpqr(A, thr, k, c64=False):
     dummy = np.zeros(2)
     _chk(lib().aifmm_ls_selftest(10 * c64 + 2, m, n, k, _dp(_cm(A)), _dp(dummy), _dp(C), ctypes.byref(r), thr))
     return r.value, C.view(np.complex128).reshape(k, m).T
+
+
+def _params(n_levels, kappa, tol_pow, qchoice, n_cheb, degree, L, bumps, quantize):
+    p = _Params()
+    p.n_cheb, p.degree, p.n_levels, p.tol_pow, p.qchoice = n_cheb, degree, n_levels, tol_pow, qchoice
+    p.quantize_tables = int(quantize)
+    p.kappa, p.L = float(kappa), float(L)
+    keep = None
+    if bumps is not None:
+        keep = np.ascontiguousarray(np.asarray(bumps, np.float64).reshape(-1, 4))
+        p.bumps = _dp(keep)
+        p.n_bumps = keep.shape[0]
+    return p, keep
+
+
+SERVER_STAT_NAMES = ["resident_problems", "resident_bytes", "budget", "evictions", "refactorisations", "completed",
+                     "batches", "merged_requests", "peak_running", "queued"]
+
+
+class Server:
+    """The serving loop of the C ABI (aifmm_ls_server_*): factorised problems
+    stay resident on the GPU, requests run on `slots` worker threads (each on
+    its problem's own HIP stream, so different problems overlap on the
+    device), queued solves of one problem are merged into one multi-RHS solve,
+    and new factorisations are admitted by device memory against
+    `budget_bytes` with least-recently-used eviction.
+
+    ``factor(...)`` and ``solve(...)`` queue a request and return at once;
+    ``wait(ticket)`` returns (x or None, info dict)."""
+
+    def __init__(self, budget_bytes, slots=4):
+        s = ctypes.c_void_p()
+        _chk(lib().aifmm_ls_server_create(int(budget_bytes), int(slots), ctypes.byref(s)))
+        self.s = s
+        self._out = {}
+        self._N = {}
+
+    def factor(self, n_levels, kappa, tol_pow=8, qchoice=0, bumps=None, precision=PREC_C64, n_cheb=6, degree=6,
+               L=1.0, quantize=True):
+        """Queue a factorisation; returns (problem id, ticket)."""
+        p, _keep = _params(n_levels, kappa, tol_pow, qchoice, n_cheb, degree, L, bumps, quantize)
+        pid, t = ctypes.c_int64(), ctypes.c_int64()
+        _chk(lib().aifmm_ls_server_factor(self.s, ctypes.byref(p), int(precision), ctypes.byref(pid),
+                                          ctypes.byref(t)))
+        self._N[pid.value] = (1 << (2 * n_levels)) * n_cheb * n_cheb
+        self._out[t.value] = None
+        return pid.value, t.value
+
+    def solve(self, problem, b, tol=1e-9, max_refine=30):
+        """Queue a solve of b (N or N x nrhs complex); returns the ticket."""
+        b = np.asarray(b, np.complex128)
+        vec = b.ndim == 1
+        B = b.reshape(b.shape[0], -1)
+        N = self._N[problem]
+        if B.shape[0] != N:
+            raise AifmmError(f"right-hand side has {B.shape[0]} rows, problem has {N}")
+        nrhs = B.shape[1]
+        bb = _cm(B)
+        x = np.zeros(2 * N * nrhs)
+        t = ctypes.c_int64()
+        _chk(lib().aifmm_ls_server_solve(self.s, int(problem), nrhs, _dp(bb), _dp(x), float(tol), int(max_refine),
+                                         ctypes.byref(t)))
+        self._out[t.value] = (x, N, nrhs, vec)
+        return t.value
+
+    def wait(self, ticket):
+        info = np.zeros(8)
+        rc = lib().aifmm_ls_server_wait(self.s, int(ticket), _dp(info))
+        out = self._out.pop(ticket, None)
+        if rc != 0:
+            raise AifmmError(lib().aifmm_ls_last_error().decode())
+        d = dict(latency=info[1], queue=info[2], run=info[3], steps=int(info[4]), relres=info[5],
+                 batch_rhs=int(info[6]), refactored=bool(info[7]))
+        if out is None:
+            return None, d
+        x, N, nrhs, vec = out
+        X = x.view(np.complex128).reshape(nrhs, N).T
+        return (X[:, 0].copy() if vec else X.copy()), d
+
+    def release(self, problem):
+        _chk(lib().aifmm_ls_server_release(self.s, int(problem)))
+
+    def stats(self):
+        out = np.zeros(len(SERVER_STAT_NAMES))
+        _chk(lib().aifmm_ls_server_stats(self.s, _dp(out), len(out)))
+        return dict(zip(SERVER_STAT_NAMES, out.tolist()))
+
+    def close(self):
+        if getattr(self, "s", None):
+            lib().aifmm_ls_server_destroy(self.s)
+            self.s = None
+
+    def __del__(self):
+        self.close()

@@ -11,6 +11,8 @@ namespace aifmm {
 class Arena {
  public:
   explicit Arena(size_t chunk = size_t(1) << 30) : chunk_(chunk) {}
+  // chunk size of later allocations (problems size their arenas by N)
+  void set_chunk(size_t chunk) { chunk_ = chunk; }
   ~Arena() { release(); }
   Arena(const Arena&) = delete;
   Arena& operator=(const Arena&) = delete;
@@ -63,6 +65,12 @@ class Arena {
     return s;
   }
   size_t used() const { return total_; }
+
+  // chunk size for an arena holding about per_unknown bytes per unknown
+  static size_t chunk_for(int64_t n, size_t per_unknown, size_t lo, size_t hi) {
+    const size_t want = (size_t)n * per_unknown;
+    return want < lo ? lo : want > hi ? hi : want;
+  }
 
  private:
   size_t chunk_;

@@ -34,6 +34,10 @@ struct aifmm_ls_problem {
 
 static thread_local std::string g_err;
 
+namespace aifmm {
+void set_last_error(const std::string& msg) { g_err = msg; }
+}  // namespace aifmm
+
 #define GUARD(...)                          \
   try {                                     \
     __VA_ARGS__;                            \
@@ -78,6 +82,8 @@ int aifmm_ls_create(const aifmm_ls_params* p, aifmm_ls_problem** out) {
       delete h;
       throw;
     }
+    h->io.set_chunk(Arena::chunk_for(h->pb->N, 256, 4 << 20, 64 << 20));
+    h->sc.set_chunk(Arena::chunk_for(h->pb->N, 64, 4 << 20, 16 << 20));
     *out = h;
   })
 }
@@ -249,6 +255,18 @@ int aifmm_ls_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y) {
     h->h2->apply(dx, dy, nrhs);
     HIP_OK(hipStreamSynchronize(h->pb->stream));
     HIP_OK(hipMemcpy(y, dy, N * nrhs * 16, hipMemcpyDeviceToHost));
+  })
+}
+
+int aifmm_ls_device_bytes(aifmm_ls_problem* h, int64_t* bytes) {
+  GUARD({
+    if (!h || !bytes) throw std::runtime_error("null argument");
+    size_t b = h->pb->device_bytes() + h->sc.reserved() + h->io.reserved();
+    if (h->f64) b += h->f64->device_bytes();
+    if (h->f32) b += h->f32->device_bytes();
+    if (h->h2) b += h->h2->device_bytes();
+    if (h->ref) b += h->ref->device_bytes();
+    *bytes = (int64_t)b;
   })
 }
 

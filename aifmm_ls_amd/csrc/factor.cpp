@@ -40,7 +40,17 @@ static const double2 ONE = {1.0, 0.0}, MONE = {-1.0, 0.0}, ZERO = {0.0, 0.0};
 static constexpr double kPivEps = 1e-12;
 
 template <class T>
-Factor<T>::Factor(Problem& p) : pb(p), L(p.L), R(p.R) {}
+Factor<T>::Factor(Problem& p) : pb(p), L(p.L), R(p.R) {
+  ar.set_chunk(Arena::chunk_for(p.N, sizeof(T) * (2 << 10), 32 << 20, 1 << 30));
+  sc.set_chunk(Arena::chunk_for(p.N, 2 << 10, 16 << 20, 256 << 20));
+  sv.set_chunk(Arena::chunk_for(p.N, 2 << 10, 16 << 20, 256 << 20));
+  pv.set_chunk(Arena::chunk_for(p.N, 1 << 10, 16 << 20, 256 << 20));
+}
+
+template <class T>
+size_t Factor<T>::device_bytes() const {
+  return ar.reserved() + sc.reserved() + sv.reserved() + pv.reserved();
+}
 
 template <class T>
 void Factor<T>::init_level(int j) {
@@ -620,6 +630,7 @@ void Factor<T>::factor() {
   sweep_operators();
   build_plan(1);
   HIP_OK(hipStreamSynchronize(pb.stream));
+  sc.release();  // factorisation scratch; later users allocate afresh
   st.t_factor = now_s() - t0;
   st.bytes = ar.used();
   for (int j = 2; j <= L; ++j)

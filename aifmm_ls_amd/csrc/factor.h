@@ -107,6 +107,7 @@ class Factor {
   void solve(const double2* b, double2* x, int nrhs);
   FactorStats st;
   size_t solve_bytes(int nrhs) const;  // algorithmic bytes of one sweep pair
+  size_t device_bytes() const;         // device memory held (arena chunks)
 
   struct FBox {
     int m = 0, r = 0, color = 0;

@@ -12,7 +12,7 @@ namespace aifmm {
 
 class H2Op {
  public:
-  explicit H2Op(Problem& pb) : pb(pb) {}
+  explicit H2Op(Problem& pb) : pb(pb) { ws.set_chunk(Arena::chunk_for(pb.N, 512, 16 << 20, 64 << 20)); }
   // y = A_H x ; x, y device arrays N x nrhs (column-major, complex128).
   // Asynchronous on the problem's stream.  The launch program works on its
   // own input and output buffers (x and y are copied in and out), built once
@@ -20,6 +20,7 @@ class H2Op {
   void apply(const double2* x, double2* y, int nrhs);
   // algorithmic bytes of one application (operator blocks + vectors)
   size_t bytes(int nrhs) const;
+  size_t device_bytes() const { return ws.reserved(); }
 
  private:
   Problem& pb;

@@ -35,17 +35,32 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
       const T* A = tm.A + rb + i;
       const T* x = tm.x ? tm.x + (size_t)tm.ldx * c0 : ylds;
       const size_t la = tm.lda;
-      T pa = czero<T>();
+      // eight independent loads in flight: a program's run time is a chain
+      // of dependent memory round trips, and this loop sets their number
+      T pa = czero<T>(), pb = czero<T>();
       int j = q;
+      for (; j + 7 * spt < tm.cols; j += 8 * spt) {
+        T a[8], xv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) a[u] = A[la * (j + u * spt)];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xv[u] = x[j + u * spt];
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+          pa = cfma(a[u], xv[u], pa);
+          pb = cfma(a[u + 1], xv[u + 1], pb);
+        }
+      }
       for (; j + 3 * spt < tm.cols; j += 4 * spt) {
         const T a0 = A[la * j], a1 = A[la * (j + spt)], a2 = A[la * (j + 2 * spt)], a3 = A[la * (j + 3 * spt)];
         const T x0 = x[j], x1 = x[j + spt], x2 = x[j + 2 * spt], x3 = x[j + 3 * spt];
         pa = cfma(a0, x0, pa);
-        pa = cfma(a1, x1, pa);
+        pb = cfma(a1, x1, pb);
         pa = cfma(a2, x2, pa);
-        pa = cfma(a3, x3, pa);
+        pb = cfma(a3, x3, pb);
       }
       for (; j < tm.cols; j += spt) pa = cfma(A[la * j], x[j], pa);
+      pa = cadd(pa, pb);
       acc[0] = cmul(from2<T>(tm.alpha), pa);
     }
   } else if (s < ns && i < rr) {
@@ -89,7 +104,38 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
       for (int c = 0; c < CH; ++c) acc[c] = cfma(al, pa[c], acc[c]);
     }
   }
-  if (ns > 1) {
+  if (ns > 1 && rp < 64) {
+    // rp = 16 or 32: the 64 / rp slices inside a wave hold the same rows
+    // (lane = slice * rp + row); add them across lanes, then combine the
+    // per-wave partials of each row in wave 0 the same way
+    constexpr int NW = NTH / 64;
+    const int lane = tid & 63, w = tid >> 6;
+    for (int off = rp; off < 64; off <<= 1)
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        acc[c].x += __shfl_xor(acc[c].x, off);
+        acc[c].y += __shfl_xor(acc[c].y, off);
+      }
+    if (lane < rp)
+#pragma unroll
+      for (int c = 0; c < CH; ++c) red[w * rp + lane + NTH * c] = acc[c];
+    __syncthreads();
+    if (w == 0) {
+      const int g = lane / rp, ng = 64 / rp;
+#pragma unroll
+      for (int c = 0; c < CH; ++c) acc[c] = czero<T>();
+      for (int q = g; q < NW; q += ng)
+#pragma unroll
+        for (int c = 0; c < CH; ++c) acc[c] = cadd(acc[c], red[q * rp + i + NTH * c]);
+      for (int off = rp; off < 64; off <<= 1)
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+          acc[c].x += __shfl_xor(acc[c].x, off);
+          acc[c].y += __shfl_xor(acc[c].y, off);
+        }
+    }
+    __syncthreads();
+  } else if (ns > 1) {
 #pragma unroll
     for (int c = 0; c < CH; ++c) red[tid + NTH * c] = acc[c];
     __syncthreads();
@@ -117,13 +163,24 @@ __global__ __launch_bounds__(NTH) void k_prog(const PProg<T>* __restrict__ progs
       const int rr = min(256, m - rb);
       T acc[CH];
       int row;
+      // the source rows are loaded ahead of the term sums (same row mapping
+      // as row_block_sum: row rb + tid % rp, written by slice 0)
+      T sv[CH];
+      {
+        const int rp = rr <= 16 ? 16 : rr <= 32 ? 32 : ((rr + 63) >> 6) << 6;
+        const int i = threadIdx.x % rp;
+#pragma unroll
+        for (int c = 0; c < CH; ++c)
+          sv[c] = (P->src && threadIdx.x < rp && i < rr && c < nc) ? P->src[rb + i + (size_t)P->ldsrc * (c0 + c)]
+                                                                  : czero<T>();
+      }
       if (row_block_sum<T, CH, NTH>(rb, rr, terms, P->t0, P->t1, c0, nc, ylds, ldy, red, acc, row)) {
         const T sa = from2<T>(P->src_alpha);
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
           if (c >= nc) break;
           T v = acc[c];
-          if (P->src) v = cfma(sa, P->src[row + (size_t)P->ldsrc * (c0 + c)], v);
+          if (P->src) v = cfma(sa, sv[c], v);
           ylds[row + (size_t)ldy * c] = v;
           if (P->save) P->save[row + (size_t)P->ldsave * (c0 + c)] = v;
         }
@@ -180,7 +237,8 @@ template <class T>
 void launch_prog(int nprog, const PProg<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, hipStream_t s) {
   if (!nprog || !nrhs) return;
   if (ldy < 1) ldy = 1;
-  if (nprog > 1024) launch_nth<T, 256>(nprog, progs, terms, nrhs, ldy, s);
+  static const int force = getenv("AIFMM_PROG_NTH") ? atoi(getenv("AIFMM_PROG_NTH")) : 0;
+  if (force == 256 || (force != 1024 && nprog > 1024)) launch_nth<T, 256>(nprog, progs, terms, nrhs, ldy, s);
   else launch_nth<T, 1024>(nprog, progs, terms, nrhs, ldy, s);
 }
 

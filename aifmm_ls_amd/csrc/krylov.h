@@ -23,7 +23,8 @@ struct RefineResult {
 class Refiner {
  public:
   using Precond = std::function<void(const double2* r, double2* z)>;  // N x nrhs each
-  Refiner(Problem& pb, H2Op& A) : pb_(pb), A_(A) {}
+  Refiner(Problem& pb, H2Op& A) : pb_(pb), A_(A) { ws_.set_chunk(Arena::chunk_for(pb.N, 1 << 10, 16 << 20, 256 << 20)); }
+  size_t device_bytes() const { return ws_.reserved(); }
   // x <- solution of A_H x = b to relative residual tol (at most maxit
   // preconditioner applications after x0 = M b).  method 0: FGMRES, 1: Richardson.
   RefineResult run(const Precond& M, const double2* b, double2* x, int nrhs, double tol, int maxit, int method);

@@ -134,13 +134,22 @@ static double round6(double v) {
 }
 
 Problem::Problem(const Params& Pin) : P(Pin) {
-  HIP_OK(hipStreamCreate(&stream));
+  // non-blocking: no implicit ordering with the null stream, so problems on
+  // different host threads overlap on the device
+  HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
   R = P.n_cheb * P.n_cheb;
   nbasis = P.degree * (P.degree + 1) / 2;
   L = P.n_levels;
   if (L < 2 || L > 9) throw std::runtime_error("n_levels must be in [2, 9]");
   nleaf = 1 << (2 * L);
   N = (int64_t)nleaf * R;
+  // arena chunks sized by N, so that small problems stay small (a cache of
+  // resident problems is admitted by their device bytes)
+  persist.set_chunk(Arena::chunk_for(N, 64, 4 << 20, 64 << 20));
+  op.set_chunk(Arena::chunk_for(N, 8 << 10, 32 << 20, 1 << 30));
+  acaws.set_chunk(Arena::chunk_for(N, 8 << 10, 32 << 20, 1 << 30));
+  scratch.set_chunk(Arena::chunk_for(N, 2 << 10, 16 << 20, 256 << 20));
+  acakeep.set_chunk(Arena::chunk_for(N, 2 << 10, 16 << 20, 256 << 20));
   rleaf = P.L;
   for (int j = 0; j < L; ++j) rleaf *= 0.5;
   // points: leaf k holds points 36k..36k+35, index jy*n+kx at
@@ -641,8 +650,15 @@ void Problem::assemble() {
     scratch.reset();
   }
   assembled = true;
+  // the ACA workspaces are only needed while assembling
+  acaws.release();
+  acakeep.release();
   t_nca = t1 - t0;
   t_assemble = now_s() - t0;
+}
+
+size_t Problem::device_bytes() const {
+  return persist.reserved() + op.reserved() + scratch.reserved() + acaws.reserved() + acakeep.reserved();
 }
 
 }  // namespace aifmm

@@ -63,6 +63,7 @@ class Problem {
   void entries(int64_t n, const int64_t* I, const int64_t* J, double2* out);
   int well_sep(int j, int k, int l) const;
   size_t operator_bytes() const { return op.used(); }
+  size_t device_bytes() const;  // device memory held (arena chunks)
 
  private:
   void nca_level(int j);

@@ -83,33 +83,59 @@ ProfScope::~ProfScope() {
   if (pending.size() > 8192) fold();
 }
 
+// One staging ring per host thread.  A thread queues the copy out of its
+// region before it asks for the next one, so after a device synchronise every
+// copy the thread queued from its ring has completed: a ring that wraps
+// synchronises and starts over.  Threads that end hand their ring back; the
+// next owner starts with a wrap, since the old owner's copies may be queued.
 namespace {
+struct Ring {
+  char* base = nullptr;
+  size_t cap = 0, off = 0;
+};
 std::mutex ring_mu;
-char* ring_base = nullptr;
-size_t ring_cap = 0, ring_off = 0;
+std::vector<Ring*> spare_rings;
+struct RingSlot {
+  Ring* r = nullptr;
+  ~RingSlot() {
+    if (!r) return;
+    std::lock_guard<std::mutex> g(ring_mu);
+    spare_rings.push_back(r);
+  }
+};
+thread_local RingSlot my_ring;
 }  // namespace
 
 void* stage_pinned(const void* src, size_t bytes, hipStream_t s) {
   (void)s;
-  std::lock_guard<std::mutex> g(ring_mu);
-  const size_t need = (bytes + 255) & ~size_t(255);
-  if (!ring_base || need > ring_cap) {
-    if (ring_base) {
-      HIP_OK(hipDeviceSynchronize());
-      HIP_OK(hipHostFree(ring_base));
+  if (!my_ring.r) {
+    std::lock_guard<std::mutex> g(ring_mu);
+    if (!spare_rings.empty()) {
+      my_ring.r = spare_rings.back();
+      spare_rings.pop_back();
+      my_ring.r->off = my_ring.r->cap;  // force a synchronised wrap first
+    } else {
+      my_ring.r = new Ring();
     }
-    ring_cap = std::max<size_t>(size_t(64) << 20, need);
-    HIP_OK(hipHostMalloc((void**)&ring_base, ring_cap, hipHostMallocDefault));
-    ring_off = 0;
   }
-  if (ring_off + need > ring_cap) {
-    // every copy queued from the ring so far (any stream) must have finished
+  Ring& R = *my_ring.r;
+  const size_t need = (bytes + 255) & ~size_t(255);
+  if (!R.base || need > R.cap) {
+    if (R.base) {
+      HIP_OK(hipDeviceSynchronize());
+      HIP_OK(hipHostFree(R.base));
+    }
+    R.cap = std::max<size_t>(size_t(64) << 20, need);
+    HIP_OK(hipHostMalloc((void**)&R.base, R.cap, hipHostMallocDefault));
+    R.off = 0;
+  }
+  if (R.off + need > R.cap) {
     HIP_OK(hipDeviceSynchronize());
-    ring_off = 0;
+    R.off = 0;
   }
-  void* p = ring_base + ring_off;
+  void* p = R.base + R.off;
   std::memcpy(p, src, bytes);
-  ring_off += need;
+  R.off += need;
   return p;
 }
 

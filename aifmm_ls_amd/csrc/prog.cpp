@@ -3,6 +3,7 @@
 #include "prog.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <string>
 #include <type_traits>
@@ -149,13 +150,16 @@ void Program<T>::launch_all(int nrhs, hipStream_t s, bool prof) const {
 }
 
 // AIFMM_GRAPH=0 issues the launches one by one instead of replaying a graph
-// (for tools that do not follow stream capture)
+// (for tools that do not follow stream capture); so does a serving loop,
+// whose threads free memory and synchronise while others would capture
+static std::atomic<int> graph_holds{0};
+void prog_graphs_hold(bool on) { graph_holds += on ? 1 : -1; }
 static bool graphs_enabled() {
   static const bool on = [] {
     const char* e = getenv("AIFMM_GRAPH");
     return !(e && e[0] == '0');
   }();
-  return on;
+  return on && graph_holds.load() == 0;
 }
 
 template <class T>

@@ -50,6 +50,8 @@ template <class T>
 void launch_prog(int nprog, const PProg<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, hipStream_t s);
 // largest phase-1 length a stage may use
 int prog_max_m(int elem_bytes);
+// while held (counted), programs launch directly instead of through graphs
+void prog_graphs_hold(bool on);
 
 // One box operation before lowering.
 template <class T>

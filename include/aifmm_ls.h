@@ -84,6 +84,10 @@ int aifmm_ls_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y);
  * last solve time, NCA time, boxes whose sweeps keep the reference's order
  * (badly conditioned L2M_f) */
 int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n);
+/* Device memory the handle holds (arena chunks: operator, factors, sweep and
+ * refinement vectors).  No reference counterpart: the reference runs one
+ * problem per process; a serving loop admits problems by this figure. */
+int aifmm_ls_device_bytes(aifmm_ls_problem* h, int64_t* bytes);
 
 /* LowRank::ACA_only_nodes (ACA.hpp:368-577) on explicit point lists: pivot rows rb
  * and columns cb (rank entries each), un-updated pivot columns Ac (n1 x rank) and rows
@@ -108,6 +112,40 @@ int aifmm_ls_selftest(int op, int m, int n, int k, const double* A, const double
  * of kernels, or a negative code.  reset != 0 clears the totals afterwards. */
 int aifmm_ls_profile_enable(int on);
 int aifmm_ls_profile_read(int reset, int max, double* out, char* names, int names_len);
+
+/* ---- Serving loop.  No reference counterpart: the reference factorises and
+ * solves one problem per process (testFMM2D_find_error.cpp:41-176).  A server
+ * keeps factorised problems resident on the GPU and runs requests on `slots`
+ * worker threads, each on its problem's own HIP stream, so requests for
+ * different problems overlap on the device; queued solves for one problem
+ * with equal tol / max_refine are merged into one multi-RHS solve (at most 64
+ * right-hand sides).  New factorisations are admitted by device memory against
+ * budget_bytes, evicting idle problems least-recently used first; a solve for
+ * an evicted problem factorises it again.  Submission copies the inputs and
+ * returns at once; x must stay valid until the request is waited for.  While a
+ * server exists, sweeps launch directly instead of through HIP graphs. */
+typedef struct aifmm_ls_server aifmm_ls_server;
+int aifmm_ls_server_create(int64_t budget_bytes, int slots, aifmm_ls_server** out);
+/* queue a factorisation (precision 0: complex128, 1: complex64); *problem names
+ * the problem in later solves; *ticket (optional) is the request's ticket */
+int aifmm_ls_server_factor(aifmm_ls_server* s, const aifmm_ls_params* p, int precision, int64_t* problem,
+                           int64_t* ticket);
+/* queue a solve of nrhs right-hand sides (N x nrhs interleaved complex, column-major) */
+int aifmm_ls_server_solve(aifmm_ls_server* s, int64_t problem, int nrhs, const double* b, double* x, double tol,
+                          int max_refine, int64_t* ticket);
+/* block until the request ends; info[0..7] = status (0 ok, -1 failed), latency
+ * (submit to end, s), queue wait (s), run time (s), refinement steps, relative
+ * residual, right-hand sides in the merged solve, 1 if the problem was
+ * factorised again for this request.  Each ticket is waited for once. */
+int aifmm_ls_server_wait(aifmm_ls_server* s, int64_t ticket, double* info);
+/* drop a problem: its memory is freed now, or when its running request ends */
+int aifmm_ls_server_release(aifmm_ls_server* s, int64_t problem);
+/* out[0..9] = resident problems, resident bytes, budget, evictions,
+ * re-factorisations, completed requests, executed batches, requests merged
+ * into another's solve, most batches running at once, queued requests */
+int aifmm_ls_server_stats(aifmm_ls_server* s, double* out, int n);
+/* finish every queued request, then stop the slots and free all problems */
+int aifmm_ls_server_destroy(aifmm_ls_server* s);
 
 #ifdef __cplusplus
 }

@@ -35,22 +35,9 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
       const T* A = tm.A + rb + i;
       const T* x = tm.x ? tm.x + (size_t)tm.ldx * c0 : ylds;
       const size_t la = tm.lda;
-      // eight independent loads in flight: a program's run time is a chain
-      // of dependent memory round trips, and this loop sets their number
+      // two accumulators: alternate products do not wait on each other
       T pa = czero<T>(), pb = czero<T>();
       int j = q;
-      for (; j + 7 * spt < tm.cols; j += 8 * spt) {
-        T a[8], xv[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) a[u] = A[la * (j + u * spt)];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) xv[u] = x[j + u * spt];
-#pragma unroll
-        for (int u = 0; u < 8; u += 2) {
-          pa = cfma(a[u], xv[u], pa);
-          pb = cfma(a[u + 1], xv[u + 1], pb);
-        }
-      }
       for (; j + 3 * spt < tm.cols; j += 4 * spt) {
         const T a0 = A[la * j], a1 = A[la * (j + spt)], a2 = A[la * (j + 2 * spt)], a3 = A[la * (j + 3 * spt)];
         const T x0 = x[j], x1 = x[j + spt], x2 = x[j + 2 * spt], x3 = x[j + 3 * spt];
@@ -233,12 +220,13 @@ static void launch_nth(int nprog, const PProg<T>* progs, const PTerm<T>* terms, 
 // 1024-thread workgroups: with one right-hand side each term of a program
 // gets its own column slices, so a program's loads are all in flight at once.
 // Stages of more than 1024 programs already fill the chip with 256 threads.
+// (Measured on MI355X: choosing 256 threads for stages of small programs
+// shortens their kernels under a profiler but not the replayed sweep.)
 template <class T>
 void launch_prog(int nprog, const PProg<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, hipStream_t s) {
   if (!nprog || !nrhs) return;
   if (ldy < 1) ldy = 1;
-  static const int force = getenv("AIFMM_PROG_NTH") ? atoi(getenv("AIFMM_PROG_NTH")) : 0;
-  if (force == 256 || (force != 1024 && nprog > 1024)) launch_nth<T, 256>(nprog, progs, terms, nrhs, ldy, s);
+  if (nprog > 1024) launch_nth<T, 256>(nprog, progs, terms, nrhs, ldy, s);
   else launch_nth<T, 1024>(nprog, progs, terms, nrhs, ldy, s);
 }
 

@@ -60,6 +60,111 @@ def measured_traffic(kernel):
     return None if t is None else t.get("hbm_bytes_per_launch")
 
 
+def serving_trace(A, seconds, rate, budget_gib, slots, seed, refine_tol, sample):
+    """A shortened serving trace: Poisson arrivals for `seconds`; 30 % are new
+    factorisations (levels 4-6, kappa ~ U[10,60], 2-6 random bumps, complex64),
+    70 % solve 1-64 plane waves against a problem submitted earlier.  The server
+    keeps factorisations resident within the device-memory budget (LRU
+    eviction).  Latency is submission to completion, measured by the server;
+    `sample` solves are checked against an FP64 solution (complex128 factor
+    refined to 1e-13)."""
+    import threading
+    rng = np.random.default_rng(seed)
+    pts = {}
+    for lv in (4, 5, 6):  # point coordinates per depth
+        g = A.LSProblem(lv, 10.0, 7)
+        pts[lv] = g.points()[0]
+        g.close()
+    srv = A.Server(int(budget_gib * (1 << 30)), slots=slots)
+    probs, waves, tickets = [], [], []
+    done = {}
+
+    def waiter():
+        i = 0
+        while True:
+            while i >= len(tickets):
+                if getattr(waiter, "stop", False) and i >= len(tickets):
+                    return
+                time.sleep(0.002)
+            kind, t, meta = tickets[i]
+            try:
+                x, info = srv.wait(t)
+                info["ok"] = True
+            except A.AifmmError as e:
+                x, info = None, {"ok": False, "err": str(e)}
+            if kind == "solve" and meta["check"]:
+                meta["x"] = x
+            done[t] = (kind, info, meta)
+            i += 1
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    t_next = 0.0
+    n_sample = 0
+    while True:
+        t_next += rng.exponential(1.0 / rate)
+        if t_next > seconds:
+            break
+        delay = t0 + t_next - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        if not probs or rng.uniform() < 0.3:
+            lv = int(rng.integers(4, 7))
+            kap = float(rng.uniform(10, 60))
+            bm = bumps_for(int(rng.integers(1 << 30)), int(rng.integers(2, 7)))
+            pid, t = srv.factor(lv, kap, tol_pow=7, qchoice=6, bumps=bm, precision=A.PREC_C64)
+            xy = pts[lv]
+            q = (bm[:, 0][None, :] * np.exp(-((xy[:, 0:1] - bm[:, 2][None, :]) ** 2 + (xy[:, 1:2] - bm[:, 3][None, :]) ** 2)
+                                            / bm[:, 1][None, :] ** 2)).sum(1)
+            th8 = rng.uniform(0, 2 * np.pi, 8)
+            W = -kap * kap * q[:, None] * np.exp(1j * kap * (np.cos(th8)[None, :] * xy[:, 0:1] +
+                                                             np.sin(th8)[None, :] * xy[:, 1:2]))
+            probs.append(dict(pid=pid, levels=lv, kappa=kap, bumps=bm))
+            waves.append(W)
+            tickets.append(("factor", t, {"levels": lv}))
+        else:
+            j = int(rng.integers(len(probs)))
+            k = int(rng.integers(1, 65))
+            B = waves[j][:, np.arange(k) % 8]
+            check = n_sample < sample and probs[j]["levels"] <= 5 and k <= 8
+            n_sample += check
+            t = srv.solve(probs[j]["pid"], B, tol=refine_tol, max_refine=40)
+            tickets.append(("solve", t, {"k": k, "prob": j, "check": check, "B": B if check else None}))
+    waiter.stop = True
+    th.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    sol = [(info, meta) for kind, info, meta in done.values() if kind == "solve"]
+    fac = [info for kind, info, meta in done.values() if kind == "factor"]
+    lat_s = np.array([i["latency"] for i, m in sol if i["ok"]])
+    lat_f = np.array([i["latency"] for i in fac if i["ok"]])
+    nrhs = sum(m["k"] for i, m in sol if i["ok"])
+    failed = sum(not i["ok"] for i, m in sol) + sum(not i["ok"] for i in fac)
+    # accuracy of sampled solves against the FP64 method
+    errs = []
+    for info, meta in sol:
+        if not (info["ok"] and meta["check"]):
+            continue
+        pr = probs[meta["prob"]]
+        g = A.LSProblem(pr["levels"], pr["kappa"], 7, qchoice=6, bumps=pr["bumps"])
+        g.assemble()
+        g.factor(A.PREC_C128)
+        xr, _ = g.solve(meta["B"], tol=1e-13, max_refine=60)
+        g.close()
+        errs.append(float(np.linalg.norm(meta["x"] - xr) / np.linalg.norm(xr)))
+    pct = lambda a, q: round(1e3 * float(np.percentile(a, q)), 3) if a.size else None
+    return {"seconds": seconds, "arrival_rate_per_s": rate, "slots": slots, "budget_gib": budget_gib,
+            "requests": len(tickets), "factor_requests": len(fac), "solve_requests": len(sol), "failed": failed,
+            "rhs_solves_per_s": round(nrhs / wall, 3), "wall_s": round(wall, 3),
+            "solve_p50_ms": pct(lat_s, 50), "solve_p99_ms": pct(lat_s, 99),
+            "factor_p50_ms": pct(lat_f, 50), "factor_p99_ms": pct(lat_f, 99),
+            "evictions": int(st["evictions"]), "refactorisations": int(st["refactorisations"]),
+            "merged_requests": int(st["merged_requests"]), "peak_concurrent": int(st["peak_running"]),
+            "refine_tol": refine_tol, "rel_err_vs_fp64_max": max(errs) if errs else None, "rel_err_samples": len(errs)}
+
+
 def cpu_baseline(levels, kappa, tol_pow, bumps, theta, n_full):
     """The oracle (FP64 CPU restatement of the reference, sequential box order)
     on a smaller tree of the same workload: per-RHS work of the reference =
@@ -98,6 +203,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=5)
     ap.add_argument("--cpu-levels", type=int, default=5, help="tree depth of the CPU baseline sample (0: skip)")
     ap.add_argument("--batch-rhs", type=int, default=16, help="RHS count of the extra batched-throughput line (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the serving trace (0: skip)")
+    ap.add_argument("--serve-rate", type=float, default=4.0, help="serving arrivals per second")
+    ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="serving device-memory budget")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} refused: this solver runs on one GPU; "
@@ -181,6 +289,11 @@ def main():
                 "share_of_profiled_pass": round(dom["ms"] * 1e-3 / total_prof, 4),
                 "profiled_pass_ms_per_step": round(1e3 * total_prof / args.steps, 4)}
 
+    serving = None
+    if args.serve_seconds > 0:
+        serving = serving_trace(A, args.serve_seconds, args.serve_rate, args.serve_budget_gib, 4, args.seed,
+                                1e-8, 3)
+
     cpu = None
     if args.cpu_levels > 0:
         cpu = cpu_baseline(args.cpu_levels, args.kappa, args.tol_pow, bumps, args.theta, g.N)
@@ -212,6 +325,7 @@ def main():
         "factor_flops": st["factor_flops"],
         "reference_order_boxes": int(st["ill_boxes"]),
         "batched_rhs": batched,
+        "serving": serving,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

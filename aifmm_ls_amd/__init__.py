@@ -67,6 +67,9 @@ def lib():
         L.aifmm_ls_set_refinement.argtypes = [P, ctypes.c_int]
         L.aifmm_ls_profile_read.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_char_p, ctypes.c_int]
         L.aifmm_ls_device_bytes.argtypes = [P, ip64]
+        L.aifmm_ls_solve_batch.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
+                                           ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.c_double, ctypes.c_int,
+                                           dp, dp]
         L.aifmm_ls_server_create.argtypes = [ctypes.c_int64, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
         L.aifmm_ls_server_factor.argtypes = [P, ctypes.POINTER(_Params), ctypes.c_int, ip64, ip64]
         L.aifmm_ls_server_solve.argtypes = [P, ctypes.c_int64, ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_int,
@@ -309,6 +312,27 @@ def _params(n_levels, kappa, tol_pow, qchoice, n_cheb, degree, L, bumps, quantiz
         p.bumps = _dp(keep)
         p.n_bumps = keep.shape[0]
     return p, keep
+
+
+def solve_batch(problems, rhs, tol=1e-9, max_refine=30):
+    """Solve one right-hand side per problem for a batch of factorised
+    LSProblems with merged level-synchronous launches (aifmm_ls_solve_batch).
+    Returns (list of x, per-problem relative residuals, stats = [time, steps,
+    max relres])."""
+    n = len(problems)
+    hs = (ctypes.c_void_p * n)(*[p.h.value for p in problems])
+    bs = [np.ascontiguousarray(np.asarray(b, np.complex128)).view(np.float64) for b in rhs]
+    xs = [np.zeros(2 * p.N) for p in problems]
+    for p, b in zip(problems, bs):
+        if b.size != 2 * p.N:
+            raise AifmmError("right-hand side size does not match its problem")
+    dp = ctypes.POINTER(ctypes.c_double)
+    bp = (dp * n)(*[_dp(b) for b in bs])
+    xp = (dp * n)(*[_dp(x) for x in xs])
+    rel = np.zeros(n)
+    st = np.zeros(3)
+    _chk(lib().aifmm_ls_solve_batch(hs, n, bp, xp, float(tol), int(max_refine), _dp(rel), _dp(st)))
+    return [x.view(np.complex128) for x in xs], rel, st
 
 
 SERVER_STAT_NAMES = ["resident_problems", "resident_bytes", "budget", "evictions", "refactorisations", "completed",

@@ -234,6 +234,135 @@ int aifmm_ls_get_solution(aifmm_ls_problem* h, double* x) {
   })
 }
 
+int aifmm_ls_solve_batch(aifmm_ls_problem* const* hs, int n, const double* const* b, double* const* x, double tol,
+                         int max_refine, double* relres, double* stats) {
+  GUARD({
+    if (n < 1 || !hs || !b || !x) throw std::runtime_error("empty batch");
+    const int prec = hs[0]->precision;
+    if (prec < 0) throw std::runtime_error("factor first");
+    for (int i = 0; i < n; ++i) {
+      if (!hs[i] || hs[i]->precision != prec) throw std::runtime_error("batch problems must share the factor precision");
+      for (int j = 0; j < i; ++j)
+        if (hs[j] == hs[i]) throw std::runtime_error("a problem appears twice in the batch");
+    }
+    const double t0 = now_s();
+    // one stream for the whole batch; the problems' own streams are drained first
+    hipStream_t s = hs[0]->pb->stream;
+    for (int i = 0; i < n; ++i) HIP_OK(hipStreamSynchronize(hs[i]->pb->stream));
+    struct Vec4 {
+      double2 *b, *x, *r, *z;
+      int64_t N;
+      int nblk;
+    };
+    std::vector<Vec4> v(n);
+    size_t nparts = 0;
+    for (int i = 0; i < n; ++i) {
+      aifmm_ls_problem* h = hs[i];
+      const int64_t N = h->pb->N;
+      h->io.reset();
+      v[i] = Vec4{h->io.get<double2>(N), h->io.get<double2>(N), h->io.get<double2>(N), h->io.get<double2>(N), N,
+                  resid_blocks(N)};
+      h->rhs = v[i].b;
+      h->sol = v[i].x;
+      h->nrhs = 1;
+      nparts += v[i].nblk;
+      HIP_OK(hipMemcpyAsync(v[i].b, b[i], N * 16, hipMemcpyHostToDevice, s));
+    }
+    Arena a{size_t(16) << 20};
+    double* dpart = a.get<double>(nparts);
+    std::vector<double> part(nparts);
+    // merged sweep and matvec launch sequences
+    Program<double2> sweep64, mat;
+    Program<float2> sweep32;
+    std::vector<const Program<double2>*> ps64, pm;
+    std::vector<const Program<float2>*> ps32;
+    for (int i = 0; i < n; ++i) {
+      aifmm_ls_problem* h = hs[i];
+      if (prec == 0) ps64.push_back(&h->f64->sweep_program(1));
+      else ps32.push_back(&h->f32->sweep_program(1));
+      if (max_refine > 0) {
+        if (!h->h2) h->h2.reset(new H2Op(*h->pb));
+        pm.push_back(&h->h2->program(1));
+      }
+    }
+    if (prec == 0) sweep64.merge_from(ps64), sweep64.finalize(a, s);
+    else sweep32.merge_from(ps32), sweep32.finalize(a, s);
+    if (max_refine > 0) mat.merge_from(pm), mat.finalize(a, s);
+    // M applied to src (per problem) into dst
+    auto sweep = [&](double2* Vec4::*src, double2* Vec4::*dst) {
+      for (int i = 0; i < n; ++i) {
+        if (prec == 0) hs[i]->f64->sweep_load(v[i].*src, 1, s);
+        else hs[i]->f32->sweep_load(v[i].*src, 1, s);
+      }
+      if (prec == 0) sweep64.run(1, s);
+      else sweep32.run(1, s);
+      for (int i = 0; i < n; ++i) {
+        if (prec == 0) hs[i]->f64->sweep_store(v[i].*dst, 1, s);
+        else hs[i]->f32->sweep_store(v[i].*dst, 1, s);
+      }
+    };
+    // per-problem 2-norms from the partial sums in dpart
+    auto norms = [&](std::vector<double>& out) {
+      HIP_OK(hipMemcpyAsync(part.data(), dpart, nparts * 8, hipMemcpyDeviceToHost, s));
+      HIP_OK(hipStreamSynchronize(s));
+      out.assign(n, 0);
+      size_t o = 0;
+      for (int i = 0; i < n; ++i) {
+        double q = 0;
+        for (int k = 0; k < v[i].nblk; ++k) q += part[o + k];
+        out[i] = std::sqrt(q);
+        o += v[i].nblk;
+      }
+    };
+    sweep(&Vec4::b, &Vec4::x);  // x = M b
+    int steps = 0;
+    std::vector<double> rel(n, -1.0), bn, rn;
+    if (max_refine > 0) {
+      size_t o = 0;
+      for (int i = 0; i < n; ++i) {
+        launch_resid(v[i].b, nullptr, nullptr, v[i].N, 1, dpart + o, s);
+        o += v[i].nblk;
+      }
+      norms(bn);
+      // plain refinement in lockstep: r = b - A_H x ; x += M r
+      for (;;) {
+        for (int i = 0; i < n; ++i) hs[i]->h2->load(v[i].x, 1, s);
+        mat.run(1, s);
+        o = 0;
+        for (int i = 0; i < n; ++i) {
+          hs[i]->h2->store(v[i].r, 1, s);
+          launch_resid(v[i].b, v[i].r, v[i].r, v[i].N, 1, dpart + o, s);
+          o += v[i].nblk;
+        }
+        norms(rn);
+        double worst = 0;
+        for (int i = 0; i < n; ++i) {
+          rel[i] = bn[i] > 0 ? rn[i] / bn[i] : rn[i];
+          worst = std::max(worst, rel[i]);
+        }
+        if (worst <= tol || steps >= max_refine) break;
+        sweep(&Vec4::r, &Vec4::z);
+        for (int i = 0; i < n; ++i) launch_axpy(v[i].x, v[i].z, v[i].N, s);
+        ++steps;
+      }
+    }
+    for (int i = 0; i < n; ++i) HIP_OK(hipMemcpyAsync(x[i], v[i].x, v[i].N * 16, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    const double t = now_s() - t0;
+    double worst = 0;
+    for (int i = 0; i < n; ++i) {
+      if (relres) relres[i] = rel[i];
+      worst = std::max(worst, rel[i]);
+      hs[i]->t_solve = t;
+    }
+    if (stats) {
+      stats[0] = t;
+      stats[1] = steps;
+      stats[2] = worst;
+    }
+  })
+}
+
 int aifmm_ls_solve(aifmm_ls_problem* h, int nrhs, const double* b, double* x, double tol, int max_refine,
                    double* stats) {
   int rc = aifmm_ls_set_rhs(h, nrhs, b);

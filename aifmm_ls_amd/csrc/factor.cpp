@@ -971,23 +971,33 @@ void Factor<T>::build_plan(int cap) {
 }
 
 template <class T>
-void Factor<T>::solve(const double2* b, double2* x, int nrhs) {
+Program<T>& Factor<T>::sweep_program(int nrhs) {
   if (B.empty()) throw std::runtime_error("solve before factor");
   if (nrhs > plan_cap) build_plan(std::max(nrhs, 1));
-  hipStream_t s = pb.stream;
+  return *plan;
+}
+
+// b and Bt/Xt share the leading dimension N
+template <class T>
+void Factor<T>::sweep_load(const double2* b, int nrhs, hipStream_t s) {
   const int64_t N = pb.N;
-  // b and Bt/Xt share the leading dimension N
-  if (std::is_same<T, double2>::value) {
-    HIP_OK(hipMemcpyAsync(Bt.p, b, N * nrhs * sizeof(T), hipMemcpyDeviceToDevice, s));
-  } else {
-    launch_d2f(b, (float2*)Bt.p, N * nrhs, s);
-  }
-  plan->run(nrhs, s);
-  if (std::is_same<T, double2>::value) {
-    HIP_OK(hipMemcpyAsync(x, Xt.p, N * nrhs * sizeof(T), hipMemcpyDeviceToDevice, s));
-  } else {
-    launch_f2d((const float2*)Xt.p, x, N * nrhs, s);
-  }
+  if (std::is_same<T, double2>::value) HIP_OK(hipMemcpyAsync(Bt.p, b, N * nrhs * sizeof(T), hipMemcpyDeviceToDevice, s));
+  else launch_d2f(b, (float2*)Bt.p, N * nrhs, s);
+}
+
+template <class T>
+void Factor<T>::sweep_store(double2* x, int nrhs, hipStream_t s) {
+  const int64_t N = pb.N;
+  if (std::is_same<T, double2>::value) HIP_OK(hipMemcpyAsync(x, Xt.p, N * nrhs * sizeof(T), hipMemcpyDeviceToDevice, s));
+  else launch_f2d((const float2*)Xt.p, x, N * nrhs, s);
+}
+
+template <class T>
+void Factor<T>::solve(const double2* b, double2* x, int nrhs) {
+  Program<T>& P = sweep_program(nrhs);
+  sweep_load(b, nrhs, pb.stream);
+  P.run(nrhs, pb.stream);
+  sweep_store(x, nrhs, pb.stream);
 }
 
 template <class T>

@@ -108,6 +108,13 @@ class Factor {
   FactorStats st;
   size_t solve_bytes(int nrhs) const;  // algorithmic bytes of one sweep pair
   size_t device_bytes() const;         // device memory held (arena chunks)
+  // The pieces of solve() for callers that merge the sweeps of several
+  // problems into one launch sequence (all on stream s): load b into the
+  // sweep input, the sweep program (built for at least nrhs columns), store
+  // the sweep output into x.
+  Program<T>& sweep_program(int nrhs);
+  void sweep_load(const double2* b, int nrhs, hipStream_t s);
+  void sweep_store(double2* x, int nrhs, hipStream_t s);
 
   struct FBox {
     int m = 0, r = 0, color = 0;

@@ -107,13 +107,25 @@ void H2Op::build(int nrhs) {
   G.finalize(ws, P.stream);
 }
 
-void H2Op::apply(const double2* x, double2* y, int nrhs) {
+Program<double2>& H2Op::program(int nrhs) {
   if (!pb.assembled) pb.assemble();
   if (!plan || nrhs > cap) build(nrhs);
-  const size_t bytes = (size_t)pb.N * nrhs * sizeof(double2);
-  HIP_OK(hipMemcpyAsync(xin, x, bytes, hipMemcpyDeviceToDevice, pb.stream));
-  plan->run(nrhs, pb.stream);
-  HIP_OK(hipMemcpyAsync(y, yout, bytes, hipMemcpyDeviceToDevice, pb.stream));
+  return *plan;
+}
+
+void H2Op::load(const double2* x, int nrhs, hipStream_t s) {
+  HIP_OK(hipMemcpyAsync(xin, x, (size_t)pb.N * nrhs * sizeof(double2), hipMemcpyDeviceToDevice, s));
+}
+
+void H2Op::store(double2* y, int nrhs, hipStream_t s) {
+  HIP_OK(hipMemcpyAsync(y, yout, (size_t)pb.N * nrhs * sizeof(double2), hipMemcpyDeviceToDevice, s));
+}
+
+void H2Op::apply(const double2* x, double2* y, int nrhs) {
+  Program<double2>& P = program(nrhs);
+  load(x, nrhs, pb.stream);
+  P.run(nrhs, pb.stream);
+  store(y, nrhs, pb.stream);
 }
 
 size_t H2Op::bytes(int nrhs) const {

@@ -21,6 +21,12 @@ class H2Op {
   // algorithmic bytes of one application (operator blocks + vectors)
   size_t bytes(int nrhs) const;
   size_t device_bytes() const { return ws.reserved(); }
+  // The pieces of apply() for callers that merge several problems' launch
+  // sequences (all on stream s): the program (built for nrhs columns), load
+  // x into its input, store its output into y.
+  Program<double2>& program(int nrhs);
+  void load(const double2* x, int nrhs, hipStream_t s);
+  void store(double2* y, int nrhs, hipStream_t s);
 
  private:
   Problem& pb;

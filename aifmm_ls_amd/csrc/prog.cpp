@@ -89,6 +89,30 @@ void Program<T>::add(const BoxOp<T>& op) {
 }
 
 template <class T>
+void Program<T>::merge_from(const std::vector<const Program<T>*>& ps) {
+  size_t ns = 0;
+  for (const Program<T>* p : ps) ns = std::max(ns, p->st_.size());
+  st_.assign(ns, Stage{});
+  for (const Program<T>* p : ps) {
+    const int off = (int)terms_.size();
+    terms_.insert(terms_.end(), p->terms_.begin(), p->terms_.end());
+    for (size_t i = 0; i < p->st_.size(); ++i) {
+      Stage& g = st_[i];
+      for (PProg<T> q : p->st_[i].progs) {
+        q.t0 += off;
+        q.t1 += off;
+        for (int o = 0; o < q.nout; ++o) {
+          q.u0[o] += off;
+          q.u1[o] += off;
+        }
+        g.progs.push_back(q);
+      }
+      g.maxm = std::max(g.maxm, p->st_[i].maxm);
+    }
+  }
+}
+
+template <class T>
 void Program<T>::finalize(Arena& a, hipStream_t s) {
   close();
   const int lim = prog_max_m(sizeof(T));

@@ -101,6 +101,11 @@ class Program {
 
   // Copy descriptors to the device (arena-owned memory).
   void finalize(Arena& a, hipStream_t s);
+  // Build (before finalize) a program whose stage i runs stage i of every
+  // program in ps: the programs belong to independent problems, so their
+  // stages can share launches (one level-synchronous launch sequence for a
+  // batch of problems).
+  void merge_from(const std::vector<const Program<T>*>& ps);
   // Launch every stage in order on s (asynchronous).  The launch sequence is
   // captured once per nrhs into a HIP graph and replayed; with kernel
   // profiling on, the launches are issued one by one instead.

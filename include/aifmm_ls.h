@@ -71,6 +71,17 @@ int aifmm_ls_solve(aifmm_ls_problem* h, int nrhs, const double* b, double* x, do
 int aifmm_ls_set_rhs(aifmm_ls_problem* h, int nrhs, const double* b);
 int aifmm_ls_solve_resident(aifmm_ls_problem* h, double tol, int max_refine, double* stats);
 int aifmm_ls_get_solution(aifmm_ls_problem* h, double* x);
+/* A batch of independent problems solved together: eliminate_phase_efficient's RHS
+ * part + back_substitution_phase (FMM2DTree_find_error.hpp:2407-2532) of every
+ * problem, with stage i of each problem's sweep merged into one launch (one
+ * level-synchronous launch sequence for the batch), and likewise the FP64
+ * matvecs of plain refinement x += M (b - A_H x), run in lockstep until every
+ * problem's relative residual is <= tol or max_refine steps.  One right-hand side
+ * per problem: b[i], x[i] hold N_i complex values; relres (n, optional) gets
+ * each problem's final relative residual; stats = time, steps, max relres.  All
+ * problems must be factorised in the same precision. */
+int aifmm_ls_solve_batch(aifmm_ls_problem* const* hs, int n, const double* const* b, double* const* x, double tol,
+                         int max_refine, double* relres, double* stats);
 /* Refinement method for max_refine > 0 (no reference counterpart: the reference
  * solves once, FMM2DTree_find_error.hpp:2407-2532): 0 = flexible GMRES with the
  * factorisation as right preconditioner (default), 1 = plain iterative

@@ -60,9 +60,10 @@ RefineResult Refiner::run(const Precond& M, const double2* b, double2* x, int nr
   RefineResult R;
   M(b, x);
   if (maxit <= 0) return R;
-  // restart length: up to 20, the Krylov vectors within 4 GiB
+  // restart length: up to 20, the Krylov vectors (V and Z) within 32 GiB of
+  // the 288 GB HBM (at 256 right-hand sides of N = 589,824: 5)
   int m = std::min(maxit, 20);
-  const size_t lim = (size_t(4) << 30) / (vb * 32);
+  const size_t lim = (size_t(32) << 30) / (vb * 32);
   m = std::max(1, std::min<int>(m, (int)std::max<size_t>(lim, 2) - 1));
   ws_.reset();
   double2* V = ws_.get<double2>((size_t)(m + 1) * vb);

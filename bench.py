@@ -165,6 +165,86 @@ def serving_trace(A, seconds, rate, budget_gib, slots, seed, refine_tol, sample)
             "refine_tol": refine_tol, "rel_err_vs_fp64_max": max(errs) if errs else None, "rel_err_samples": len(errs)}
 
 
+def uneven_batch(A, seed, refine_tol, n=32):
+    """The uneven batch: n independent problems (levels 3-5, kappa ~ U[5,40],
+    tolerance 1e-6, 2-6 bumps, one plane wave at a random angle each),
+    factorised in complex64 and solved together with merged level-synchronous
+    launches and lockstep FP64 refinement, beside the same problems solved one
+    after another (flexible GMRES each).  Host-to-device copies are included on
+    both sides."""
+    rng = np.random.default_rng(seed)
+    gs, bs, cfg, t_fac = [], [], [], 0.0
+    for _ in range(n):
+        nb = int(rng.integers(2, 7))
+        bm = np.c_[rng.uniform(0.2, 1.5, nb), rng.uniform(0.05, 0.2, nb), rng.uniform(-0.7, 0.7, (nb, 2))]
+        c = dict(levels=int(rng.integers(3, 6)), kappa=float(rng.uniform(5, 40)), bumps=bm,
+                 theta=float(rng.uniform(0, 2 * np.pi)))
+        g = A.LSProblem(c["levels"], c["kappa"], 6, qchoice=6, bumps=bm)
+        t0 = time.perf_counter()
+        g.assemble()
+        g.factor(A.PREC_C64)
+        t_fac += time.perf_counter() - t0
+        gs.append(g)
+        bs.append(g.rhs(c["theta"]))
+        cfg.append(c)
+    A.solve_batch(gs, bs, tol=refine_tol, max_refine=40)  # builds the merged launch sequences
+    t0 = time.perf_counter()
+    xs, rr, st = A.solve_batch(gs, bs, tol=refine_tol, max_refine=40)
+    t_batch = time.perf_counter() - t0
+    for g, b in zip(gs, bs):
+        g.solve(b, tol=refine_tol, max_refine=40)
+    t0 = time.perf_counter()
+    for g, b in zip(gs, bs):
+        g.solve(b, tol=refine_tol, max_refine=40)
+    t_seq = time.perf_counter() - t0
+    errs = []
+    for i in [i for i, c in enumerate(cfg) if c["levels"] == 5][:2]:
+        gs[i].factor(A.PREC_C128)
+        xr, _ = gs[i].solve(bs[i], tol=1e-13, max_refine=60)
+        errs.append(float(np.linalg.norm(xs[i] - xr) / np.linalg.norm(xr)))
+    for g in gs:
+        g.close()
+    return {"problems": n, "levels_3_4_5": [sum(c["levels"] == lv for c in cfg) for lv in (3, 4, 5)],
+            "factor_s_total": round(t_fac, 3), "batch_solve_ms": round(1e3 * t_batch, 3),
+            "sequential_solve_ms": round(1e3 * t_seq, 3), "problems_per_s": round(n / t_batch, 3),
+            "speedup_vs_sequential": round(t_seq / t_batch, 3), "lockstep_steps": int(st[1]),
+            "max_relres": float(st[2]), "refine_tol": refine_tol, "rel_err_vs_fp64_max": max(errs) if errs else None}
+
+
+def many_rhs(A, seed, nrhs, refine_tol):
+    """Many right-hand sides: one factorisation at 7 levels (N = 589,824),
+    kappa 60, tolerance 1e-6, a smooth 4-bump contrast, complex64 factors;
+    then nrhs plane waves at equally spaced angles in [0, 2 pi) in one refined
+    solve (resident in HBM before timing)."""
+    rng = np.random.default_rng(seed)
+    bm = np.c_[rng.uniform(0.2, 1.0, 4), rng.uniform(0.15, 0.25, 4), rng.uniform(-0.5, 0.5, (4, 2))]
+    g = A.LSProblem(7, 60.0, 6, qchoice=6, bumps=bm)
+    t0 = time.perf_counter()
+    g.assemble()
+    t_asm = time.perf_counter() - t0
+    g.factor(A.PREC_C64)
+    t_fac = g.stats()["t_factor"]
+    B = np.stack([g.rhs(t) for t in 2 * np.pi * np.arange(nrhs) / nrhs], axis=1)
+    g.set_rhs(B)
+    g.solve_resident(refine_tol, 0)  # plan and vectors for nrhs columns
+    t0 = time.perf_counter()
+    s = g.solve_resident(refine_tol, 40)
+    t_solve = time.perf_counter() - t0
+    X = g.get_solution()
+    cols = [0, nrhs // 3]
+    g.factor(A.PREC_C128)
+    errs = []
+    for c in cols:
+        xr, _ = g.solve(B[:, c], tol=1e-13, max_refine=60)
+        errs.append(float(np.linalg.norm(X[:, c] - xr) / np.linalg.norm(xr)))
+    N = g.N
+    g.close()
+    return {"levels": 7, "n": int(N), "kappa": 60.0, "tol": 1e-6, "nrhs": nrhs, "assemble_s": round(t_asm, 3),
+            "factor_s": round(t_fac, 4), "solve_s": round(t_solve, 4), "rhs_solves_per_s": round(nrhs / t_solve, 3),
+            "refine_steps": int(s[1]), "max_relres": float(s[2]), "refine_tol": refine_tol,
+            "rel_err_vs_fp64_max": max(errs), "rel_err_columns": cols}
+
+
 def cpu_baseline(levels, kappa, tol_pow, bumps, theta, n_full):
     """The oracle (FP64 CPU restatement of the reference, sequential box order)
     on a smaller tree of the same workload: per-RHS work of the reference =
@@ -203,6 +283,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=5)
     ap.add_argument("--cpu-levels", type=int, default=5, help="tree depth of the CPU baseline sample (0: skip)")
     ap.add_argument("--batch-rhs", type=int, default=16, help="RHS count of the extra batched-throughput line (0: skip)")
+    ap.add_argument("--batch-problems", type=int, default=32, help="problems in the uneven-batch line (0: skip)")
+    ap.add_argument("--many-rhs", type=int, default=256, help="RHS count of the 7-level many-RHS line (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the serving trace (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="serving arrivals per second")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="serving device-memory budget")
@@ -289,6 +371,10 @@ def main():
                 "share_of_profiled_pass": round(dom["ms"] * 1e-3 / total_prof, 4),
                 "profiled_pass_ms_per_step": round(1e3 * total_prof / args.steps, 4)}
 
+    g.close()
+    batch32 = uneven_batch(A, args.seed, 1e-8, args.batch_problems) if args.batch_problems > 0 else None
+    many = many_rhs(A, args.seed, args.many_rhs, 1e-8) if args.many_rhs > 0 else None
+
     serving = None
     if args.serve_seconds > 0:
         serving = serving_trace(A, args.serve_seconds, args.serve_rate, args.serve_budget_gib, 4, args.seed,
@@ -325,6 +411,8 @@ def main():
         "factor_flops": st["factor_flops"],
         "reference_order_boxes": int(st["ill_boxes"]),
         "batched_rhs": batched,
+        "uneven_batch": batch32,
+        "many_rhs": many,
         "serving": serving,
         "roofline": roof,
         "cpu_baseline": cpu,

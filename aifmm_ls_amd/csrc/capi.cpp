@@ -324,7 +324,10 @@ int aifmm_ls_solve_batch(aifmm_ls_problem* const* hs, int n, const double* const
         o += v[i].nblk;
       }
       norms(bn);
-      // plain refinement in lockstep: r = b - A_H x ; x += M r
+      // plain refinement in lockstep: r = b - A_H x ; x += M r, for a few
+      // steps; it converges only while |I - A_H M| < 1, so problems still above
+      // tol afterwards finish with their own flexible GMRES from x
+      const int lock_max = std::min(max_refine, 6);
       for (;;) {
         for (int i = 0; i < n; ++i) hs[i]->h2->load(v[i].x, 1, s);
         mat.run(1, s);
@@ -340,11 +343,27 @@ int aifmm_ls_solve_batch(aifmm_ls_problem* const* hs, int n, const double* const
           rel[i] = bn[i] > 0 ? rn[i] / bn[i] : rn[i];
           worst = std::max(worst, rel[i]);
         }
-        if (worst <= tol || steps >= max_refine) break;
+        if (worst <= tol || steps >= lock_max) break;
         sweep(&Vec4::r, &Vec4::z);
         for (int i = 0; i < n; ++i) launch_axpy(v[i].x, v[i].z, v[i].N, s);
         ++steps;
       }
+      HIP_OK(hipStreamSynchronize(s));
+      int extra = 0;
+      for (int i = 0; i < n; ++i) {
+        if (rel[i] <= tol) continue;
+        aifmm_ls_problem* h = hs[i];
+        if (!h->ref) h->ref.reset(new Refiner(*h->pb, *h->h2));
+        auto M = [&](const double2* bb, double2* xx) {
+          if (prec == 0) h->f64->solve(bb, xx, 1);
+          else h->f32->solve(bb, xx, 1);
+        };
+        RefineResult R = h->ref->run(M, v[i].b, v[i].x, 1, tol, std::max(1, max_refine - steps), 0, true);
+        HIP_OK(hipStreamSynchronize(h->pb->stream));
+        rel[i] = R.relres;
+        extra = std::max(extra, R.steps);
+      }
+      steps += extra;
     }
     for (int i = 0; i < n; ++i) HIP_OK(hipMemcpyAsync(x[i], v[i].x, v[i].N * 16, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));

@@ -4,6 +4,8 @@
 // formation and strided block copies.  One workgroup (4 wave64) per matrix
 // or per output tile; every batch is described by a device array of
 // descriptors built on the host.
+#include <type_traits>
+
 #include "common.h"
 #include "cplx_dev.h"
 #include "kernels.h"
@@ -106,6 +108,73 @@ __global__ __launch_bounds__(NT) void k_gemm(const TileTask* __restrict__ tiles,
         *c = v;
       }
     }
+}
+
+// ------------------------------------------------------- GEMM on MFMA (complex64)
+// C = alpha op(A) op(B) + beta C with one wave64 per 32x32 output tile on the
+// matrix cores: the complex product is four real products (4M split,
+// Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br) of v_mfma_f32_32x32x2f32 with FP32
+// accumulation.  A 32x16 tile of op(A) and a 16x32 tile of op(B) are staged
+// through LDS per K step with coalesced loads (conjugate transposes applied
+// while staging).  The MFMA is fed B^T as its first operand and A^T as its
+// second, so it forms C^T: a lane then holds one row of C in 16 columns, and
+// the stores of a wave run down columns of the column-major C.
+//   v_mfma_f32_32x32x2f32 operands: lane l gives P[l % 32][l / 32] of the
+//   32x2 first operand and Q[l / 32][l % 32] of the 2x32 second; result
+//   element r of lane l is D[8 (r / 4) + 4 (l / 32) + r % 4][l % 32].
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__global__ __launch_bounds__(64) void k_gemm_mfma_c64(const TileTask* __restrict__ tiles,
+                                                      const GemmDesc<float2>* __restrict__ ds) {
+  const TileTask tt = tiles[blockIdx.x];
+  const GemmDesc<float2> d = ds[tt.g];
+  __shared__ float2 As[16][33];  // [k][m]
+  __shared__ float2 Bs[16][33];  // [k][n]
+  const int l = threadIdx.x;
+  const int m0 = tt.tm * 32, n0 = tt.tn * 32;
+  f32x16 cr, ci;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) cr[r] = ci[r] = 0.f;
+  for (int k0 = 0; k0 < d.k; k0 += 16) {
+    for (int e = l; e < 512; e += 64) {
+      // op(A)(m, k): contiguous along m for A, along k for A^H
+      const int mm = d.opA == 0 ? (e & 31) : (e >> 4), ka = d.opA == 0 ? (e >> 5) : (e & 15);
+      const int gm = m0 + mm, gka = k0 + ka;
+      float2 v = make_float2(0.f, 0.f);
+      if (gm < d.m && gka < d.k) v = d.opA == 0 ? d.A[gm + (size_t)d.lda * gka] : cconj(d.A[gka + (size_t)d.lda * gm]);
+      As[ka][mm] = v;
+      // op(B)(k, n): contiguous along k for B, along n for B^H
+      const int nn = d.opB == 0 ? (e >> 4) : (e & 31), kb = d.opB == 0 ? (e & 15) : (e >> 5);
+      const int gn = n0 + nn, gkb = k0 + kb;
+      float2 w = make_float2(0.f, 0.f);
+      if (gn < d.n && gkb < d.k) w = d.opB == 0 ? d.B[gkb + (size_t)d.ldb * gn] : cconj(d.B[gn + (size_t)d.ldb * gkb]);
+      Bs[kb][nn] = w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 8; ++ks) {
+      const float2 a = As[2 * ks + (l >> 5)][l & 31];  // A^T operand: Q[k][m]
+      const float2 b = Bs[2 * ks + (l >> 5)][l & 31];  // B^T operand: P[n][k]
+      cr = __builtin_amdgcn_mfma_f32_32x32x2f32(b.x, a.x, cr, 0, 0, 0);
+      cr = __builtin_amdgcn_mfma_f32_32x32x2f32(-b.y, a.y, cr, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f32_32x32x2f32(b.y, a.x, ci, 0, 0, 0);
+      ci = __builtin_amdgcn_mfma_f32_32x32x2f32(b.x, a.y, ci, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const float2 al = from2<float2>(d.alpha), be = from2<float2>(d.beta);
+  const bool bz = d.beta.x == 0.0 && d.beta.y == 0.0;
+  const int gm = m0 + (l & 31);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int gn = n0 + 8 * (r >> 2) + 4 * (l >> 5) + (r & 3);
+    if (gm < d.m && gn < d.n) {
+      float2* c = d.C + gm + (size_t)d.ldc * gn;
+      float2 v = cmul(al, make_float2(cr[r], ci[r]));
+      if (!bz) v = cfma(be, *c, v);
+      *c = v;
+    }
+  }
 }
 
 // ------------------------------------------------------------------ LU
@@ -483,10 +552,14 @@ __global__ void k_f2d(const float2* __restrict__ s, double2* __restrict__ d, int
 }
 
 // ------------------------------------------------------------ launchers
+// complex64 products run on the matrix cores; complex128 on the vector ALUs
 template <class T>
 void launch_gemm(int ntiles, const TileTask* tiles, const GemmDesc<T>* d, hipStream_t s) {
   if (!ntiles) return;
-  hipLaunchKernelGGL(k_gemm<T>, dim3(ntiles), dim3(NT), 0, s, tiles, d);
+  if constexpr (std::is_same<T, float2>::value)
+    hipLaunchKernelGGL(k_gemm_mfma_c64, dim3(ntiles), dim3(64), 0, s, tiles, d);
+  else
+    hipLaunchKernelGGL(k_gemm<T>, dim3(ntiles), dim3(NT), 0, s, tiles, d);
   HIP_OK(hipGetLastError());
 }
 template <class T>

@@ -53,12 +53,13 @@ void Refiner::lincomb(double2* w, const double2* V, int64_t vstride, int nv, con
   HIP_OK(hipStreamSynchronize(pb_.stream));
 }
 
-RefineResult Refiner::run(const Precond& M, const double2* b, double2* x, int nr, double tol, int maxit, int method) {
+RefineResult Refiner::run(const Precond& M, const double2* b, double2* x, int nr, double tol, int maxit, int method,
+                          bool x_given) {
   const int64_t N = pb_.N;
   hipStream_t s = pb_.stream;
   const size_t vb = (size_t)N * nr;
   RefineResult R;
-  M(b, x);
+  if (!x_given) M(b, x);
   if (maxit <= 0) return R;
   // restart length: up to 20, the Krylov vectors (V and Z) within 32 GiB of
   // the 288 GB HBM (at 256 right-hand sides of N = 589,824: 5)

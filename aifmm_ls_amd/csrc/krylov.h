@@ -26,8 +26,10 @@ class Refiner {
   Refiner(Problem& pb, H2Op& A) : pb_(pb), A_(A) { ws_.set_chunk(Arena::chunk_for(pb.N, 1 << 10, 16 << 20, 256 << 20)); }
   size_t device_bytes() const { return ws_.reserved(); }
   // x <- solution of A_H x = b to relative residual tol (at most maxit
-  // preconditioner applications after x0 = M b).  method 0: FGMRES, 1: Richardson.
-  RefineResult run(const Precond& M, const double2* b, double2* x, int nrhs, double tol, int maxit, int method);
+  // preconditioner applications after x0 = M b, or after the x passed in when
+  // x_given).  method 0: FGMRES, 1: Richardson.
+  RefineResult run(const Precond& M, const double2* b, double2* x, int nrhs, double tol, int maxit, int method,
+                   bool x_given = false);
 
  private:
   Problem& pb_;

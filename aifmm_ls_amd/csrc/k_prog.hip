@@ -5,12 +5,16 @@
 // segment of the matrix per step (coalesced), and slice partial sums are
 // combined through LDS.  The phase-1 vector lives in LDS; right-hand sides
 // are processed CH columns at a time.
+#include <cstddef>
+
 #include "cplx_dev.h"
 #include "prog.h"
 
 namespace aifmm {
 
-constexpr int PROG_LDS = 64 * 1024;
+// dynamic LDS of a program (the phase-1 vector and the reduction buffer); the
+// staged term descriptors take the rest of the 64 KB a workgroup may address
+constexpr int PROG_LDS = 63 * 1024;
 
 // Sum over terms of alpha_t A_t[rb + i, :] x_t for this thread's row i and
 // column slice s (columns s, s + ns, ...), four independent loads in flight.
@@ -136,13 +140,11 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
   return s == 0 && i < rr;
 }
 
+// One box program run by the whole workgroup (ylds: ldy x CH phase-1
+// vector, red: NTH x CH reduction buffer, both in LDS).
 template <class T, int CH, int NTH>
-__global__ __launch_bounds__(NTH) void k_prog(const PProg<T>* __restrict__ progs, const PTerm<T>* __restrict__ terms,
-                                              int nrhs, int ldy) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  T* ylds = (T*)smem;
-  T* red = ylds + (size_t)ldy * CH;
-  const PProg<T>* P = progs + blockIdx.x;
+__device__ __forceinline__ void run_program(const PProg<T>* __restrict__ P, const PTerm<T>* __restrict__ terms,
+                                            int nrhs, int ldy, T* ylds, T* red) {
   const int m = P->m;
   for (int c0 = 0; c0 < nrhs; c0 += CH) {
     const int nc = min(CH, nrhs - c0);
@@ -195,6 +197,27 @@ __global__ __launch_bounds__(NTH) void k_prog(const PProg<T>* __restrict__ progs
   }
 }
 
+// The program's header is read with scalar loads from its slot; its term
+// descriptors are staged in LDS in one round of 16-byte loads (a program with
+// more terms than a slot holds reads them from the stage-wide array).
+template <class T, int CH, int NTH>
+__global__ __launch_bounds__(NTH) void k_prog(const PSlot<T>* __restrict__ slots, const PTerm<T>* __restrict__ terms,
+                                              int nrhs, int ldy, int slot_words) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ PTerm<T> tl[PSLOT_TERMS];
+  static_assert(sizeof(PSlot<T>) % 16 == 0 && offsetof(PSlot<T>, t) % 16 == 0, "slot layout");
+  T* ylds = (T*)smem;
+  T* red = ylds + (size_t)ldy * CH;
+  const PSlot<T>* S = slots + blockIdx.x;
+  const int tw = slot_words - (int)(offsetof(PSlot<T>, t) / 16);
+  const int4* src = (const int4*)S->t;
+  int4* dst = (int4*)tl;
+  for (int i = threadIdx.x; i < tw; i += NTH) dst[i] = src[i];
+  __syncthreads();
+  if (S->p.ext) run_program<T, CH, NTH>(&S->p, terms, nrhs, ldy, ylds, red);
+  else run_program<T, CH, NTH>(&S->p, tl, nrhs, ldy, ylds, red);
+}
+
 // right-hand-side columns per pass: 1 for a single RHS, else the most that
 // fit the LDS with the reduction buffer of NTH threads
 static int pick_ch(int nrhs, int ldy, int eb, int nth) {
@@ -207,13 +230,16 @@ static int pick_ch(int nrhs, int ldy, int eb, int nth) {
 int prog_max_m(int elem_bytes) { return PROG_LDS / elem_bytes - 1024; }
 
 template <class T, int NTH>
-static void launch_nth(int nprog, const PProg<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, hipStream_t s) {
+static void launch_nth(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, int sw,
+                       hipStream_t s) {
+  if (sw < 1 || sw > (int)(sizeof(PSlot<T>) / 16)) throw std::runtime_error("bad program slot size");
   const int ch = pick_ch(nrhs, ldy, sizeof(T), NTH);
   if (!ch) throw std::runtime_error("box program vector of " + std::to_string(ldy) + " rows exceeds LDS");
   const size_t lds = (size_t)(ldy + NTH) * ch * sizeof(T);
-  if (ch == 4) hipLaunchKernelGGL((k_prog<T, 4, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy);
-  else if (ch == 2) hipLaunchKernelGGL((k_prog<T, 2, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy);
-  else hipLaunchKernelGGL((k_prog<T, 1, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy);
+  if (ch == 4) hipLaunchKernelGGL((k_prog<T, 4, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
+  else if (ch == 2)
+    hipLaunchKernelGGL((k_prog<T, 2, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
+  else hipLaunchKernelGGL((k_prog<T, 1, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
   HIP_OK(hipGetLastError());
 }
 
@@ -223,14 +249,15 @@ static void launch_nth(int nprog, const PProg<T>* progs, const PTerm<T>* terms, 
 // (Measured on MI355X: choosing 256 threads for stages of small programs
 // shortens their kernels under a profiler but not the replayed sweep.)
 template <class T>
-void launch_prog(int nprog, const PProg<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, hipStream_t s) {
+void launch_prog(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, int slot_words,
+                 hipStream_t s) {
   if (!nprog || !nrhs) return;
   if (ldy < 1) ldy = 1;
-  if (nprog > 1024) launch_nth<T, 256>(nprog, progs, terms, nrhs, ldy, s);
-  else launch_nth<T, 1024>(nprog, progs, terms, nrhs, ldy, s);
+  if (nprog > 1024) launch_nth<T, 256>(nprog, progs, terms, nrhs, ldy, slot_words, s);
+  else launch_nth<T, 1024>(nprog, progs, terms, nrhs, ldy, slot_words, s);
 }
 
-template void launch_prog<double2>(int, const PProg<double2>*, const PTerm<double2>*, int, int, hipStream_t);
-template void launch_prog<float2>(int, const PProg<float2>*, const PTerm<float2>*, int, int, hipStream_t);
+template void launch_prog<double2>(int, const PSlot<double2>*, const PTerm<double2>*, int, int, int, hipStream_t);
+template void launch_prog<float2>(int, const PSlot<float2>*, const PTerm<float2>*, int, int, int, hipStream_t);
 
 }  // namespace aifmm

@@ -3,6 +3,7 @@
 #include "prog.h"
 
 #include <algorithm>
+#include <cstddef>
 #include <atomic>
 #include <cstdlib>
 #include <string>
@@ -125,8 +126,31 @@ void Program<T>::finalize(Arena& a, hipStream_t s) {
       throw std::runtime_error("box program of " + std::to_string(g.maxm) + " rows exceeds the LDS limit " +
                                std::to_string(lim));
     if (g.progs.empty()) continue;
-    g.dprogs = a.get<PProg<T>>(g.progs.size());
-    HIP_OK(hipMemcpyAsync(g.dprogs, g.progs.data(), g.progs.size() * sizeof(PProg<T>), hipMemcpyHostToDevice, s));
+    std::vector<PSlot<T>> sl(g.progs.size());
+    int inl = 0;  // most terms any program of the stage keeps in its slot
+    for (size_t i = 0; i < g.progs.size(); ++i) {
+      PProg<T> q = g.progs[i];
+      int n = q.t1 - q.t0;
+      for (int o = 0; o < q.nout; ++o) n += q.u1[o] - q.u0[o];
+      q.ext = n > PSLOT_TERMS;
+      if (!q.ext) inl = std::max(inl, n);
+      if (!q.ext) {
+        // terms copied into the slot, indices rebased to it
+        int k = 0;
+        auto take = [&](int& lo, int& hi) {
+          const int b = k;
+          for (int t = lo; t < hi; ++t) sl[i].t[k++] = terms_[t];
+          lo = b;
+          hi = k;
+        };
+        take(q.t0, q.t1);
+        for (int o = 0; o < q.nout; ++o) take(q.u0[o], q.u1[o]);
+      }
+      sl[i].p = q;
+    }
+    g.slot_words = (int)((offsetof(PSlot<T>, t) + inl * sizeof(PTerm<T>) + 15) / 16);
+    g.dslots = a.get<PSlot<T>>(sl.size());
+    HIP_OK(hipMemcpyAsync(g.dslots, sl.data(), sl.size() * sizeof(PSlot<T>), hipMemcpyHostToDevice, s));
   }
   HIP_OK(hipStreamSynchronize(s));
 }
@@ -169,7 +193,7 @@ void Program<T>::launch_all(int nrhs, hipStream_t s, bool prof) const {
     if (g.progs.empty()) continue;
     ProfScope ps(s, std::is_same<T, float2>::value ? "k_prog<c64>" : "k_prog<c128>",
                  prof ? stage_bytes(g.progs, terms_, nrhs) : 0, prof ? stage_flops(g.progs, terms_, nrhs) : 0);
-    launch_prog<T>((int)g.progs.size(), g.dprogs, dterms_, nrhs, g.maxm, s);
+    launch_prog<T>((int)g.progs.size(), g.dslots, dterms_, nrhs, g.maxm, g.slot_words, s);
   }
 }
 

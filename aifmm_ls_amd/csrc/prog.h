@@ -43,11 +43,25 @@ struct PProg {
   int nout;
   T* out[2];
   int ldo[2], rows[2], u0[2], u1[2];
+  int ext;  // in a PSlot: 1 if the term indices refer to the stage-wide array
 };
 
-// Launch one stage: nprog workgroups; ldy >= every program's m.
+// A program as the kernel reads it: the descriptor and (up to PSLOT_TERMS) its
+// terms in one fixed-size slot, so one workgroup fetches all of it in a
+// single round of loads (a program with more terms keeps its indices into the
+// shared term array, ext = 1).
+constexpr int PSLOT_TERMS = 18;
 template <class T>
-void launch_prog(int nprog, const PProg<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, hipStream_t s);
+struct PSlot {
+  PProg<T> p;
+  PTerm<T> t[PSLOT_TERMS];
+};
+
+// Launch one stage: nprog workgroups; ldy >= every program's m; the first
+// slot_words 16-byte words of each slot hold the descriptor and its terms.
+template <class T>
+void launch_prog(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, int slot_words,
+                 hipStream_t s);
 // largest phase-1 length a stage may use
 int prog_max_m(int elem_bytes);
 // while held (counted), programs launch directly instead of through graphs
@@ -124,7 +138,8 @@ class Program {
   struct Stage {
     std::vector<PProg<T>> progs;
     int maxm = 0;
-    PProg<T>* dprogs = nullptr;
+    PSlot<T>* dslots = nullptr;
+    int slot_words = 0;
   };
   std::vector<Stage> st_;
   std::vector<PTerm<T>> terms_;

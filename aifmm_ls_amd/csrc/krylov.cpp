@@ -1,5 +1,7 @@
 #include "krylov.h"
 
+#include "batch.h"
+
 #include <algorithm>
 #include <cmath>
 #include <complex>
@@ -46,11 +48,15 @@ void Refiner::dots(const double2* V, int64_t vstride, int nv, const double2* w, 
 
 void Refiner::lincomb(double2* w, const double2* V, int64_t vstride, int nv, const std::vector<double2>& coef,
                       const std::vector<double2>& beta, int nrhs) {
-  // the host vectors are copied before this returns (pageable source)
-  if (nv) HIP_OK(hipMemcpyAsync(dcoef_, coef.data(), (size_t)nv * nrhs * 16, hipMemcpyHostToDevice, pb_.stream));
-  HIP_OK(hipMemcpyAsync(dbeta_, beta.data(), (size_t)nrhs * 16, hipMemcpyHostToDevice, pb_.stream));
+  // coefficients go through the pinned staging ring: the copies are queued
+  // without waiting, and the host vectors may go out of scope on return
+  if (nv) {
+    const size_t b = (size_t)nv * nrhs * 16;
+    HIP_OK(hipMemcpyAsync(dcoef_, stage_pinned(coef.data(), b, pb_.stream), b, hipMemcpyHostToDevice, pb_.stream));
+  }
+  const size_t bb = (size_t)nrhs * 16;
+  HIP_OK(hipMemcpyAsync(dbeta_, stage_pinned(beta.data(), bb, pb_.stream), bb, hipMemcpyHostToDevice, pb_.stream));
   launch_lincomb(w, V, vstride, nv, dcoef_, dbeta_, pb_.N, nrhs, pb_.stream);
-  HIP_OK(hipStreamSynchronize(pb_.stream));
 }
 
 RefineResult Refiner::run(const Precond& M, const double2* b, double2* x, int nr, double tol, int maxit, int method,

@@ -129,18 +129,47 @@ RefineResult Refiner::run(const Precond& M, const double2* b, double2* x, int nr
       M(vj, zj);
       A_.apply(zj, w, nr);
       ++R.steps;
-      std::vector<double2> h, h2, neg((size_t)(j + 1) * nr);
+      std::vector<double2> h, h2w, neg((size_t)(j + 1) * nr);
       std::vector<double2> keep(nr, one);
       dots(V, (int64_t)vb, j + 1, w, nr, h);
       for (size_t q = 0; q < neg.size(); ++q) neg[q] = make_double2(-h[q].x, -h[q].y);
       lincomb(w, V, (int64_t)vb, j + 1, neg, keep, nr);
-      dots(V, (int64_t)vb, j + 1, w, nr, h2);
+      // second pass: the products with V_0..V_j and, as basis slot j + 1 is w
+      // itself, |w|^2; then |w - V h2|^2 = |w|^2 - |h2|^2 (V orthonormal)
+      // gives the new norm, and one update both projects and normalises
+      dots(V, (int64_t)vb, j + 2, w, nr, h2w);
+      std::vector<double2> h2((size_t)(j + 1) * nr);
+      std::vector<double> hn(nr);
+      bool exact = false;
+      for (int c = 0; c < nr; ++c) {
+        double w2 = h2w[(size_t)c * (j + 2) + j + 1].x, p2 = 0;
+        for (int i = 0; i <= j; ++i) {
+          const double2 v = h2w[(size_t)c * (j + 2) + i];
+          h2[(size_t)c * (j + 1) + i] = v;
+          p2 += v.x * v.x + v.y * v.y;
+        }
+        const double r2 = w2 - p2;
+        if (!(r2 > 1e-8 * w2)) exact = true;  // cancellation: measure instead
+        hn[c] = std::sqrt(std::max(r2, 0.0));
+      }
       for (size_t q = 0; q < neg.size(); ++q) neg[q] = make_double2(-h2[q].x, -h2[q].y);
-      lincomb(w, V, (int64_t)vb, j + 1, neg, keep, nr);
-      std::vector<double> hn;
-      norms(w, nr, hn);
-      for (int c = 0; c < nr; ++c) sc[c] = make_double2(hn[c] > 0 ? 1.0 / hn[c] : 0.0, 0);
-      lincomb(w, nullptr, 0, 0, {}, sc, nr);
+      if (exact) {
+        lincomb(w, V, (int64_t)vb, j + 1, neg, keep, nr);
+        norms(w, nr, hn);
+        for (int c = 0; c < nr; ++c) sc[c] = make_double2(hn[c] > 0 ? 1.0 / hn[c] : 0.0, 0);
+        lincomb(w, nullptr, 0, 0, {}, sc, nr);
+      } else {
+        for (int c = 0; c < nr; ++c) {
+          const double s = hn[c] > 0 ? 1.0 / hn[c] : 0.0;
+          sc[c] = make_double2(s, 0);
+          for (int i = 0; i <= j; ++i) {
+            double2& v = neg[(size_t)c * (j + 1) + i];
+            v.x *= s;
+            v.y *= s;
+          }
+        }
+        lincomb(w, V, (int64_t)vb, j + 1, neg, sc, nr);
+      }
       double worst = 0;
       for (int c = 0; c < nr; ++c) {
         for (int i = 0; i <= j; ++i) {

@@ -116,7 +116,7 @@ __global__ __launch_bounds__(NT) void k_gemm(const TileTask* __restrict__ tiles,
 // Re += Ar Br - Ai Bi, Im += Ar Bi + Ai Br) of v_mfma_f32_32x32x2f32 with FP32
 // accumulation.  A 32x16 tile of op(A) and a 16x32 tile of op(B) are staged
 // through LDS per K step with coalesced loads (conjugate transposes applied
-// while staging).  The MFMA is fed B^T as its first operand and A^T as its
+// while staging), double-buffered through registers.  The MFMA is fed B^T as its first operand and A^T as its
 // second, so it forms C^T: a lane then holds one row of C in 16 columns, and
 // the stores of a wave run down columns of the column-major C.
 //   v_mfma_f32_32x32x2f32 operands: lane l gives P[l % 32][l / 32] of the
@@ -135,22 +135,39 @@ __global__ __launch_bounds__(64) void k_gemm_mfma_c64(const TileTask* __restrict
   f32x16 cr, ci;
 #pragma unroll
   for (int r = 0; r < 16; ++r) cr[r] = ci[r] = 0.f;
-  for (int k0 = 0; k0 < d.k; k0 += 16) {
-    for (int e = l; e < 512; e += 64) {
+  // K blocks of 16 are double-buffered through registers: the loads of block
+  // k0 + 16 are in flight while block k0 runs on the matrix cores
+  float2 ra[8], rb[8];
+  auto fetch = [&](int k0) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = l + 64 * u;
       // op(A)(m, k): contiguous along m for A, along k for A^H
       const int mm = d.opA == 0 ? (e & 31) : (e >> 4), ka = d.opA == 0 ? (e >> 5) : (e & 15);
       const int gm = m0 + mm, gka = k0 + ka;
-      float2 v = make_float2(0.f, 0.f);
-      if (gm < d.m && gka < d.k) v = d.opA == 0 ? d.A[gm + (size_t)d.lda * gka] : cconj(d.A[gka + (size_t)d.lda * gm]);
-      As[ka][mm] = v;
+      ra[u] = make_float2(0.f, 0.f);
+      if (gm < d.m && gka < d.k)
+        ra[u] = d.opA == 0 ? d.A[gm + (size_t)d.lda * gka] : cconj(d.A[gka + (size_t)d.lda * gm]);
       // op(B)(k, n): contiguous along k for B, along n for B^H
       const int nn = d.opB == 0 ? (e >> 4) : (e & 31), kb = d.opB == 0 ? (e & 15) : (e >> 5);
       const int gn = n0 + nn, gkb = k0 + kb;
-      float2 w = make_float2(0.f, 0.f);
-      if (gn < d.n && gkb < d.k) w = d.opB == 0 ? d.B[gkb + (size_t)d.ldb * gn] : cconj(d.B[gn + (size_t)d.ldb * gkb]);
-      Bs[kb][nn] = w;
+      rb[u] = make_float2(0.f, 0.f);
+      if (gn < d.n && gkb < d.k)
+        rb[u] = d.opB == 0 ? d.B[gkb + (size_t)d.ldb * gn] : cconj(d.B[gn + (size_t)d.ldb * gkb]);
+    }
+  };
+  if (d.k > 0) fetch(0);
+  for (int k0 = 0; k0 < d.k; k0 += 16) {
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int e = l + 64 * u;
+      const int mm = d.opA == 0 ? (e & 31) : (e >> 4), ka = d.opA == 0 ? (e >> 5) : (e & 15);
+      const int nn = d.opB == 0 ? (e >> 4) : (e & 31), kb = d.opB == 0 ? (e & 15) : (e >> 5);
+      As[ka][mm] = ra[u];
+      Bs[kb][nn] = rb[u];
     }
     __syncthreads();
+    if (k0 + 16 < d.k) fetch(k0 + 16);
 #pragma unroll
     for (int ks = 0; ks < 8; ++ks) {
       const float2 a = As[2 * ks + (l >> 5)][l & 31];  // A^T operand: Q[k][m]

@@ -22,6 +22,16 @@ struct aifmm_ls_problem {
   std::unique_ptr<Factor<float2>> f32;
   std::unique_ptr<H2Op> h2;
   std::unique_ptr<Refiner> ref;
+  std::unique_ptr<BatchRefiner> bref;  // batches led by this handle
+  // merged launch sequences of the last batch led by this handle, reused
+  // while its members and their programs stay the same
+  struct BatchPlan {
+    std::vector<uint64_t> key;  // ids of the members' sweep and matvec programs
+    std::unique_ptr<Arena> arena;
+    std::unique_ptr<Program<double2>> s64, mat;
+    std::unique_ptr<Program<float2>> s32;
+  };
+  std::unique_ptr<BatchPlan> bplan;
   int refine_method = 0;
   int precision = -1;
   Arena sc{size_t(16) << 20};
@@ -249,121 +259,81 @@ int aifmm_ls_solve_batch(aifmm_ls_problem* const* hs, int n, const double* const
     // one stream for the whole batch; the problems' own streams are drained first
     hipStream_t s = hs[0]->pb->stream;
     for (int i = 0; i < n; ++i) HIP_OK(hipStreamSynchronize(hs[i]->pb->stream));
-    struct Vec4 {
-      double2 *b, *x, *r, *z;
+    struct Vec2 {
+      double2 *b, *x;
       int64_t N;
-      int nblk;
     };
-    std::vector<Vec4> v(n);
-    size_t nparts = 0;
+    std::vector<Vec2> v(n);
     for (int i = 0; i < n; ++i) {
       aifmm_ls_problem* h = hs[i];
       const int64_t N = h->pb->N;
       h->io.reset();
-      v[i] = Vec4{h->io.get<double2>(N), h->io.get<double2>(N), h->io.get<double2>(N), h->io.get<double2>(N), N,
-                  resid_blocks(N)};
+      v[i] = Vec2{h->io.get<double2>(N), h->io.get<double2>(N), N};
       h->rhs = v[i].b;
       h->sol = v[i].x;
       h->nrhs = 1;
-      nparts += v[i].nblk;
       HIP_OK(hipMemcpyAsync(v[i].b, b[i], N * 16, hipMemcpyHostToDevice, s));
     }
-    Arena a{size_t(16) << 20};
-    double* dpart = a.get<double>(nparts);
-    std::vector<double> part(nparts);
-    // merged sweep and matvec launch sequences
-    Program<double2> sweep64, mat;
-    Program<float2> sweep32;
+    // merged sweep and matvec launch sequences (cached on the leading handle)
     std::vector<const Program<double2>*> ps64, pm;
     std::vector<const Program<float2>*> ps32;
+    std::vector<uint64_t> key;
     for (int i = 0; i < n; ++i) {
       aifmm_ls_problem* h = hs[i];
+      if (!h->h2) h->h2.reset(new H2Op(*h->pb));
       if (prec == 0) ps64.push_back(&h->f64->sweep_program(1));
       else ps32.push_back(&h->f32->sweep_program(1));
-      if (max_refine > 0) {
-        if (!h->h2) h->h2.reset(new H2Op(*h->pb));
-        pm.push_back(&h->h2->program(1));
+      pm.push_back(&h->h2->program(1));
+      key.push_back(prec == 0 ? ps64.back()->id() : ps32.back()->id());
+      key.push_back(pm.back()->id());
+    }
+    auto& bp = hs[0]->bplan;
+    if (!bp || bp->key != key) {
+      bp.reset(new aifmm_ls_problem::BatchPlan());
+      bp->key = key;
+      bp->arena.reset(new Arena(size_t(16) << 20));
+      bp->mat.reset(new Program<double2>());
+      bp->mat->merge_from(pm);
+      bp->mat->finalize(*bp->arena, s);
+      if (prec == 0) {
+        bp->s64.reset(new Program<double2>());
+        bp->s64->merge_from(ps64);
+        bp->s64->finalize(*bp->arena, s);
+      } else {
+        bp->s32.reset(new Program<float2>());
+        bp->s32->merge_from(ps32);
+        bp->s32->finalize(*bp->arena, s);
       }
     }
-    if (prec == 0) sweep64.merge_from(ps64), sweep64.finalize(a, s);
-    else sweep32.merge_from(ps32), sweep32.finalize(a, s);
-    if (max_refine > 0) mat.merge_from(pm), mat.finalize(a, s);
-    // M applied to src (per problem) into dst
-    auto sweep = [&](double2* Vec4::*src, double2* Vec4::*dst) {
+    Program<double2>& mat = *bp->mat;
+    // lockstep flexible GMRES; preconditioner and operator of all members in
+    // merged launch sequences
+    auto Mall = [&](const std::vector<const double2*>& src, const std::vector<double2*>& dst) {
       for (int i = 0; i < n; ++i) {
-        if (prec == 0) hs[i]->f64->sweep_load(v[i].*src, 1, s);
-        else hs[i]->f32->sweep_load(v[i].*src, 1, s);
+        if (prec == 0) hs[i]->f64->sweep_load(src[i], 1, s);
+        else hs[i]->f32->sweep_load(src[i], 1, s);
       }
-      if (prec == 0) sweep64.run(1, s);
-      else sweep32.run(1, s);
+      if (prec == 0) bp->s64->run(1, s);
+      else bp->s32->run(1, s);
       for (int i = 0; i < n; ++i) {
-        if (prec == 0) hs[i]->f64->sweep_store(v[i].*dst, 1, s);
-        else hs[i]->f32->sweep_store(v[i].*dst, 1, s);
-      }
-    };
-    // per-problem 2-norms from the partial sums in dpart
-    auto norms = [&](std::vector<double>& out) {
-      HIP_OK(hipMemcpyAsync(part.data(), dpart, nparts * 8, hipMemcpyDeviceToHost, s));
-      HIP_OK(hipStreamSynchronize(s));
-      out.assign(n, 0);
-      size_t o = 0;
-      for (int i = 0; i < n; ++i) {
-        double q = 0;
-        for (int k = 0; k < v[i].nblk; ++k) q += part[o + k];
-        out[i] = std::sqrt(q);
-        o += v[i].nblk;
+        if (prec == 0) hs[i]->f64->sweep_store(dst[i], 1, s);
+        else hs[i]->f32->sweep_store(dst[i], 1, s);
       }
     };
-    sweep(&Vec4::b, &Vec4::x);  // x = M b
+    auto Aall = [&](const std::vector<const double2*>& src, const std::vector<double2*>& dst) {
+      for (int i = 0; i < n; ++i) hs[i]->h2->load(src[i], 1, s);
+      mat.run(1, s);
+      for (int i = 0; i < n; ++i) hs[i]->h2->store(dst[i], 1, s);
+    };
+    std::vector<BatchVec> bv(n);
+    for (int i = 0; i < n; ++i) bv[i] = BatchVec{v[i].N, v[i].b, v[i].x};
+    if (!hs[0]->bref) hs[0]->bref.reset(new BatchRefiner());
+    std::vector<RefineResult> R = hs[0]->bref->run(bv, Mall, Aall, tol, max_refine, s);
     int steps = 0;
-    std::vector<double> rel(n, -1.0), bn, rn;
-    if (max_refine > 0) {
-      size_t o = 0;
-      for (int i = 0; i < n; ++i) {
-        launch_resid(v[i].b, nullptr, nullptr, v[i].N, 1, dpart + o, s);
-        o += v[i].nblk;
-      }
-      norms(bn);
-      // plain refinement in lockstep: r = b - A_H x ; x += M r, for a few
-      // steps; it converges only while |I - A_H M| < 1, so problems still above
-      // tol afterwards finish with their own flexible GMRES from x
-      const int lock_max = std::min(max_refine, 6);
-      for (;;) {
-        for (int i = 0; i < n; ++i) hs[i]->h2->load(v[i].x, 1, s);
-        mat.run(1, s);
-        o = 0;
-        for (int i = 0; i < n; ++i) {
-          hs[i]->h2->store(v[i].r, 1, s);
-          launch_resid(v[i].b, v[i].r, v[i].r, v[i].N, 1, dpart + o, s);
-          o += v[i].nblk;
-        }
-        norms(rn);
-        double worst = 0;
-        for (int i = 0; i < n; ++i) {
-          rel[i] = bn[i] > 0 ? rn[i] / bn[i] : rn[i];
-          worst = std::max(worst, rel[i]);
-        }
-        if (worst <= tol || steps >= lock_max) break;
-        sweep(&Vec4::r, &Vec4::z);
-        for (int i = 0; i < n; ++i) launch_axpy(v[i].x, v[i].z, v[i].N, s);
-        ++steps;
-      }
-      HIP_OK(hipStreamSynchronize(s));
-      int extra = 0;
-      for (int i = 0; i < n; ++i) {
-        if (rel[i] <= tol) continue;
-        aifmm_ls_problem* h = hs[i];
-        if (!h->ref) h->ref.reset(new Refiner(*h->pb, *h->h2));
-        auto M = [&](const double2* bb, double2* xx) {
-          if (prec == 0) h->f64->solve(bb, xx, 1);
-          else h->f32->solve(bb, xx, 1);
-        };
-        RefineResult R = h->ref->run(M, v[i].b, v[i].x, 1, tol, std::max(1, max_refine - steps), 0, true);
-        HIP_OK(hipStreamSynchronize(h->pb->stream));
-        rel[i] = R.relres;
-        extra = std::max(extra, R.steps);
-      }
-      steps += extra;
+    std::vector<double> rel(n, -1.0);
+    for (int i = 0; i < n; ++i) {
+      rel[i] = R[i].relres;
+      steps = std::max(steps, R[i].steps);
     }
     for (int i = 0; i < n; ++i) HIP_OK(hipMemcpyAsync(x[i], v[i].x, v[i].N * 16, hipMemcpyDeviceToHost, s));
     HIP_OK(hipStreamSynchronize(s));

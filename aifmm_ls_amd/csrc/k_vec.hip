@@ -111,6 +111,37 @@ void launch_lincomb(double2* w, const double2* V, int64_t vstride, int nv, const
   HIP_OK(hipGetLastError());
 }
 
+// Sums of per-block partials for several problems at once: out[i * nv + q] =
+// sum over b < nblk[i] of part[off[i] + q * nblk[i] + b] (complex), or of the
+// real partials ((double*)(part + off[i]))[b] when real (nv = 1).  One wave64
+// per (problem, q); the order of the additions is fixed.
+__global__ __launch_bounds__(64) void k_sum_partials(const double2* __restrict__ part, const int64_t* __restrict__ off,
+                                                     const int* __restrict__ nblk, int nv, int real,
+                                                     double2* __restrict__ out) {
+  const int i = blockIdx.x, q = blockIdx.y, l = threadIdx.x;
+  const int nb = nblk[i];
+  double2 t = make_double2(0.0, 0.0);
+  if (real) {
+    const double* p = (const double*)(part + off[i]);
+    for (int b = l; b < nb; b += 64) t.x += p[b];
+  } else {
+    const double2* p = part + off[i] + (int64_t)q * nb;
+    for (int b = l; b < nb; b += 64) t.x += p[b].x, t.y += p[b].y;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    t.x += __shfl_xor(t.x, o);
+    t.y += __shfl_xor(t.y, o);
+  }
+  if (l == 0) out[(int64_t)i * nv + q] = t;
+}
+
+void launch_sum_partials(const double2* part, const int64_t* off, const int* nblk, int n, int nv, int real,
+                         double2* out, hipStream_t s) {
+  if (!n || !nv) return;
+  hipLaunchKernelGGL(k_sum_partials, dim3(n, nv), dim3(64), 0, s, part, off, nblk, nv, real, out);
+  HIP_OK(hipGetLastError());
+}
+
 int resid_blocks(int64_t n) {
   int64_t b = (n + VNT - 1) / VNT;
   return (int)(b < 1024 ? (b < 1 ? 1 : b) : 1024);

@@ -184,6 +184,10 @@ void launch_multidot(const double2* V, int64_t vstride, int nv, const double2* w
                      double2* partial, hipStream_t s);
 void launch_lincomb(double2* w, const double2* V, int64_t vstride, int nv, const double2* coef, const double2* beta,
                     int64_t n, int nrhs, hipStream_t s);
+// out[i * nv + q] = sum_b part[off[i] + q * nblk[i] + b] for n problems (real
+// != 0: the nblk[i] real partials at (double*)(part + off[i]), nv = 1)
+void launch_sum_partials(const double2* part, const int64_t* off, const int* nblk, int n, int nv, int real,
+                         double2* out, hipStream_t s);
 void launch_d2f(const double2* src, float2* dst, int64_t n, hipStream_t s);
 void launch_f2d(const float2* src, double2* dst, int64_t n, hipStream_t s);
 

@@ -223,4 +223,214 @@ RefineResult Refiner::run(const Precond& M, const double2* b, double2* x, int nr
   return R;
 }
 
+std::vector<RefineResult> BatchRefiner::run(const std::vector<BatchVec>& v, const Apply& M, const Apply& A, double tol,
+                                            int maxit, hipStream_t s) {
+  const int n = (int)v.size();
+  std::vector<RefineResult> R(n);
+  std::vector<const double2*> src(n);
+  std::vector<double2*> dst(n);
+  for (int i = 0; i < n; ++i) src[i] = v[i].b, dst[i] = v[i].x;
+  M(src, dst);  // x = M b
+  if (maxit <= 0 || n == 0) return R;
+  int m = std::min(maxit, 20);
+  ws_.reset();
+  struct Mem {
+    double2 *V, *Z, *r;
+    int nblk;
+    size_t poff;  // offset of this member's partials
+  };
+  std::vector<Mem> mem(n);
+  size_t ptot = 0;
+  for (int i = 0; i < n; ++i) {
+    const size_t N = (size_t)v[i].N;
+    mem[i].V = ws_.get<double2>((m + 1) * N);
+    mem[i].Z = ws_.get<double2>(m * N);
+    mem[i].r = ws_.get<double2>(N);
+    mem[i].nblk = resid_blocks(v[i].N);
+    mem[i].poff = ptot;
+    ptot += (size_t)mem[i].nblk * (m + 2);
+  }
+  double2* dpart = ws_.get<double2>(ptot);
+  double2* dcoef = ws_.get<double2>((size_t)n * (m + 1));
+  double2* dbeta = ws_.get<double2>(n);
+  double2* dsum = ws_.get<double2>((size_t)n * (m + 2));
+  int64_t* doff = ws_.get<int64_t>(n);
+  int* dnb = ws_.get<int>(n);
+  {
+    std::vector<int64_t> off(n);
+    std::vector<int> nb(n);
+    for (int i = 0; i < n; ++i) off[i] = (int64_t)mem[i].poff, nb[i] = mem[i].nblk;
+    HIP_OK(hipMemcpyAsync(doff, stage_pinned(off.data(), n * 8, s), n * 8, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(dnb, stage_pinned(nb.data(), n * 4, s), n * 4, hipMemcpyHostToDevice, s));
+  }
+  std::vector<double2> sums((size_t)n * (m + 2));
+  // the members' partial sums reduced on the device, then nv values each to
+  // the host: out[i][0..nv)
+  auto fetch = [&](int nv, std::vector<std::vector<double2>>& out) {
+    launch_sum_partials(dpart, doff, dnb, n, nv, 0, dsum, s);
+    HIP_OK(hipMemcpyAsync(sums.data(), dsum, (size_t)n * nv * 16, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    out.assign(n, {});
+    for (int i = 0; i < n; ++i) out[i].assign(sums.begin() + (size_t)i * nv, sums.begin() + (size_t)(i + 1) * nv);
+  };
+  // 2-norms from the real partials of launch_resid
+  auto fetch_norms = [&](std::vector<double>& out) {
+    launch_sum_partials(dpart, doff, dnb, n, 1, 1, dsum, s);
+    HIP_OK(hipMemcpyAsync(sums.data(), dsum, (size_t)n * 16, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    out.resize(n);
+    for (int i = 0; i < n; ++i) out[i] = std::sqrt(sums[i].x);
+  };
+  // w_i = beta_i w_i + sum_q coef_i[q] V_i[q] for every member (coefficients
+  // staged through the pinned ring; no host wait)
+  auto lincomb = [&](const std::vector<double2*>& w, const std::vector<int>& nv,
+                     const std::vector<std::vector<double2>>& coef, const std::vector<double2>& beta) {
+    std::vector<double2> hc((size_t)n * (m + 1), make_double2(0, 0));
+    for (int i = 0; i < n; ++i)
+      for (int q = 0; q < nv[i]; ++q) hc[(size_t)i * (m + 1) + q] = coef[i][q];
+    HIP_OK(hipMemcpyAsync(dcoef, stage_pinned(hc.data(), hc.size() * 16, s), hc.size() * 16, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(dbeta, stage_pinned(beta.data(), n * 16, s), n * 16, hipMemcpyHostToDevice, s));
+    for (int i = 0; i < n; ++i)
+      launch_lincomb(w[i], nv[i] ? mem[i].V : nullptr, v[i].N, nv[i], dcoef + (size_t)i * (m + 1), dbeta + i,
+                     v[i].N, 1, s);
+  };
+  std::vector<double> bn(n), rn(n);
+  for (int i = 0; i < n; ++i) launch_resid(v[i].b, nullptr, nullptr, v[i].N, 1, (double*)(dpart + mem[i].poff), s);
+  fetch_norms(bn);
+  int steps = 0;
+  for (;;) {
+    // true residuals r = b - A x
+    for (int i = 0; i < n; ++i) src[i] = v[i].x, dst[i] = mem[i].r;
+    A(src, dst);
+    for (int i = 0; i < n; ++i)
+      launch_resid(v[i].b, mem[i].r, mem[i].r, v[i].N, 1, (double*)(dpart + mem[i].poff), s);
+    fetch_norms(rn);
+    double worst = 0;
+    for (int i = 0; i < n; ++i) {
+      R[i].relres = bn[i] > 0 ? rn[i] / bn[i] : rn[i];
+      R[i].steps = steps;
+      worst = std::max(worst, R[i].relres);
+    }
+    if (worst <= tol || steps >= maxit) break;
+    // V_0 = r / |r|
+    std::vector<int> nv0(n, 0);
+    std::vector<std::vector<double2>> none(n);
+    std::vector<double2> sc(n);
+    for (int i = 0; i < n; ++i) {
+      HIP_OK(hipMemcpyAsync(mem[i].V, mem[i].r, v[i].N * 16, hipMemcpyDeviceToDevice, s));
+      sc[i] = make_double2(rn[i] > 0 ? 1.0 / rn[i] : 0.0, 0);
+      dst[i] = mem[i].V;
+    }
+    lincomb(dst, nv0, none, sc);
+    std::vector<std::vector<std::complex<double>>> H(n, std::vector<std::complex<double>>((size_t)(m + 1) * m, 0.0)),
+        g(n, std::vector<std::complex<double>>(m + 1, 0.0)), cs(n, std::vector<std::complex<double>>(m, 0.0)),
+        sn(n, std::vector<std::complex<double>>(m, 0.0));
+    for (int i = 0; i < n; ++i) g[i][0] = rn[i];
+    int k = 0;
+    for (int j = 0; j < m && steps < maxit; ++j) {
+      for (int i = 0; i < n; ++i) src[i] = mem[i].V + (size_t)j * v[i].N, dst[i] = mem[i].Z + (size_t)j * v[i].N;
+      M(src, dst);
+      for (int i = 0; i < n; ++i) src[i] = mem[i].Z + (size_t)j * v[i].N, dst[i] = mem[i].V + (size_t)(j + 1) * v[i].N;
+      A(src, dst);
+      ++steps;
+      std::vector<int> nvj(n, j + 1);
+      std::vector<double2*> w(n);
+      for (int i = 0; i < n; ++i) {
+        w[i] = mem[i].V + (size_t)(j + 1) * v[i].N;
+        launch_multidot(mem[i].V, v[i].N, j + 1, w[i], v[i].N, 1, dpart + mem[i].poff, s);
+      }
+      std::vector<std::vector<double2>> h, h2w, neg(n);
+      fetch(j + 1, h);
+      std::vector<double2> one(n, make_double2(1, 0));
+      for (int i = 0; i < n; ++i) {
+        neg[i].resize(j + 1);
+        for (int q = 0; q <= j; ++q) neg[i][q] = make_double2(-h[i][q].x, -h[i][q].y);
+      }
+      lincomb(w, nvj, neg, one);
+      // second pass with |w|^2 (w is basis slot j + 1)
+      for (int i = 0; i < n; ++i) launch_multidot(mem[i].V, v[i].N, j + 2, w[i], v[i].N, 1, dpart + mem[i].poff, s);
+      fetch(j + 2, h2w);
+      std::vector<double> hn(n);
+      bool exact = false;
+      for (int i = 0; i < n; ++i) {
+        double p2 = 0;
+        for (int q = 0; q <= j; ++q) p2 += h2w[i][q].x * h2w[i][q].x + h2w[i][q].y * h2w[i][q].y;
+        const double w2 = h2w[i][j + 1].x, r2 = w2 - p2;
+        if (!(r2 > 1e-8 * w2)) exact = true;
+        hn[i] = std::sqrt(std::max(r2, 0.0));
+        for (int q = 0; q <= j; ++q) neg[i][q] = make_double2(-h2w[i][q].x, -h2w[i][q].y);
+      }
+      if (exact) {
+        lincomb(w, nvj, neg, one);
+        for (int i = 0; i < n; ++i) launch_resid(w[i], nullptr, nullptr, v[i].N, 1, (double*)(dpart + mem[i].poff), s);
+        fetch_norms(hn);
+        for (int i = 0; i < n; ++i) sc[i] = make_double2(hn[i] > 0 ? 1.0 / hn[i] : 0.0, 0);
+        lincomb(w, nv0, none, sc);
+      } else {
+        for (int i = 0; i < n; ++i) {
+          const double sI = hn[i] > 0 ? 1.0 / hn[i] : 0.0;
+          sc[i] = make_double2(sI, 0);
+          for (auto& c : neg[i]) c.x *= sI, c.y *= sI;
+        }
+        lincomb(w, nvj, neg, sc);
+      }
+      double worst_est = 0;
+      for (int i = 0; i < n; ++i) {
+        auto Hc = [&](int a, int b) -> std::complex<double>& { return H[i][(size_t)a * m + b]; };
+        for (int q = 0; q <= j; ++q)
+          Hc(q, j) = std::complex<double>(h[i][q].x + h2w[i][q].x, h[i][q].y + h2w[i][q].y);
+        Hc(j + 1, j) = hn[i];
+        for (int q = 0; q < j; ++q) {
+          const auto c = cs[i][q], sn_ = sn[i][q];
+          const auto a = Hc(q, j), bb = Hc(q + 1, j);
+          Hc(q, j) = c * a + sn_ * bb;
+          Hc(q + 1, j) = -std::conj(sn_) * a + c * bb;
+        }
+        const auto h1 = Hc(j, j), h2v = Hc(j + 1, j);
+        const double t = std::sqrt(std::norm(h1) + std::norm(h2v));
+        std::complex<double> cj = 0, sj = 1;
+        if (t > 0 && std::abs(h1) > 0) {
+          cj = std::abs(h1) / t;
+          sj = (h1 / std::abs(h1)) * std::conj(h2v) / t;
+        }
+        cs[i][j] = cj;
+        sn[i][j] = sj;
+        Hc(j, j) = cj * h1 + sj * h2v;
+        Hc(j + 1, j) = 0;
+        g[i][j + 1] = -std::conj(sj) * g[i][j];
+        g[i][j] = cj * g[i][j];
+        const double est = std::abs(g[i][j + 1]);
+        worst_est = std::max(worst_est, bn[i] > 0 ? est / bn[i] : est);
+      }
+      k = j + 1;
+      if (worst_est <= 0.5 * tol) break;
+    }
+    // x_i += Z_i y_i
+    std::vector<std::vector<double2>> y(n);
+    std::vector<int> nvk(n, k);
+    for (int i = 0; i < n; ++i) {
+      std::vector<std::complex<double>> yy(k, 0.0);
+      for (int a = k - 1; a >= 0; --a) {
+        std::complex<double> acc = g[i][a];
+        for (int b = a + 1; b < k; ++b) acc -= H[i][(size_t)a * m + b] * yy[b];
+        const auto d = H[i][(size_t)a * m + a];
+        yy[a] = std::abs(d) > 0 ? acc / d : 0.0;
+      }
+      y[i].resize(k);
+      for (int a = 0; a < k; ++a) y[i][a] = make_double2(yy[a].real(), yy[a].imag());
+    }
+    // lincomb works on V; point it at Z for this update
+    std::vector<double2> one(n, make_double2(1, 0));
+    std::vector<double2> hc((size_t)n * (m + 1), make_double2(0, 0));
+    for (int i = 0; i < n; ++i)
+      for (int q = 0; q < k; ++q) hc[(size_t)i * (m + 1) + q] = y[i][q];
+    HIP_OK(hipMemcpyAsync(dcoef, stage_pinned(hc.data(), hc.size() * 16, s), hc.size() * 16, hipMemcpyHostToDevice, s));
+    HIP_OK(hipMemcpyAsync(dbeta, stage_pinned(one.data(), n * 16, s), n * 16, hipMemcpyHostToDevice, s));
+    for (int i = 0; i < n; ++i)
+      launch_lincomb(v[i].x, mem[i].Z, v[i].N, k, dcoef + (size_t)i * (m + 1), dbeta + i, v[i].N, 1, s);
+  }
+  HIP_OK(hipStreamSynchronize(s));
+  return R;
+}
+
 }  // namespace aifmm

@@ -47,4 +47,29 @@ class Refiner {
   double2* dbeta_ = nullptr;
 };
 
+// Flexible GMRES(m) for a batch of independent problems in lockstep, one
+// right-hand side each: every step applies all members' preconditioners in
+// one merged launch sequence and all their operators in another (the
+// callbacks), and the members' dot products come back to the host together.
+// Each member has its own Arnoldi process; members that have converged keep
+// stepping until all have (their corrections are then below the tolerance).
+struct BatchVec {
+  int64_t N;
+  const double2* b;
+  double2* x;
+};
+
+class BatchRefiner {
+ public:
+  // dst[i] = Op_i(src[i]) for every member, asynchronous on stream s
+  using Apply = std::function<void(const std::vector<const double2*>& src, const std::vector<double2*>& dst)>;
+  // x_i <- solution of A_i x = b_i to relative residual tol, starting from
+  // x_i = M_i b_i; at most maxit preconditioner applications after that.
+  std::vector<RefineResult> run(const std::vector<BatchVec>& v, const Apply& M, const Apply& A, double tol, int maxit,
+                                hipStream_t s);
+
+ private:
+  Arena ws_{size_t(256) << 20};
+};
+
 }  // namespace aifmm

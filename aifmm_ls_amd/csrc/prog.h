@@ -16,6 +16,8 @@
 // outputs from it), because one workgroup alone cannot keep enough loads in
 // flight for it.
 #pragma once
+#include <atomic>
+#include <cstdint>
 #include <vector>
 
 #include "arena.h"
@@ -124,7 +126,9 @@ class Program {
   // captured once per nrhs into a HIP graph and replayed; with kernel
   // profiling on, the launches are issued one by one instead.
   void run(int nrhs, hipStream_t s);
-  Program() = default;
+  Program() : id_(next_id()) {}
+  // distinct for every program built in the process (cache keys)
+  uint64_t id() const { return id_; }
   Program(const Program&) = delete;
   Program& operator=(const Program&) = delete;
   ~Program();
@@ -141,6 +145,11 @@ class Program {
     PSlot<T>* dslots = nullptr;
     int slot_words = 0;
   };
+  static uint64_t next_id() {
+    static std::atomic<uint64_t> n{0};
+    return ++n;
+  }
+  const uint64_t id_;
   std::vector<Stage> st_;
   std::vector<PTerm<T>> terms_;
   PTerm<T>* dterms_ = nullptr;

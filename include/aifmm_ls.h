@@ -75,9 +75,9 @@ int aifmm_ls_get_solution(aifmm_ls_problem* h, double* x);
  * part + back_substitution_phase (FMM2DTree_find_error.hpp:2407-2532) of every
  * problem, with stage i of each problem's sweep merged into one launch (one
  * level-synchronous launch sequence for the batch), and likewise the FP64
- * matvecs of plain refinement x += M (b - A_H x), run in lockstep for up to 6
- * steps; a problem still above tol then continues with its own flexible GMRES
- * (up to max_refine steps in all).  One right-hand side
+ * matvecs of a flexible GMRES that runs every problem's Arnoldi process in
+ * lockstep until each relative residual is <= tol (at most max_refine
+ * preconditioner applications after x0 = M b).  One right-hand side
  * per problem: b[i], x[i] hold N_i complex values; relres (n, optional) gets
  * each problem's final relative residual; stats = time, steps, max relres.  All
  * problems must be factorised in the same precision. */

@@ -30,6 +30,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+_T0 = time.perf_counter()
+
+
+def note(msg):
+    """One progress line on stderr as each phase ends (stdout carries only the
+    result line)."""
+    print(f"bench.py [{time.perf_counter() - _T0:7.1f} s] {msg}", file=sys.stderr, flush=True)
 
 
 def bumps_for(seed, n):
@@ -96,6 +103,8 @@ def serving_trace(A, seconds, rate, budget_gib, slots, seed, refine_tol, sample)
                 meta["x"] = x
             done[t] = (kind, info, meta)
             i += 1
+            if i % 16 == 0:
+                note(f"serving: {i} of {len(tickets)} submitted requests completed")
 
     th = threading.Thread(target=waiter)
     th.start()
@@ -285,7 +294,9 @@ def main():
     ap.add_argument("--batch-rhs", type=int, default=16, help="RHS count of the extra batched-throughput line (0: skip)")
     ap.add_argument("--batch-problems", type=int, default=32, help="problems in the uneven-batch line (0: skip)")
     ap.add_argument("--many-rhs", type=int, default=256, help="RHS count of the 7-level many-RHS line (0: skip)")
-    ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the serving trace (0: skip)")
+    ap.add_argument("--serve-seconds", type=float, default=0.0,
+                    help="length of the serving trace (0: skip; off by default: this trace has stalled "
+                         "without finishing, cause not yet found)")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="serving arrivals per second")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="serving device-memory budget")
     args = ap.parse_args()
@@ -302,6 +313,7 @@ def main():
     t0 = time.perf_counter()
     g.assemble()
     t_assemble = time.perf_counter() - t0
+    note(f"assembled N={g.N} in {t_assemble:.2f} s")
     b = g.rhs(args.theta)
     xn = None
 
@@ -323,6 +335,7 @@ def main():
                 "p99_ms": round(1e3 * float(np.percentile(lat64, 99)), 4), "refine_steps": int(np.median(steps64)),
                 "rel_err_vs_fp64": rel64, "direct_rel_err": rel_direct, "reference_relres": float(ref[2])}
 
+    note("FP64 reference solution and complex128 method done")
     # mixed precision: complex64 factor + FP64 refinement
     g.factor(A.PREC_C64)
     st = g.stats()
@@ -333,6 +346,7 @@ def main():
     total = time.perf_counter() - t0
     x = g.get_solution()[:, 0]
     rel_err = float(np.linalg.norm(x - x_ref) / xn)
+    note(f"headline: {args.steps} solves in {total:.3f} s, rel.err vs FP64 {rel_err:.2e}")
 
     # many right-hand sides in one refined solve (plane waves at equally
     # spaced angles): throughput per RHS; the headline stays the single solve
@@ -356,6 +370,7 @@ def main():
     total_prof = time.perf_counter() - t0
     A.profile_enable(False)
     prof = A.profile_read(reset=True)
+    note("batched-RHS line and per-kernel profiled pass done")
 
     # dominant kernel of the profiled pass (by summed event time)
     dom = max(prof, key=lambda k: k["ms"]) if prof else None
@@ -373,12 +388,17 @@ def main():
 
     g.close()
     batch32 = uneven_batch(A, args.seed, 1e-8, args.batch_problems) if args.batch_problems > 0 else None
+    if batch32:
+        note("uneven batch done")
     many = many_rhs(A, args.seed, args.many_rhs, 1e-8) if args.many_rhs > 0 else None
+    if many:
+        note("many-RHS line done")
 
     serving = None
     if args.serve_seconds > 0:
         serving = serving_trace(A, args.serve_seconds, args.serve_rate, args.serve_budget_gib, 4, args.seed,
                                 1e-8, 3)
+        note("serving trace done")
 
     cpu = None
     if args.cpu_levels > 0:

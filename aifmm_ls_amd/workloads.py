@@ -1,0 +1,57 @@
+"""Named workloads for profiling runs: ``python -m aifmm_ls_amd.workloads NAME``.
+
+Each one builds the 6-level problem of the mixed-precision solve benchmark
+(N = 147,456 unknowns, kappa = 40, compression tolerance 1e-7, 8 Gaussian
+bumps drawn from seed 2024) and runs one part of the solve path, so that a
+profiler sees that part on its own:
+
+  sweep   a complex64 factorisation, then three direct sweeps
+          (``solve_resident(tol, 0)``: no refinement)
+  factor  one assembly, then two complex64 factorisations
+
+Run under ``rocprofv3`` with ``AIFMM_GRAPH=0`` (launches issued one by one):
+the profiler crashes in the host runtime during HIP stream capture on this
+ROCm image."""
+import argparse
+import sys
+
+import numpy as np
+
+
+def gaussian_bumps(seed, n):
+    """n Gaussian contrast bumps (amplitude, width, centre x, centre y) drawn
+    from `seed`: the contrast of the benchmark problems."""
+    rng = np.random.default_rng(seed)
+    return np.c_[rng.uniform(0.2, 1.5, n), rng.uniform(0.05, 0.2, n), rng.uniform(-0.7, 0.7, (n, 2))]
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("name", choices=("sweep", "factor"))
+    ap.add_argument("--levels", type=int, default=6)
+    ap.add_argument("--kappa", type=float, default=40.0)
+    ap.add_argument("--tol-pow", type=int, default=7)
+    ap.add_argument("--bumps", type=int, default=8)
+    ap.add_argument("--seed", type=int, default=2024)
+    args = ap.parse_args(argv)
+
+    import aifmm_ls_amd as A
+
+    g = A.LSProblem(args.levels, args.kappa, args.tol_pow, qchoice=6, bumps=gaussian_bumps(args.seed, args.bumps))
+    g.assemble()
+    if args.name == "factor":
+        for i in range(2):
+            g.factor(A.PREC_C64)
+            print(f"factor {i}: {g.stats()['t_factor']:.3f} s", flush=True)
+    else:
+        g.factor(A.PREC_C64)
+        g.set_rhs(g.rhs(0.3))
+        for i in range(3):
+            st = g.solve_resident(10.0 ** -args.tol_pow, 0)
+            print(f"sweep {i}: {1e3 * st[0]:.3f} ms", flush=True)
+    g.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

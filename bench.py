@@ -40,8 +40,8 @@ def note(msg):
 
 
 def bumps_for(seed, n):
-    rng = np.random.default_rng(seed)
-    return np.c_[rng.uniform(0.2, 1.5, n), rng.uniform(0.05, 0.2, n), rng.uniform(-0.7, 0.7, (n, 2))]
+    from aifmm_ls_amd.workloads import gaussian_bumps
+    return gaussian_bumps(seed, n)
 
 
 def timed_solves(g, k, tol, max_refine):

@@ -66,6 +66,8 @@ def lib():
         L.aifmm_ls_profile_enable.argtypes = [ctypes.c_int]
         L.aifmm_ls_set_refinement.argtypes = [P, ctypes.c_int]
         L.aifmm_ls_profile_read.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_char_p, ctypes.c_int]
+        L.aifmm_ls_profile_launches.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_char_p, ctypes.c_int]
+        L.aifmm_ls_set_tuning.argtypes = [ctypes.c_int]
         L.aifmm_ls_device_bytes.argtypes = [P, ip64]
         L.aifmm_ls_solve_batch.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                            ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.c_double, ctypes.c_int,
@@ -91,9 +93,29 @@ def _dp(a):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
 
 
-def profile_enable(on=True):
-    """Record HIP event pairs around every batched launch (see profile_read)."""
-    _chk(lib().aifmm_ls_profile_enable(1 if on else 0))
+def profile_enable(on=True, launches=False):
+    """Record HIP event pairs around every batched launch (see profile_read);
+    launches=True also keeps one record per launch (see profile_launches)."""
+    _chk(lib().aifmm_ls_profile_enable((2 if launches else 1) if on else 0))
+
+
+def set_tuning(mode=1):
+    """Launch tuning for programs not yet tuned: 0/False off, 1/True timed,
+    or 256 / 512 / 1024 threads per workgroup wherever they fit (see
+    aifmm_ls_set_tuning)."""
+    _chk(lib().aifmm_ls_set_tuning(int(mode)))
+
+
+def profile_launches(reset=True, max_launches=100000):
+    """Per-launch records since the last reset, in launch order: list of
+    (kernel name, algorithmic bytes, milliseconds)."""
+    out = np.zeros(2 * max_launches)
+    names = ctypes.create_string_buffer(48 * max_launches)
+    n = lib().aifmm_ls_profile_launches(1 if reset else 0, max_launches, _dp(out), names, len(names))
+    if n < 0:
+        raise AifmmError(lib().aifmm_ls_last_error().decode())
+    nm = names.value.decode().split("\n")
+    return [(nm[i], out[2 * i], out[2 * i + 1]) for i in range(n)]
 
 
 def profile_read(reset=True, max_kernels=64):
@@ -110,7 +132,8 @@ def profile_read(reset=True, max_kernels=64):
 
 
 STAT_NAMES = ["t_tables", "t_assemble", "t_factor", "factor_flops", "factor_bytes", "base_n", "max_rank",
-              "solve_bytes_1rhs", "recompressed_boxes", "operator_bytes", "t_solve", "t_nca", "ill_boxes"]
+              "solve_bytes_1rhs", "recompressed_boxes", "operator_bytes", "t_solve", "t_nca", "ill_boxes",
+              "sweep_tuned_time_ratio", "sweep_tuned_stages"]
 
 
 class LSProblem:

@@ -390,7 +390,7 @@ int aifmm_ls_device_bytes(aifmm_ls_problem* h, int64_t* bytes) {
 
 int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
   GUARD({
-    double v[13] = {0};
+    double v[15] = {0};
     Problem& P = *h->pb;
     v[0] = P.t_tables;
     v[1] = P.t_assemble;
@@ -409,7 +409,15 @@ int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
     v[9] = (double)P.operator_bytes();
     v[10] = h->t_solve;
     v[11] = P.t_nca;
-    for (int i = 0; i < n && i < 13; ++i) out[i] = v[i];
+    // the latest launch tuning of the active factorisation's sweep
+    auto tuned = [&](const auto* tr) {
+      if (!tr || tr->default_ms <= 0) return;
+      v[13] = (tr->adopted ? tr->tuned_ms : tr->default_ms) / tr->default_ms;
+      v[14] = tr->adopted ? tr->changed : 0;
+    };
+    if (h->precision == 0 && h->f64) tuned(h->f64->sweep_tune());
+    if (h->precision == 1 && h->f32) tuned(h->f32->sweep_tune());
+    for (int i = 0; i < n && i < 15; ++i) out[i] = v[i];
   })
 }
 
@@ -575,7 +583,39 @@ extern "C" int aifmm_ls_selftest(int op, int m, int n, int k, const double* A, c
 
 
 extern "C" int aifmm_ls_profile_enable(int on) {
-  GUARD({ prof_enable(on != 0); })
+  GUARD({
+    if (on < 0 || on > 2) throw std::runtime_error("profile mode must be 0, 1 or 2");
+    prof_enable(on != 0, on == 2);
+  })
+}
+
+extern "C" int aifmm_ls_set_tuning(int mode) {
+  GUARD({ prog_set_tuning(mode); })
+}
+
+extern "C" int aifmm_ls_profile_launches(int reset, int max, double* out, char* names, int names_len) {
+  try {
+    if (max < 0 || (max > 0 && !out)) throw std::runtime_error("null argument");
+    std::vector<LaunchRecord> ls = prof_launches(reset != 0);
+    std::string nm;
+    int n = 0;
+    for (auto& l : ls) {
+      if (n >= max) break;
+      out[2 * n] = l.bytes;
+      out[2 * n + 1] = l.ms;
+      nm += std::string(l.name) + "\n";
+      ++n;
+    }
+    if (names && names_len > 0) {
+      size_t c = std::min(nm.size(), (size_t)names_len - 1);
+      std::memcpy(names, nm.data(), c);
+      names[c] = 0;
+    }
+    return n;
+  } catch (const std::exception& e) {
+    g_err = e.what();
+    return -1;
+  }
 }
 
 extern "C" int aifmm_ls_profile_read(int reset, int max, double* out, char* names, int names_len) {

@@ -996,7 +996,7 @@ template <class T>
 void Factor<T>::solve(const double2* b, double2* x, int nrhs) {
   Program<T>& P = sweep_program(nrhs);
   sweep_load(b, nrhs, pb.stream);
-  P.run(nrhs, pb.stream);
+  P.run(nrhs, pb.stream, [&] { sweep_load(b, nrhs, pb.stream); });
   sweep_store(x, nrhs, pb.stream);
 }
 

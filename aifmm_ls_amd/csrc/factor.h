@@ -113,6 +113,8 @@ class Factor {
   // sweep input, the sweep program (built for at least nrhs columns), store
   // the sweep output into x.
   Program<T>& sweep_program(int nrhs);
+  // the sweep's most recent launch tuning (null before its first solve)
+  const typename Program<T>::TuneResult* sweep_tune() const { return plan ? plan->last_tune() : nullptr; }
   void sweep_load(const double2* b, int nrhs, hipStream_t s);
   void sweep_store(double2* x, int nrhs, hipStream_t s);
 

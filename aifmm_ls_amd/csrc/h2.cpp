@@ -124,7 +124,7 @@ void H2Op::store(double2* y, int nrhs, hipStream_t s) {
 void H2Op::apply(const double2* x, double2* y, int nrhs) {
   Program<double2>& P = program(nrhs);
   load(x, nrhs, pb.stream);
-  P.run(nrhs, pb.stream);
+  P.run(nrhs, pb.stream, [&] { load(x, nrhs, pb.stream); });
   store(y, nrhs, pb.stream);
 }
 

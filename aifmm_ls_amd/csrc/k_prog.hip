@@ -302,21 +302,32 @@ static void launch_nth(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, 
   HIP_OK(hipGetLastError());
 }
 
-// 1024-thread workgroups: with one right-hand side each term of a program
-// gets its own column slices, so a program's loads are all in flight at once.
-// Stages of more than 1024 programs already fill the chip with 256 threads.
-// (Measured on MI355X: choosing 256 threads for stages of small programs
-// shortens their kernels under a profiler but not the replayed sweep.)
+// Workgroup size of a stage: nth = 256, 512 or 1024 threads, or 0 for the
+// default, 1024 threads, or 256 for stages of more than 1024 programs (those
+// already fill the chip).  The program's tuner picks other sizes per stage
+// where they measure faster.
+int prog_default_nth(int nprog) { return nprog > 1024 ? 256 : 1024; }
+
 template <class T>
-void launch_prog(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, int slot_words,
-                 hipStream_t s) {
-  if (!nprog || !nrhs) return;
-  if (ldy < 1) ldy = 1;
-  if (nprog > 1024) launch_nth<T, 256>(nprog, progs, terms, nrhs, ldy, slot_words, s);
-  else launch_nth<T, 1024>(nprog, progs, terms, nrhs, ldy, slot_words, s);
+bool prog_shape_fits(int nrhs, int ldy, int nth) {
+  return (nth == 256 || nth == 512 || nth == 1024) && pick_ch(nrhs, ldy < 1 ? 1 : ldy, sizeof(T), nth) > 0;
 }
 
-template void launch_prog<double2>(int, const PSlot<double2>*, const PTerm<double2>*, int, int, int, hipStream_t);
-template void launch_prog<float2>(int, const PSlot<float2>*, const PTerm<float2>*, int, int, int, hipStream_t);
+template <class T>
+void launch_prog(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, int slot_words,
+                 hipStream_t s, int nth) {
+  if (!nprog || !nrhs) return;
+  if (ldy < 1) ldy = 1;
+  if (!nth) nth = prog_default_nth(nprog);
+  if (nth == 256) launch_nth<T, 256>(nprog, progs, terms, nrhs, ldy, slot_words, s);
+  else if (nth == 512) launch_nth<T, 512>(nprog, progs, terms, nrhs, ldy, slot_words, s);
+  else if (nth == 1024) launch_nth<T, 1024>(nprog, progs, terms, nrhs, ldy, slot_words, s);
+  else throw std::runtime_error("box program workgroup size must be 256, 512 or 1024");
+}
+
+template void launch_prog<double2>(int, const PSlot<double2>*, const PTerm<double2>*, int, int, int, hipStream_t, int);
+template void launch_prog<float2>(int, const PSlot<float2>*, const PTerm<float2>*, int, int, int, hipStream_t, int);
+template bool prog_shape_fits<double2>(int, int, int);
+template bool prog_shape_fits<float2>(int, int, int);
 
 }  // namespace aifmm

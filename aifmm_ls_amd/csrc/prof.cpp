@@ -15,7 +15,8 @@ struct Pending {
   hipEvent_t e0, e1;
 };
 std::mutex mu;
-bool enabled = false;
+bool enabled = false, log_on = false;
+std::vector<LaunchRecord> launch_log;
 std::vector<Pending> pending;
 std::vector<hipEvent_t> pool;
 std::map<std::string, KernelProfile> totals;
@@ -42,6 +43,7 @@ void fold() {
     k.ms += ms;
     k.bytes += p.bytes;
     k.flops += p.flops;
+    if (log_on) launch_log.push_back(LaunchRecord{p.name, p.bytes, (double)ms});
     pool.push_back(p.e0);
     pool.push_back(p.e1);
   }
@@ -49,9 +51,10 @@ void fold() {
 }
 }  // namespace
 
-void prof_enable(bool on) {
+void prof_enable(bool on, bool log_launches) {
   std::lock_guard<std::mutex> g(mu);
   enabled = on;
+  log_on = on && log_launches;
 }
 
 bool prof_on() { return enabled; }
@@ -62,6 +65,14 @@ std::vector<KernelProfile> prof_collect(bool reset) {
   std::vector<KernelProfile> out;
   for (auto& t : totals) out.push_back(t.second);
   if (reset) totals.clear();
+  return out;
+}
+
+std::vector<LaunchRecord> prof_launches(bool reset) {
+  std::lock_guard<std::mutex> g(mu);
+  fold();
+  std::vector<LaunchRecord> out = launch_log;
+  if (reset) launch_log.clear();
   return out;
 }
 

@@ -18,10 +18,18 @@ struct KernelProfile {
   double flops = 0;   // algorithmic flops (complex FMA = 8)
 };
 
-void prof_enable(bool on);
+// log_launches: also keep one record per launch (name, bytes, duration)
+void prof_enable(bool on, bool log_launches = false);
 bool prof_on();
 // Synchronises on the recorded events and folds them into the totals.
 std::vector<KernelProfile> prof_collect(bool reset);
+
+struct LaunchRecord {
+  const char* name;
+  double bytes, ms;
+};
+// The per-launch records in launch order (after folding the pending events).
+std::vector<LaunchRecord> prof_launches(bool reset);
 
 class ProfScope {
  public:

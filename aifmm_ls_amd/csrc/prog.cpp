@@ -189,12 +189,126 @@ static double stage_flops(const std::vector<PProg<T>>& ps, const std::vector<PTe
 
 template <class T>
 void Program<T>::launch_all(int nrhs, hipStream_t s, bool prof) const {
-  for (auto& g : st_) {
+  auto sh = shapes_.find(nrhs);
+  const std::vector<int>* shape = sh == shapes_.end() ? nullptr : &sh->second;
+  for (size_t k = 0; k < st_.size(); ++k) {
+    const Stage& g = st_[k];
     if (g.progs.empty()) continue;
     ProfScope ps(s, std::is_same<T, float2>::value ? "k_prog<c64>" : "k_prog<c128>",
                  prof ? stage_bytes(g.progs, terms_, nrhs) : 0, prof ? stage_flops(g.progs, terms_, nrhs) : 0);
-    launch_prog<T>((int)g.progs.size(), g.dslots, dterms_, nrhs, g.maxm, g.slot_words, s);
+    launch_prog<T>((int)g.progs.size(), g.dslots, dterms_, nrhs, g.maxm, g.slot_words, s, shape ? (*shape)[k] : 0);
   }
+}
+
+// 0 off, 1 timed tuning (the default unless AIFMM_TUNE=0 is set), or a fixed
+// workgroup size (256, 512, 1024) for every stage it fits
+static std::atomic<int> tuning_mode{-1};
+static int tuning() {
+  int m = tuning_mode.load();
+  if (m < 0) {
+    const char* e = getenv("AIFMM_TUNE");
+    int expect = -1;
+    tuning_mode.compare_exchange_strong(expect, (e && e[0] == '0') ? 0 : 1);
+    m = tuning_mode.load();
+  }
+  return m;
+}
+void prog_set_tuning(int mode) {
+  if (mode != 0 && mode != 1 && mode != 256 && mode != 512 && mode != 1024)
+    throw std::runtime_error("tuning mode must be 0, 1, 256, 512 or 1024");
+  tuning_mode = mode;
+}
+
+template <class T>
+void Program<T>::tune(int nrhs, hipStream_t s, const std::function<void()>& reload) {
+  TuneResult& R = tune_[nrhs];
+  last_tune_ = &R;
+  const int ng = (int)st_.size();
+  const std::vector<int> def(ng, 0);
+  shapes_[nrhs] = def;
+  const int mode = tuning();
+  if (!mode || !ng) return;
+  if (mode > 1) {
+    // a fixed size wherever it fits (exercises every size deterministically)
+    std::vector<int> shape(ng, 0);
+    for (int k = 0; k < ng; ++k)
+      if (!st_[k].progs.empty() && prog_shape_fits<T>(nrhs, st_[k].maxm, mode)) {
+        shape[k] = mode;
+        R.changed += mode != prog_default_nth((int)st_[k].progs.size());
+      }
+    shapes_[nrhs] = shape;
+    R.adopted = true;
+    R.default_ms = R.tuned_ms = 1;
+    return;
+  }
+  std::vector<hipEvent_t> ev(ng + 1, nullptr);
+  for (auto& e : ev) HIP_OK(hipEventCreate(&e));
+  // one whole run on reloaded input; with `per`, the time of every stage
+  auto timed = [&](const std::vector<int>& shape, std::vector<float>* per) {
+    reload();
+    HIP_OK(hipEventRecord(ev[0], s));
+    for (int k = 0; k < ng; ++k) {
+      const Stage& g = st_[k];
+      if (!g.progs.empty())
+        launch_prog<T>((int)g.progs.size(), g.dslots, dterms_, nrhs, g.maxm, g.slot_words, s, shape[k]);
+      if (per || k == ng - 1) HIP_OK(hipEventRecord(ev[k + 1], s));
+    }
+    HIP_OK(hipEventSynchronize(ev[ng]));
+    float tot = 0;
+    HIP_OK(hipEventElapsedTime(&tot, ev[0], ev[ng]));
+    if (per) {
+      per->assign(ng, 0.f);
+      for (int k = 0; k < ng; ++k) HIP_OK(hipEventElapsedTime(&(*per)[k], ev[k], ev[k + 1]));
+    }
+    return tot;
+  };
+  try {
+    // About a dozen whole runs: worth it for programs that are run many times
+    // (a refined solve runs its sweep and operator several times per call).
+    // A run longer than 20 ms (many right-hand sides of a large problem)
+    // keeps the default sizes, since tuning would cost more than it returns.
+    const float t0 = timed(def, nullptr);
+    const int reps = t0 > 20.f ? 0 : 2;
+    if (reps) {
+      std::vector<float> best(ng, 3.0e38f), per;
+      std::vector<int> pick(ng, 0);
+      for (int c : {0, 1024, 512, 256}) {
+        std::vector<int> shape(ng, 0);
+        for (int k = 0; k < ng; ++k)
+          if (c && prog_shape_fits<T>(nrhs, st_[k].maxm, c)) shape[k] = c;
+        std::vector<float> mn(ng, 3.0e38f);
+        for (int r = 0; r < reps; ++r) {
+          timed(shape, &per);
+          for (int k = 0; k < ng; ++k) mn[k] = std::min(mn[k], per[k]);
+        }
+        // the default (c = 0) is measured first; another size must beat it by 3 %
+        for (int k = 0; k < ng; ++k)
+          if (mn[k] < (c ? 0.97f : 1.f) * best[k]) {
+            best[k] = mn[k];
+            pick[k] = shape[k];
+          }
+      }
+      double td = 0, tt = 0;
+      for (int r = 0; r < 3; ++r) {
+        td += timed(def, nullptr);
+        tt += timed(pick, nullptr);
+      }
+      R.default_ms = td / 3;
+      R.tuned_ms = tt / 3;
+      for (int k = 0; k < ng; ++k)
+        R.changed += pick[k] && !st_[k].progs.empty() && pick[k] != prog_default_nth((int)st_[k].progs.size());
+      if (tt < 0.99 * td) {
+        shapes_[nrhs] = pick;
+        R.adopted = true;
+      }
+    } else {
+      R.default_ms = R.tuned_ms = t0;
+    }
+  } catch (...) {
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    throw;
+  }
+  for (auto& e : ev) HIP_OK(hipEventDestroy(e));
 }
 
 // AIFMM_GRAPH=0 issues the launches one by one instead of replaying a graph
@@ -211,7 +325,13 @@ static bool graphs_enabled() {
 }
 
 template <class T>
-void Program<T>::run(int nrhs, hipStream_t s) {
+void Program<T>::run(int nrhs, hipStream_t s, const std::function<void()>& reload) {
+  if (reload && !tune_.count(nrhs)) {
+    tune(nrhs, s, reload);
+    reload();
+    // a graph captured earlier at this nrhs holds the default sizes
+    if (gnrhs_ == nrhs) gnrhs_ = -1;
+  }
   if (prof_on() || !graphs_enabled()) {
     launch_all(nrhs, s, prof_on());
     return;

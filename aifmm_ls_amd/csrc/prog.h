@@ -17,6 +17,8 @@
 // flight for it.
 #pragma once
 #include <atomic>
+#include <functional>
+#include <map>
 #include <cstdint>
 #include <vector>
 
@@ -61,9 +63,18 @@ struct PSlot {
 
 // Launch one stage: nprog workgroups; ldy >= every program's m; the first
 // slot_words 16-byte words of each slot hold the descriptor and its terms.
+// nth: workgroup threads (256, 512 or 1024; 0 = prog_default_nth(nprog)).
 template <class T>
 void launch_prog(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, int slot_words,
-                 hipStream_t s);
+                 hipStream_t s, int nth = 0);
+int prog_default_nth(int nprog);
+// Launch tuning for programs not yet tuned at an nrhs: 0 off, 1 timed (the
+// default unless AIFMM_TUNE=0 is set in the environment), or 256 / 512 /
+// 1024 to give every stage that workgroup size wherever it fits.
+void prog_set_tuning(int mode);
+// whether a stage of phase-1 length ldy fits the LDS at nth threads
+template <class T>
+bool prog_shape_fits(int nrhs, int ldy, int nth);
 // largest phase-1 length a stage may use
 int prog_max_m(int elem_bytes);
 // while held (counted), programs launch directly instead of through graphs
@@ -125,7 +136,24 @@ class Program {
   // Launch every stage in order on s (asynchronous).  The launch sequence is
   // captured once per nrhs into a HIP graph and replayed; with kernel
   // profiling on, the launches are issued one by one instead.
-  void run(int nrhs, hipStream_t s);
+  //
+  // With `reload` (which re-issues the copy of the program's input on s), the
+  // first run at each nrhs first tunes the launch shapes: see tune().
+  void run(int nrhs, hipStream_t s, const std::function<void()>& reload = nullptr);
+  // Outcome of tuning at one nrhs: mean milliseconds of a whole run with the
+  // default and with the picked workgroup sizes, stages given another size,
+  // and whether the picked sizes were kept.
+  struct TuneResult {
+    double default_ms = 0, tuned_ms = 0;
+    int changed = 0;
+    bool adopted = false;
+  };
+  const TuneResult* tune_result(int nrhs) const {
+    auto it = tune_.find(nrhs);
+    return it == tune_.end() ? nullptr : &it->second;
+  }
+  // the most recent tuning (any nrhs), or null
+  const TuneResult* last_tune() const { return last_tune_; }
   Program() : id_(next_id()) {}
   // distinct for every program built in the process (cache keys)
   uint64_t id() const { return id_; }
@@ -155,7 +183,19 @@ class Program {
   PTerm<T>* dterms_ = nullptr;
   hipGraphExec_t gexec_ = nullptr;
   int gnrhs_ = -1;
+  // workgroup size per stage and nrhs (0 = the default size)
+  std::map<int, std::vector<int>> shapes_;
+  std::map<int, TuneResult> tune_;
+  const TuneResult* last_tune_ = nullptr;
   void launch_all(int nrhs, hipStream_t s, bool prof) const;
+  // Launch autotuner: the whole program is run (on freshly reloaded input)
+  // once per candidate workgroup size (256, 512, 1024 threads, and the
+  // default) with an event after every stage; each stage takes the size that
+  // measured fastest (by at least 3 % over the default).  The picked sizes
+  // are kept only if whole runs with them beat the default by 1 %.  Whole
+  // runs rather than stages in isolation: a stage repeated alone finds its
+  // data in cache and times unlike its place in the sequence.
+  void tune(int nrhs, hipStream_t s, const std::function<void()>& reload);
   // pending launches of the open stage: A = fused programs and tile phase 1,
   // B = tile phase 2
   Stage pa_, pb_;

@@ -94,8 +94,20 @@ int aifmm_ls_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y);
 /* statistics: out[0..] = t_tables, t_assemble, t_factor, factor flops, factor bytes,
  * base size, max rank, solve bytes (nrhs=1), recompressed boxes, operator bytes,
  * last solve time, NCA time, boxes whose sweeps keep the reference's order
- * (badly conditioned L2M_f) */
+ * (badly conditioned L2M_f), and the latest launch tuning of the active
+ * factorisation's sweep (see aifmm_ls_set_tuning): run time with the tuned
+ * workgroup sizes over the default's (1 when the default was kept; 0 before
+ * the first solve), stages whose size was changed. */
 int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n);
+/* Launch tuning (no reference counterpart): the first solve at each RHS count
+ * times the sweep's and the FP64 operator's stages at 256, 512 and 1024
+ * threads per workgroup, over whole runs of about 20 ms or less, and keeps
+ * per-stage sizes that beat the default by 1 % over whole runs.  Results
+ * then differ from untuned runs by summation order only.  1 = on (the
+ * default, unless AIFMM_TUNE=0 is set), 0 = off, 256 / 512 / 1024 = that
+ * workgroup size for every stage it fits (for tests of each size); programs
+ * already tuned keep their sizes.  A new factorisation tunes afresh. */
+int aifmm_ls_set_tuning(int mode);
 /* Device memory the handle holds (arena chunks: operator, factors, sweep and
  * refinement vectors).  No reference counterpart: the reference runs one
  * problem per process; a serving loop admits problems by this figure. */
@@ -124,6 +136,11 @@ int aifmm_ls_selftest(int op, int m, int n, int k, const double* A, const double
  * of kernels, or a negative code.  reset != 0 clears the totals afterwards. */
 int aifmm_ls_profile_enable(int on);
 int aifmm_ls_profile_read(int reset, int max, double* out, char* names, int names_len);
+/* With profile_enable(2), each launch is also kept as a record: profile_launches
+ * fills, for up to max launches in launch order, out[2*i ..] = algorithmic
+ * bytes, milliseconds, and writes the kernel names '\n'-separated into names;
+ * returns the number of records, or a negative code.  reset != 0 clears them. */
+int aifmm_ls_profile_launches(int reset, int max, double* out, char* names, int names_len);
 
 /* ---- Serving loop.  No reference counterpart: the reference factorises and
  * solves one problem per process (testFMM2D_find_error.cpp:41-176).  A server

@@ -35,3 +35,12 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(A, "_lib", None)
     with pytest.raises(A.AifmmError):
         A.lib()
+
+
+def test_set_tuning_rejects_unknown_modes():
+    import aifmm_ls_amd as A
+    if not os.path.exists(A.LIB_PATH):
+        pytest.skip("library not built")
+    with pytest.raises(A.AifmmError):
+        A.set_tuning(7)
+    A.set_tuning(1)  # the default

@@ -64,7 +64,10 @@ int aifmm_ls_skeleton(aifmm_ls_problem* h, int level, int box, int which, int64_
 int aifmm_ls_factor(aifmm_ls_problem* h, int precision);
 /* forward sweep + base solve + back_substitution_phase (FMM2DTree_find_error.hpp:2487-2532)
  * for nrhs right-hand sides (N x nrhs).  stats (>= 8 doubles, optional): solve seconds,
- * refinement steps, final relative residual, ... */
+ * refinement steps, final relative residual max_c |b_c - A_H x_c| / |b_c|, ...
+ * (with flexible GMRES converged and tol >= 1e-11, the recurrence's value of
+ * that residual, which agrees with the FP64 product to rounding; otherwise
+ * measured). */
 int aifmm_ls_solve(aifmm_ls_problem* h, int nrhs, const double* b, double* x, double tol, int max_refine,
                    double* stats);
 /* device-resident variant for timing: stage RHS, solve in HBM, fetch solution */

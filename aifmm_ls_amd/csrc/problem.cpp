@@ -419,6 +419,8 @@ void Problem::nca_level(int j) {
   int p0 = 0;
   while (p0 < 2 * nb) {
     std::vector<AcaProblem> probs;
+    std::vector<size_t> roff;  // problem q: rank at roff[q], then rm primary and rm secondary pivots
+    size_t rtot = 0;
     size_t bytes = 0;
     int p1 = p0;
     for (; p1 < 2 * nb; ++p1) {
@@ -445,28 +447,36 @@ void Problem::nca_level(int j) {
       a.cross = acaws.get<double2>(np ? np : 1);
       a.remP = acaws.get<char>(np ? np : 1);
       a.remS = acaws.get<char>(ns ? ns : 1);
-      a.pb = acaws.get<int>(rm);
-      a.sb = acaws.get<int>(rm);
-      a.rank_out = acaws.get<int>(1);
+      // rank and pivot lists go to one array for the whole batch (below)
+      roff.push_back(rtot);
+      rtot += 1 + 2 * rm;
       probs.push_back(a);
       res[p1].np = np;
       res[p1].ns = ns;
       res[p1].prim_rows = a.prim_rows;
     }
+    int* dres = acaws.get<int>(rtot);
+    for (size_t q = 0; q < probs.size(); ++q) {
+      const size_t rm = probs[q].np > 0 ? probs[q].np : 1;
+      probs[q].rank_out = dres + roff[q];
+      probs[q].pb = dres + roff[q] + 1;
+      probs[q].sb = dres + roff[q] + 1 + rm;
+    }
     AcaProblem* dp = upload(scratch, probs, stream);
     launch_aca(ctx, (int)probs.size(), dp, tol, tol_robust, stream);
+    // every problem's rank and pivots in one copy
+    std::vector<int> hres(rtot);
+    HIP_OK(hipMemcpyAsync(hres.data(), dres, rtot * sizeof(int), hipMemcpyDeviceToHost, stream));
     HIP_OK(hipStreamSynchronize(stream));
     CopyBatch<double2> keepcp;
     for (int i = p0; i < p1; ++i) {
       const AcaProblem& a = probs[i - p0];
-      HIP_OK(hipMemcpy(&res[i].rank, a.rank_out, 4, hipMemcpyDeviceToHost));
-      int rk = res[i].rank;
-      res[i].pb.resize(rk);
-      res[i].sb.resize(rk);
-      if (rk) {
-        HIP_OK(hipMemcpy(res[i].pb.data(), a.pb, 4 * rk, hipMemcpyDeviceToHost));
-        HIP_OK(hipMemcpy(res[i].sb.data(), a.sb, 4 * rk, hipMemcpyDeviceToHost));
-      }
+      const size_t q = (size_t)(i - p0), rm = a.np > 0 ? a.np : 1;
+      const int rk = hres[roff[q]];
+      if (rk < 0 || (size_t)rk > rm) throw std::runtime_error("ACA returned an impossible rank");
+      res[i].rank = rk;
+      res[i].pb.assign(hres.begin() + roff[q] + 1, hres.begin() + roff[q] + 1 + rk);
+      res[i].sb.assign(hres.begin() + roff[q] + 1 + rm, hres.begin() + roff[q] + 1 + rm + rk);
       // keep only what the transfer operators need: Ac (incoming, N1 x rk)
       // or Ar^T (outgoing, N2 x rk), stored column-major.
       int dir = i % 2;

@@ -288,13 +288,30 @@ static int pick_ch(int nrhs, int ldy, int eb, int nth) {
 
 int prog_max_m(int elem_bytes) { return PROG_LDS / elem_bytes - 1024; }
 
+// columns per pass asked for (ch_req), checked against the LDS: 1, 2, 4, or
+// 8 at up to 512 threads (8 columns of accumulators need the registers
+// that 1024-thread workgroups do not leave); 0 if it does not fit
+static int fit_ch(int nrhs, int ldy, int eb, int nth, int ch_req) {
+  if (!ch_req) return pick_ch(nrhs, ldy, eb, nth);
+  const bool known = ch_req == 1 || ch_req == 2 || ch_req == 4 || (ch_req == 8 && nth <= 512);
+  if (!known || (nrhs == 1 && ch_req != 1)) return 0;
+  return (size_t)(ldy + nth) * ch_req * eb <= (size_t)PROG_LDS ? ch_req : 0;
+}
+
 template <class T, int NTH>
 static void launch_nth(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, int sw,
-                       hipStream_t s) {
+                       hipStream_t s, int ch_req) {
   if (sw < 1 || sw > (int)(sizeof(PSlot<T>) / 16)) throw std::runtime_error("bad program slot size");
-  const int ch = pick_ch(nrhs, ldy, sizeof(T), NTH);
+  const int ch = fit_ch(nrhs, ldy, sizeof(T), NTH, ch_req);
   if (!ch) throw std::runtime_error("box program vector of " + std::to_string(ldy) + " rows exceeds LDS");
   const size_t lds = (size_t)(ldy + NTH) * ch * sizeof(T);
+  if constexpr (NTH <= 512) {
+    if (ch == 8) {
+      hipLaunchKernelGGL((k_prog<T, 8, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
+      HIP_OK(hipGetLastError());
+      return;
+    }
+  }
   if (ch == 4) hipLaunchKernelGGL((k_prog<T, 4, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
   else if (ch == 2)
     hipLaunchKernelGGL((k_prog<T, 2, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
@@ -302,26 +319,30 @@ static void launch_nth(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, 
   HIP_OK(hipGetLastError());
 }
 
-// Workgroup size of a stage: nth = 256, 512 or 1024 threads, or 0 for the
-// default, 1024 threads, or 256 for stages of more than 1024 programs (those
-// already fill the chip).  The program's tuner picks other sizes per stage
-// where they measure faster.
+// Launch shape of a stage: workgroup threads nth = 256, 512 or 1024 plus
+// PROG_SHAPE_CH x right-hand-side columns per pass (0 columns: the most
+// that fit the LDS, up to 4), or 0 for the default, 1024 threads, or 256 for
+// stages of more than 1024 programs (those already fill the chip).  The
+// program's tuner picks other shapes per stage where they measure faster.
 int prog_default_nth(int nprog) { return nprog > 1024 ? 256 : 1024; }
 
 template <class T>
-bool prog_shape_fits(int nrhs, int ldy, int nth) {
-  return (nth == 256 || nth == 512 || nth == 1024) && pick_ch(nrhs, ldy < 1 ? 1 : ldy, sizeof(T), nth) > 0;
+bool prog_shape_fits(int nrhs, int ldy, int shape) {
+  const int nth = shape % PROG_SHAPE_CH, ch = shape / PROG_SHAPE_CH;
+  return (nth == 256 || nth == 512 || nth == 1024) && fit_ch(nrhs, ldy < 1 ? 1 : ldy, sizeof(T), nth, ch) > 0;
 }
 
 template <class T>
 void launch_prog(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, int slot_words,
-                 hipStream_t s, int nth) {
+                 hipStream_t s, int shape) {
   if (!nprog || !nrhs) return;
   if (ldy < 1) ldy = 1;
+  int nth = shape % PROG_SHAPE_CH;
+  const int ch = shape / PROG_SHAPE_CH;
   if (!nth) nth = prog_default_nth(nprog);
-  if (nth == 256) launch_nth<T, 256>(nprog, progs, terms, nrhs, ldy, slot_words, s);
-  else if (nth == 512) launch_nth<T, 512>(nprog, progs, terms, nrhs, ldy, slot_words, s);
-  else if (nth == 1024) launch_nth<T, 1024>(nprog, progs, terms, nrhs, ldy, slot_words, s);
+  if (nth == 256) launch_nth<T, 256>(nprog, progs, terms, nrhs, ldy, slot_words, s, ch);
+  else if (nth == 512) launch_nth<T, 512>(nprog, progs, terms, nrhs, ldy, slot_words, s, ch);
+  else if (nth == 1024) launch_nth<T, 1024>(nprog, progs, terms, nrhs, ldy, slot_words, s, ch);
   else throw std::runtime_error("box program workgroup size must be 256, 512 or 1024");
 }
 

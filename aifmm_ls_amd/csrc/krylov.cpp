@@ -204,7 +204,10 @@ RefineResult Refiner::run(const Precond& M, const double2* b, double2* x, int nr
         worst = std::max(worst, bn[c] > 0 ? est / bn[c] : est);
       }
       k = j + 1;
-      if (worst <= 0.5 * tol) {
+      // Stop bound: with the recurrence trusted (tol >= 1e-11, below) it
+      // only needs a margin for FP64 rounding; a tighter tol is confirmed by
+      // a true residual, and half of it leaves room for the rounding floor.
+      if (worst <= (tol >= 1e-11 ? 0.9 : 0.5) * tol) {
         est_end = worst;
         break;
       }
@@ -224,9 +227,9 @@ RefineResult Refiner::run(const Precond& M, const double2* b, double2* x, int nr
     lincomb(x, Z, (int64_t)vb, k, y, std::vector<double2>(nr, one), nr);
     // Converged by the recurrence: its residual is |b - A x| of the FP64
     // operator up to FP64 rounding (w = A z is formed explicitly and V is
-    // kept orthonormal by two Gram-Schmidt passes), so the half-tolerance
-    // margin needs no further product to confirm while tol is far above the
-    // rounding floor (about 1e-15; the recurrence keeps shrinking below it).
+    // kept orthonormal by two Gram-Schmidt passes), so the 10 % margin needs
+    // no further product to confirm while tol is far above the rounding
+    // floor (about 1e-15; the recurrence keeps shrinking below it).
     // A cycle that ended on its restart length or the step limit, or a
     // tighter tol, measures the true residual above.
     if (est_end >= 0 && tol >= 1e-11) {

@@ -201,7 +201,7 @@ void Program<T>::launch_all(int nrhs, hipStream_t s, bool prof) const {
 }
 
 // 0 off, 1 timed tuning (the default unless AIFMM_TUNE=0 is set), or a fixed
-// workgroup size (256, 512, 1024) for every stage it fits
+// launch shape (prog.h) for every stage it fits
 static std::atomic<int> tuning_mode{-1};
 static int tuning() {
   int m = tuning_mode.load();
@@ -214,8 +214,12 @@ static int tuning() {
   return m;
 }
 void prog_set_tuning(int mode) {
-  if (mode != 0 && mode != 1 && mode != 256 && mode != 512 && mode != 1024)
-    throw std::runtime_error("tuning mode must be 0, 1, 256, 512 or 1024");
+  const int nth = mode % PROG_SHAPE_CH, ch = mode / PROG_SHAPE_CH;
+  const bool shape = (nth == 256 || nth == 512 || nth == 1024) &&
+                     (ch == 0 || ch == 1 || ch == 2 || ch == 4 || (ch == 8 && nth <= 512));
+  if (mode != 0 && mode != 1 && !shape)
+    throw std::runtime_error("tuning mode must be 0, 1 or a launch shape (256, 512 or 1024 threads + 4096 x "
+                             "columns per pass, 8 columns at up to 512 threads)");
   tuning_mode = mode;
 }
 
@@ -229,12 +233,12 @@ void Program<T>::tune(int nrhs, hipStream_t s, const std::function<void()>& relo
   const int mode = tuning();
   if (!mode || !ng) return;
   if (mode > 1) {
-    // a fixed size wherever it fits (exercises every size deterministically)
+    // a fixed shape wherever it fits (exercises every shape deterministically)
     std::vector<int> shape(ng, 0);
     for (int k = 0; k < ng; ++k)
       if (!st_[k].progs.empty() && prog_shape_fits<T>(nrhs, st_[k].maxm, mode)) {
         shape[k] = mode;
-        R.changed += mode != prog_default_nth((int)st_[k].progs.size());
+        R.changed += mode % PROG_SHAPE_CH != prog_default_nth((int)st_[k].progs.size()) || mode / PROG_SHAPE_CH;
       }
     shapes_[nrhs] = shape;
     R.adopted = true;
@@ -272,7 +276,15 @@ void Program<T>::tune(int nrhs, hipStream_t s, const std::function<void()>& relo
     if (reps) {
       std::vector<float> best(ng, 3.0e38f), per;
       std::vector<int> pick(ng, 0);
-      for (int c : {0, 1024, 512, 256}) {
+      // candidates: every workgroup size, and with several right-hand sides
+      // every count of columns per pass, after the default (0)
+      std::vector<int> cands{0};
+      for (int nth : {1024, 512, 256})
+        for (int ch : {0, 8, 4, 2, 1}) {
+          if (ch && nrhs == 1) continue;
+          cands.push_back(nth + PROG_SHAPE_CH * ch);
+        }
+      for (int c : cands) {
         std::vector<int> shape(ng, 0);
         for (int k = 0; k < ng; ++k)
           if (c && prog_shape_fits<T>(nrhs, st_[k].maxm, c)) shape[k] = c;
@@ -281,7 +293,7 @@ void Program<T>::tune(int nrhs, hipStream_t s, const std::function<void()>& relo
           timed(shape, &per);
           for (int k = 0; k < ng; ++k) mn[k] = std::min(mn[k], per[k]);
         }
-        // the default (c = 0) is measured first; another size must beat it by 3 %
+        // the default (c = 0) is measured first; another shape must beat it by 3 %
         for (int k = 0; k < ng; ++k)
           if (mn[k] < (c ? 0.97f : 1.f) * best[k]) {
             best[k] = mn[k];
@@ -296,7 +308,9 @@ void Program<T>::tune(int nrhs, hipStream_t s, const std::function<void()>& relo
       R.default_ms = td / 3;
       R.tuned_ms = tt / 3;
       for (int k = 0; k < ng; ++k)
-        R.changed += pick[k] && !st_[k].progs.empty() && pick[k] != prog_default_nth((int)st_[k].progs.size());
+        R.changed += pick[k] && !st_[k].progs.empty() &&
+                     (pick[k] % PROG_SHAPE_CH != prog_default_nth((int)st_[k].progs.size()) ||
+                      pick[k] / PROG_SHAPE_CH != 0);
       if (tt < 0.99 * td) {
         shapes_[nrhs] = pick;
         R.adopted = true;

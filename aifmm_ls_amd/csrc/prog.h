@@ -63,18 +63,21 @@ struct PSlot {
 
 // Launch one stage: nprog workgroups; ldy >= every program's m; the first
 // slot_words 16-byte words of each slot hold the descriptor and its terms.
-// nth: workgroup threads (256, 512 or 1024; 0 = prog_default_nth(nprog)).
+// shape: workgroup threads (256, 512 or 1024; 0 = prog_default_nth(nprog))
+// plus PROG_SHAPE_CH x right-hand-side columns per pass (1, 2, 4, or 8 at up
+// to 512 threads; 0 = the most of 1, 2, 4 that fit the LDS).
+constexpr int PROG_SHAPE_CH = 4096;
 template <class T>
 void launch_prog(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, int slot_words,
-                 hipStream_t s, int nth = 0);
+                 hipStream_t s, int shape = 0);
 int prog_default_nth(int nprog);
 // Launch tuning for programs not yet tuned at an nrhs: 0 off, 1 timed (the
-// default unless AIFMM_TUNE=0 is set in the environment), or 256 / 512 /
-// 1024 to give every stage that workgroup size wherever it fits.
+// default unless AIFMM_TUNE=0 is set in the environment), or a launch shape
+// (as for launch_prog) to give every stage that shape wherever it fits.
 void prog_set_tuning(int mode);
-// whether a stage of phase-1 length ldy fits the LDS at nth threads
+// whether a stage of phase-1 length ldy fits the LDS in this shape
 template <class T>
-bool prog_shape_fits(int nrhs, int ldy, int nth);
+bool prog_shape_fits(int nrhs, int ldy, int shape);
 // largest phase-1 length a stage may use
 int prog_max_m(int elem_bytes);
 // while held (counted), programs launch directly instead of through graphs
@@ -189,9 +192,10 @@ class Program {
   const TuneResult* last_tune_ = nullptr;
   void launch_all(int nrhs, hipStream_t s, bool prof) const;
   // Launch autotuner: the whole program is run (on freshly reloaded input)
-  // once per candidate workgroup size (256, 512, 1024 threads, and the
-  // default) with an event after every stage; each stage takes the size that
-  // measured fastest (by at least 3 % over the default).  The picked sizes
+  // once per candidate launch shape (256, 512, 1024 threads; with several
+  // right-hand sides also 1, 2, 4 or 8 columns per pass; and the default)
+  // with an event after every stage; each stage takes the shape that
+  // measured fastest (by at least 3 % over the default).  The picked shapes
   // are kept only if whole runs with them beat the default by 1 %.  Whole
   // runs rather than stages in isolation: a stage repeated alone finds its
   // data in cache and times unlike its place in the sequence.

@@ -104,12 +104,15 @@ int aifmm_ls_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y);
 int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n);
 /* Launch tuning (no reference counterpart): the first solve at each RHS count
  * times the sweep's and the FP64 operator's stages at 256, 512 and 1024
- * threads per workgroup, over whole runs of about 20 ms or less, and keeps
- * per-stage sizes that beat the default by 1 % over whole runs.  Results
+ * threads per workgroup (with several right-hand sides also at 1, 2, 4 or 8
+ * columns per pass), over whole runs of about 20 ms or less, and keeps
+ * per-stage shapes that beat the default by 1 % over whole runs.  Results
  * then differ from untuned runs by summation order only.  1 = on (the
- * default, unless AIFMM_TUNE=0 is set), 0 = off, 256 / 512 / 1024 = that
- * workgroup size for every stage it fits (for tests of each size); programs
- * already tuned keep their sizes.  A new factorisation tunes afresh. */
+ * default, unless AIFMM_TUNE=0 is set), 0 = off, or a fixed shape for every
+ * stage it fits, for tests of each shape: threads (256, 512, 1024) + 4096 x
+ * columns per pass (0 = as many of 1, 2, 4 as fit; 8 at up to 512 threads).
+ * Programs already tuned keep their shapes.  A new factorisation tunes
+ * afresh. */
 int aifmm_ls_set_tuning(int mode);
 /* Device memory the handle holds (arena chunks: operator, factors, sweep and
  * refinement vectors).  No reference counterpart: the reference runs one

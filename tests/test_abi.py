@@ -41,6 +41,7 @@ def test_set_tuning_rejects_unknown_modes():
     import aifmm_ls_amd as A
     if not os.path.exists(A.LIB_PATH):
         pytest.skip("library not built")
-    with pytest.raises(A.AifmmError):
-        A.set_tuning(7)
+    for bad in (7, 1024 + 8 * 4096, 512 + 3 * 4096, 300):
+        with pytest.raises(A.AifmmError):
+            A.set_tuning(bad)
     A.set_tuning(1)  # the default

@@ -1,6 +1,7 @@
-"""Launch tuning: the stages of the sweep and of the FP64 operator run at the
-workgroup size (256, 512 or 1024 threads) the tuner picks, or at one fixed
-size for every stage (the test modes of aifmm_ls_set_tuning).
+"""Launch tuning: the stages of the sweep and of the FP64 operator run in the
+launch shape the tuner picks (256, 512 or 1024 threads per workgroup; 1, 2,
+4 or 8 right-hand-side columns per pass), or in one fixed shape for every
+stage (the test modes of aifmm_ls_set_tuning).
 
 Every size gives the same solution up to summation order.
 - Complex128 direct solves agree with the default sizes to FP64 rounding
@@ -66,7 +67,9 @@ def test_every_workgroup_size_gives_the_same_solution(prob, prec):
         print(f"  default sizes: complex64 direct vs complex128 direct: 1 RHS {e10:.1e}, 8 RHS {em0:.1e}", flush=True)
         A.set_tuning(0)
         g.factor(prec)
-    for mode in (256, 512, 1024, 1):
+    # workgroup sizes, then columns per pass (8 at 256 threads, 1 at 1024
+    # threads) on the 8-RHS solves, then the timed tuner
+    for mode in (256, 512, 1024, 256 + 8 * 4096, 1024 + 1 * 4096, 1):
         x1, xm, xr, st = solves(g, prec, mode, B)
         d1, dm, dr = rel(x1, x10), rel(xm, xm0), rel(xr, xr0)
         msg = (f"  mode {mode:4d}: direct vs default sizes 1 RHS {d1:.1e}, 8 RHS {dm:.1e}; refined {dr:.1e}; "
@@ -80,5 +83,5 @@ def test_every_workgroup_size_gives_the_same_solution(prob, prec):
             assert e1 <= 2 * e10 + 1e-6 and em <= 2 * em0 + 1e-6
         assert dr < 1e3 * TOL
         assert 0 < st["sweep_tuned_time_ratio"] <= 1.0
-        if mode in (256, 512):
+        if mode not in (1, 1024):
             assert st["sweep_tuned_stages"] >= 1

@@ -1,5 +1,6 @@
-// Box-program kernel (descriptors in prog.h): one workgroup (4 or 16 wave64)
-// per program.  Each output row block of up to 256 rows is spread over the
+// Box-program kernel (descriptors in prog.h): one workgroup (4, 8 or 16
+// wave64) per program and group of CH right-hand-side columns (blockIdx.y).
+// Each output row block of up to 256 rows is spread over the
 // workgroup as rp rows x ns column slices (rp = rows rounded up to 16, 32 or
 // a multiple of 64; ns = threads / rp), so a wave reads a contiguous column
 // segment of the matrix per step (coalesced), and slice partial sums are
@@ -205,7 +206,10 @@ template <class T, int CH, int NTH>
 __device__ __forceinline__ void run_program(const PProg<T>* __restrict__ P, const PTerm<T>* __restrict__ terms,
                                             int nrhs, int ldy, T* ylds, T* red) {
   const int m = P->m;
-  for (int c0 = 0; c0 < nrhs; c0 += CH) {
+  // column groups of CH right-hand sides are independent (each has its own
+  // phase-1 vector and writes its own columns): workgroup (program, g) runs
+  // groups g, g + gridDim.y, ...
+  for (int c0 = blockIdx.y * CH; c0 < nrhs; c0 += CH * gridDim.y) {
     const int nc = min(CH, nrhs - c0);
     for (int rb = 0; rb < m; rb += 256) {
       const int rr = min(256, m - rb);
@@ -305,17 +309,21 @@ static void launch_nth(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, 
   const int ch = fit_ch(nrhs, ldy, sizeof(T), NTH, ch_req);
   if (!ch) throw std::runtime_error("box program vector of " + std::to_string(ldy) + " rows exceeds LDS");
   const size_t lds = (size_t)(ldy + NTH) * ch * sizeof(T);
+  // one workgroup per program and group of ch right-hand-side columns
+  const int groups = (nrhs + ch - 1) / ch;
+  if (groups > 65535) throw std::runtime_error("too many right-hand-side column groups");
+  const dim3 grid(nprog, groups);
   if constexpr (NTH <= 512) {
     if (ch == 8) {
-      hipLaunchKernelGGL((k_prog<T, 8, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
+      hipLaunchKernelGGL((k_prog<T, 8, NTH>), grid, dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
       HIP_OK(hipGetLastError());
       return;
     }
   }
-  if (ch == 4) hipLaunchKernelGGL((k_prog<T, 4, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
+  if (ch == 4) hipLaunchKernelGGL((k_prog<T, 4, NTH>), grid, dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
   else if (ch == 2)
-    hipLaunchKernelGGL((k_prog<T, 2, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
-  else hipLaunchKernelGGL((k_prog<T, 1, NTH>), dim3(nprog), dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
+    hipLaunchKernelGGL((k_prog<T, 2, NTH>), grid, dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
+  else hipLaunchKernelGGL((k_prog<T, 1, NTH>), grid, dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
   HIP_OK(hipGetLastError());
 }
 

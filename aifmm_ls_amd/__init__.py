@@ -68,6 +68,7 @@ def lib():
         L.aifmm_ls_profile_read.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_char_p, ctypes.c_int]
         L.aifmm_ls_profile_launches.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_char_p, ctypes.c_int]
         L.aifmm_ls_set_tuning.argtypes = [ctypes.c_int]
+        L.aifmm_ls_selftest_prog.argtypes = [ctypes.c_int] * 7 + [dp] * 6
         L.aifmm_ls_device_bytes.argtypes = [P, ip64]
         L.aifmm_ls_solve_batch.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int,
                                            ctypes.POINTER(dp), ctypes.POINTER(dp), ctypes.c_double, ctypes.c_int,
@@ -322,6 +323,27 @@ def selftest_cpqr(A, thr, k, c64=False):
     dummy = np.zeros(2)
     _chk(lib().aifmm_ls_selftest(10 * c64 + 2, m, n, k, _dp(_cm(A)), _dp(dummy), _dp(C), ctypes.byref(r), thr))
     return r.value, C.view(np.complex128).reshape(k, m).T
+
+
+def _blocks(a):
+    """Stacked blocks (..., r, c) to consecutive column-major interleaved complex."""
+    return np.ascontiguousarray(np.swapaxes(np.asarray(a, np.complex128), -1, -2)).view(np.float64)
+
+
+def selftest_program(A, X, S, Bm, shape=0, c64=False):
+    """Box programs on the GPU (see aifmm_ls_selftest_prog): A (nprog, nterms,
+    m, cols), X (nprog, nterms, cols, nrhs), S (nprog, m, nrhs), Bm (nprog, m,
+    m); returns Y = S + sum_t alpha_t A X and Out = Bm Y, each (nprog, m, nrhs)."""
+    nprog, nterms, m, cols = A.shape
+    nrhs = X.shape[-1]
+    if X.shape != (nprog, nterms, cols, nrhs) or S.shape != (nprog, m, nrhs) or Bm.shape != (nprog, m, m):
+        raise AifmmError("self-test block shapes do not match")
+    Y = np.zeros(2 * nprog * m * nrhs)
+    O = np.zeros(2 * nprog * m * nrhs)
+    _chk(lib().aifmm_ls_selftest_prog(1 if c64 else 0, int(shape), nprog, m, nterms, cols, nrhs, _dp(_blocks(A)),
+                                      _dp(_blocks(X)), _dp(_blocks(S)), _dp(_blocks(Bm)), _dp(Y), _dp(O)))
+    back = lambda v: np.swapaxes(v.view(np.complex128).reshape(nprog, nrhs, m), -1, -2)
+    return back(Y), back(O)
 
 
 def _params(n_levels, kappa, tol_pow, qchoice, n_cheb, degree, L, bumps, quantize):

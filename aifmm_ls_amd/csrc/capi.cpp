@@ -572,6 +572,84 @@ extern "C" int aifmm_ls_aca(aifmm_ls_problem* h, int tol_pow, int n1, const int6
   })
 }
 
+// Box-program self-test: nprog programs of m rows in one stage, each
+//   y_p = S_p + sum_t alpha_t A_{p,t} X_{p,t}   (phase 1, saved to Y)
+//   out_p = B_p y_p                             (phase 2, y read from LDS)
+// with alpha_t = 1 for even t and -0.5 + 0.25 i for odd t, run in launch
+// shape `shape` (0 = default).  Host arrays column-major interleaved complex:
+// A nprog*nterms blocks m x cols, X nprog*nterms blocks cols x nrhs, S and
+// Y and Out nprog blocks m x nrhs, Bm nprog blocks m x m.
+template <class T>
+static void selftest_prog_t(int shape, int nprog, int m, int nterms, int cols, int nrhs, const double* A,
+                            const double* X, const double* S, const double* Bm, double* Y, double* Out) {
+  Arena a{size_t(64) << 20}, sc{size_t(16) << 20};
+  hipStream_t s;
+  HIP_OK(hipStreamCreate(&s));
+  try {
+    auto up = [&](const double* h, size_t n) {
+      double2* d = a.get<double2>(n);
+      HIP_OK(hipMemcpy(d, h, n * 16, hipMemcpyHostToDevice));
+      T* t = a.get<T>(n);
+      std::vector<CvtDesc<T>> cv{CvtDesc<T>{d, t, (int64_t)n}};
+      launch_cvt<T>(1, upload(sc, cv, s), s);
+      return t;
+    };
+    auto down = [&](const T* t, size_t n, double* h) {
+      double2* d = a.get<double2>(n);
+      if (std::is_same<T, double2>::value) HIP_OK(hipMemcpyAsync(d, t, n * 16, hipMemcpyDeviceToDevice, s));
+      else launch_f2d((const float2*)t, d, (int64_t)n, s);
+      HIP_OK(hipStreamSynchronize(s));
+      HIP_OK(hipMemcpy(h, d, n * 16, hipMemcpyDeviceToHost));
+    };
+    const size_t mc = (size_t)m * cols, cr = (size_t)cols * nrhs, mr = (size_t)m * nrhs;
+    T* dA = up(A, (size_t)nprog * nterms * mc);
+    T* dX = up(X, (size_t)nprog * nterms * cr);
+    T* dS = up(S, (size_t)nprog * mr);
+    T* dB = up(Bm, (size_t)nprog * m * m);
+    T* dY = a.get<T>((size_t)nprog * mr);
+    T* dO = a.get<T>((size_t)nprog * mr);
+    Program<T> P(nrhs);  // tiled as a program built for nrhs columns
+    P.stage();
+    for (int p = 0; p < nprog; ++p) {
+      BoxOp<T> op;
+      op.m = m;
+      op.src = dS + p * mr;
+      op.ldsrc = m;
+      op.save = dY + p * mr;
+      op.ldsave = m;
+      for (int t = 0; t < nterms; ++t)
+        op.term(dA + ((size_t)p * nterms + t) * mc, m, cols, dX + ((size_t)p * nterms + t) * cr, cols,
+                t % 2 ? make_double2(-0.5, 0.25) : make_double2(1, 0));
+      auto& o = op.out(dO + p * mr, m, m);
+      out_term<T>(o, dB + (size_t)p * m * m, m, m, nullptr, 0, make_double2(1, 0));
+      P.add(op);
+    }
+    P.finalize(a, s);
+    P.force_shape(nrhs, shape);
+    P.run(nrhs, s);
+    HIP_OK(hipStreamSynchronize(s));
+    down(dY, (size_t)nprog * mr, Y);
+    down(dO, (size_t)nprog * mr, Out);
+  } catch (...) {
+    (void)hipStreamSynchronize(s);
+    (void)hipStreamDestroy(s);
+    throw;
+  }
+  HIP_OK(hipStreamDestroy(s));
+}
+
+extern "C" int aifmm_ls_selftest_prog(int precision, int shape, int nprog, int m, int nterms, int cols, int nrhs,
+                                      const double* A, const double* X, const double* S, const double* Bm, double* Y,
+                                      double* Out) {
+  GUARD({
+    if (nprog < 1 || m < 1 || nterms < 1 || cols < 1 || nrhs < 1) throw std::runtime_error("empty self-test program");
+    if (!A || !X || !S || !Bm || !Y || !Out) throw std::runtime_error("null argument");
+    if (precision == 0) selftest_prog_t<double2>(shape, nprog, m, nterms, cols, nrhs, A, X, S, Bm, Y, Out);
+    else if (precision == 1) selftest_prog_t<float2>(shape, nprog, m, nterms, cols, nrhs, A, X, S, Bm, Y, Out);
+    else throw std::runtime_error("precision must be 0 or 1");
+  })
+}
+
 extern "C" int aifmm_ls_selftest(int op, int m, int n, int k, const double* A, const double* B, double* C, int* iout,
                       double thr) {
   GUARD({

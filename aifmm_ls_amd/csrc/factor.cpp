@@ -772,7 +772,7 @@ void Factor<T>::sweep_operators() {
 // is applied before the value it changes is next read.
 template <class T>
 void Factor<T>::build_plan(int cap) {
-  plan.reset(new Program<T>());
+  plan.reset(new Program<T>(cap));
   pv.reset();
   plan_cap = cap;
   const int64_t N = pb.N;
@@ -828,7 +828,7 @@ void Factor<T>::build_plan(int cap) {
     yt[j].resize(1 << (2 * j));
     vk[j].resize(1 << (2 * j));
     for (int k = 0; k < (1 << (2 * j)); ++k) {
-      if (B[j][k].m > Program<T>::TILE_M) yt[j][k] = new_blk<T>(pv, B[j][k].m, cap);
+      if (B[j][k].m > Program<T>::tile_m(cap)) yt[j][k] = new_blk<T>(pv, B[j][k].m, cap);
       if (B[j][k].ill) vk[j][k] = new_blk<T>(pv, B[j][k].r, cap);
     }
   }

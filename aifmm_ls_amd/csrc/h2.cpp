@@ -18,7 +18,7 @@ void H2Op::build(int nrhs) {
   const int L = P.L, R = P.R;
   const int N = (int)P.N;
   ws.reset();
-  plan.reset(new Program<double2>());
+  plan.reset(new Program<double2>(nrhs));
   cap = nrhs;
   xin = ws.get<double2>((size_t)N * nrhs);
   yout = ws.get<double2>((size_t)N * nrhs);

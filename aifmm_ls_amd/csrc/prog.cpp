@@ -22,7 +22,7 @@ void Program<T>::add(const BoxOp<T>& op) {
     maxrows = std::max(maxrows, o.rows);
     for (const auto& t : o.terms) uses_y |= t.x == nullptr;
   }
-  if (op.m <= TILE_M && maxrows <= TILE_M) {
+  if (op.m <= tile_m_ && maxrows <= tile_m_) {
     PProg<T> p{};
     p.m = op.m;
     p.src = op.src;
@@ -336,6 +336,18 @@ static bool graphs_enabled() {
     return !(e && e[0] == '0');
   }();
   return on && graph_holds.load() == 0;
+}
+
+template <class T>
+void Program<T>::force_shape(int nrhs, int shape) {
+  std::vector<int> sh(st_.size(), 0);
+  for (size_t k = 0; k < st_.size(); ++k)
+    if (!st_[k].progs.empty() && shape && prog_shape_fits<T>(nrhs, st_[k].maxm, shape)) sh[k] = shape;
+  shapes_[nrhs] = sh;
+  TuneResult& R = tune_[nrhs];
+  R = TuneResult{1, 1, 0, true};
+  last_tune_ = &R;
+  if (gnrhs_ == nrhs) gnrhs_ = -1;
 }
 
 template <class T>

@@ -118,9 +118,19 @@ inline void out_term(typename BoxOp<T>::Out& o, const T* A, int lda, int cols, c
 template <class T>
 class Program {
  public:
-  // boxes taller than this are tiled
-  static constexpr int TILE_M = 64;
+  // Boxes taller than tile_m() rows are cut into TILE-row tiles over two
+  // launches (phase 1 per tile, then the outputs per tile).  The height
+  // depends on the right-hand-side capacity the program is built for.  One
+  // column: 16 rows, so even the 36-row leaves spread over many short
+  // workgroups (measured on MI355X at N = 147,456, complex64 sweep: 64 ->
+  // 1.57 ms, 32 -> 1.40 ms, 16 -> 1.35 ms, 128 -> 1.85 ms; tiles of 8, 12,
+  // 18 or 32 rows were slower than 16).  Several columns: 64 rows, since
+  // each tile's outputs read the box's whole phase-1 vector from memory
+  // (16 RHS: 3.26 ms at 64 rows against 3.63 ms at 16; 64 RHS: 8.1 against
+  // 10.7 ms).
   static constexpr int TILE = 16;
+  static int tile_m(int rhs_capacity) { return rhs_capacity <= 1 ? 16 : 64; }
+  explicit Program(int rhs_capacity = 1) : id_(next_id()), tile_m_(tile_m(rhs_capacity)) {}
 
   // Start a new stage (a launch boundary; a stage with tiled boxes becomes
   // two launches).
@@ -151,13 +161,16 @@ class Program {
     int changed = 0;
     bool adopted = false;
   };
+  // Give every stage `shape` (where it fits) at nrhs instead of tuning (test
+  // hook; a graph captured at nrhs is dropped).
+  void force_shape(int nrhs, int shape);
   const TuneResult* tune_result(int nrhs) const {
     auto it = tune_.find(nrhs);
     return it == tune_.end() ? nullptr : &it->second;
   }
   // the most recent tuning (any nrhs), or null
   const TuneResult* last_tune() const { return last_tune_; }
-  Program() : id_(next_id()) {}
+
   // distinct for every program built in the process (cache keys)
   uint64_t id() const { return id_; }
   Program(const Program&) = delete;
@@ -181,6 +194,7 @@ class Program {
     return ++n;
   }
   const uint64_t id_;
+  const int tile_m_;
   std::vector<Stage> st_;
   std::vector<PTerm<T>> terms_;
   PTerm<T>* dterms_ = nullptr;

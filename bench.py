@@ -348,6 +348,19 @@ def main():
     rel_err = float(np.linalg.norm(x - x_ref) / xn)
     note(f"headline: {args.steps} solves in {total:.3f} s, rel.err vs FP64 {rel_err:.2e}")
 
+    # the same steps again with a HIP event pair around every launch (issued
+    # one by one instead of as a graph), for per-kernel durations; before any
+    # multi-RHS solve rebuilds the sweep for more columns, since the sweep's
+    # tiling depends on its column capacity
+    A.profile_enable(True)
+    A.profile_read(reset=True)
+    t0 = time.perf_counter()
+    timed_solves(g, args.steps, args.refine_tol, args.max_refine)
+    total_prof = time.perf_counter() - t0
+    A.profile_enable(False)
+    prof = A.profile_read(reset=True)
+    note("per-kernel profiled pass done")
+
     # many right-hand sides in one refined solve (plane waves at equally
     # spaced angles): throughput per RHS; the headline stays the single solve
     batched = None
@@ -360,17 +373,7 @@ def main():
         batched = {"nrhs": args.batch_rhs, "solves_per_s": round(3 * args.batch_rhs / float(latb.sum()), 3),
                    "ms_per_batch": round(1e3 * float(np.median(latb)), 3), "refine_steps": int(np.median(stepsb))}
         g.set_rhs(b)
-
-    # the same steps again with a HIP event pair around every launch (issued
-    # one by one instead of as a graph), for per-kernel durations
-    A.profile_enable(True)
-    A.profile_read(reset=True)
-    t0 = time.perf_counter()
-    timed_solves(g, args.steps, args.refine_tol, args.max_refine)
-    total_prof = time.perf_counter() - t0
-    A.profile_enable(False)
-    prof = A.profile_read(reset=True)
-    note("batched-RHS line and per-kernel profiled pass done")
+        note("batched-RHS line done")
 
     # dominant kernel of the profiled pass (by summed event time)
     dom = max(prof, key=lambda k: k["ms"]) if prof else None

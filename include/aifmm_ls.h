@@ -133,6 +133,16 @@ int aifmm_ls_aca(aifmm_ls_problem* h, int tol_pow, int n1, const int64_t* rows, 
  * op + 10 runs the complex64 kernels. */
 int aifmm_ls_selftest(int op, int m, int n, int k, const double* A, const double* B, double* C, int* iout,
                       double thr);
+/* Box-program self-test (no reference counterpart; for the test suite): one
+ * stage of nprog programs of m rows, y_p = S_p + sum_t alpha_t A_{p,t} X_{p,t}
+ * (alpha_t = 1 for even t, -0.5 + 0.25i for odd t; saved to Y) and out_p =
+ * B_p y_p, run in launch shape `shape` (see aifmm_ls_set_tuning; 0 = default)
+ * in complex128 (precision 0) or complex64 (1).  Column-major interleaved
+ * complex: A nprog*nterms blocks m x cols, X nprog*nterms blocks cols x nrhs,
+ * S, Y, Out nprog blocks m x nrhs, Bm nprog blocks m x m. */
+int aifmm_ls_selftest_prog(int precision, int shape, int nprog, int m, int nterms, int cols, int nrhs,
+                           const double* A, const double* X, const double* S, const double* Bm, double* Y,
+                           double* Out);
 
 /* Kernel timing (no reference counterpart; for benchmarks): when enabled, every
  * batched launch records a HIP event pair on its stream together with its

@@ -9,9 +9,11 @@ Every size gives the same solution up to summation order.
 - Complex64 direct solves are compared with the complex128 direct solve of
   the same problem.  Every size uses the same factors, so only the sweep's
   summation order differs.  Each size must be as accurate as the default
-  sizes, within a factor of two.  (Complex64 rounding is amplified by the
-  direct method: at 5 levels two summation orders differ by up to 1e-1, at
-  6 levels by about 3e-3.)
+  sizes, within a factor of five.  The direct method amplifies complex64
+  rounding: at 5 levels its own error is 3e-2 to 1.4e-1 depending on the
+  summation order, and two orders differ by up to 1.6e-1; at 6 levels by
+  about 3e-3.  The kernels themselves are checked against exact products,
+  with no amplification, in test_gpu_programs.py.
 - Refined solves agree to the refinement tolerance.
 - Repeated solves with one tuning are bitwise equal.
 
@@ -80,7 +82,7 @@ def test_every_workgroup_size_gives_the_same_solution(prob, prec):
         else:
             e1, em = rel(x1, y1), rel(xm, ym)
             print(msg + f"; vs complex128 1 RHS {e1:.1e}, 8 RHS {em:.1e}", flush=True)
-            assert e1 <= 2 * e10 + 1e-6 and em <= 2 * em0 + 1e-6
+            assert e1 <= 5 * e10 + 1e-6 and em <= 5 * em0 + 1e-6
         assert dr < 1e3 * TOL
         assert 0 < st["sweep_tuned_time_ratio"] <= 1.0
         if mode not in (1, 1024):

@@ -142,6 +142,100 @@ void launch_sum_partials(const double2* part, const int64_t* off, const int* nbl
   HIP_OK(hipGetLastError());
 }
 
+// ---- several members per launch (grid row = member) -------------------------
+
+// resid_blocks on the device (the same count)
+__device__ __forceinline__ int member_blocks(int64_t n) {
+  int64_t b = (n + VNT - 1) / VNT;
+  return (int)(b < 1024 ? (b < 1 ? 1 : b) : 1024);
+}
+
+__global__ __launch_bounds__(VNT) void k_multidot_b(const VecMember* __restrict__ mem) {
+  const VecMember M = mem[blockIdx.y];
+  const int nb = member_blocks(M.n);
+  if ((int)blockIdx.x >= nb) return;
+  __shared__ double2 sh[VNT / 64];
+  for (int i = 0; i < M.nv; ++i) {
+    const double2* v = M.V + (size_t)i * M.vstride;
+    double2 s = mk(0.0, 0.0);
+    for (int64_t r = (int64_t)blockIdx.x * VNT + threadIdx.x; r < M.n; r += (int64_t)nb * VNT) {
+      double2 a = v[r], b = M.w[r];
+      s.x += a.x * b.x + a.y * b.y;
+      s.y += a.x * b.y - a.y * b.x;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+      s.x += __shfl_xor(s.x, o);
+      s.y += __shfl_xor(s.y, o);
+    }
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double2 t = mk(0.0, 0.0);
+      for (int q = 0; q < VNT / 64; ++q) t.x += sh[q].x, t.y += sh[q].y;
+      M.part[(size_t)i * nb + blockIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(VNT) void k_lincomb_b(const VecMember* __restrict__ mem) {
+  const VecMember M = mem[blockIdx.y];
+  const double2 bt = *M.beta;
+  for (int64_t r = (int64_t)blockIdx.x * VNT + threadIdx.x; r < M.n; r += (int64_t)gridDim.x * VNT) {
+    double2 acc = cmul(bt, M.w[r]);
+    for (int i = 0; i < M.nv; ++i) {
+      const double2 a = M.coef[i], v = M.V[(size_t)i * M.vstride + r];
+      acc.x += a.x * v.x - a.y * v.y;
+      acc.y += a.x * v.y + a.y * v.x;
+    }
+    M.w[r] = acc;
+  }
+}
+
+__global__ __launch_bounds__(VNT) void k_resid_b(const VecMember* __restrict__ mem) {
+  const VecMember M = mem[blockIdx.y];
+  const int nb = member_blocks(M.n);
+  if ((int)blockIdx.x >= nb) return;
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * VNT + threadIdx.x; i < M.n; i += (int64_t)nb * VNT) {
+    double2 v = M.b[i];
+    if (M.ax) {
+      double2 a = M.ax[i];
+      v.x -= a.x;
+      v.y -= a.y;
+    }
+    if (M.r) M.r[i] = v;
+    s += v.x * v.x + v.y * v.y;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+  __shared__ double sh[VNT / 64];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0;
+    for (int w = 0; w < VNT / 64; ++w) t += sh[w];
+    ((double*)M.part)[blockIdx.x] = t;
+  }
+}
+
+void launch_multidot_batch(int nmem, int max_blocks, const VecMember* mem, hipStream_t s) {
+  if (!nmem || max_blocks < 1) return;
+  hipLaunchKernelGGL(k_multidot_b, dim3(max_blocks, nmem), dim3(VNT), 0, s, mem);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_lincomb_batch(int nmem, int max_blocks, const VecMember* mem, hipStream_t s) {
+  if (!nmem || max_blocks < 1) return;
+  hipLaunchKernelGGL(k_lincomb_b, dim3(max_blocks, nmem), dim3(VNT), 0, s, mem);
+  HIP_OK(hipGetLastError());
+}
+
+void launch_resid_batch(int nmem, int max_blocks, const VecMember* mem, hipStream_t s) {
+  if (!nmem || max_blocks < 1) return;
+  hipLaunchKernelGGL(k_resid_b, dim3(max_blocks, nmem), dim3(VNT), 0, s, mem);
+  HIP_OK(hipGetLastError());
+}
+
 int resid_blocks(int64_t n) {
   int64_t b = (n + VNT - 1) / VNT;
   return (int)(b < 1024 ? (b < 1 ? 1 : b) : 1024);

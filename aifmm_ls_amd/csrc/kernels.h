@@ -191,4 +191,25 @@ void launch_sum_partials(const double2* part, const int64_t* off, const int* nbl
 void launch_d2f(const double2* src, float2* dst, int64_t n, hipStream_t s);
 void launch_f2d(const float2* src, double2* dst, int64_t n, hipStream_t s);
 
+// The same vector operations for several independent vectors in one launch
+// (one single-column problem each; member i on grid row i).  Every member
+// uses its own block count resid_blocks(n) and row stride, so its partial
+// sums are bitwise those of the single-member launches.
+struct VecMember {
+  const double2* V;      // basis vectors, V_q = V + q * vstride
+  int64_t vstride;
+  double2* w;            // dot / update target (multidot, lincomb)
+  const double2* b;      // resid: r = b - ax (ax may be null)
+  const double2* ax;
+  double2* r;            // resid output (may be null: norms only)
+  int64_t n;
+  double2* part;         // partial sums (multidot: complex, resid: real)
+  const double2* coef;   // lincomb: nv coefficients
+  const double2* beta;   // lincomb: w scale
+  int nv;
+};
+void launch_multidot_batch(int nmem, int max_blocks, const VecMember* mem, hipStream_t s);
+void launch_lincomb_batch(int nmem, int max_blocks, const VecMember* mem, hipStream_t s);
+void launch_resid_batch(int nmem, int max_blocks, const VecMember* mem, hipStream_t s);
+
 }  // namespace aifmm

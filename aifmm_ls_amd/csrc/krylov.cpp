@@ -281,6 +281,60 @@ std::vector<RefineResult> BatchRefiner::run(const std::vector<BatchVec>& v, cons
     HIP_OK(hipMemcpyAsync(doff, stage_pinned(off.data(), n * 8, s), n * 8, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(dnb, stage_pinned(nb.data(), n * 4, s), n * 4, hipMemcpyHostToDevice, s));
   }
+  // member descriptors of the batched vector launches (one launch per
+  // operation for all members; uploads and launches share the stream, so the
+  // one device array is rewritten only after the previous launch read it)
+  VecMember* dmem = ws_.get<VecMember>(n);
+  int maxblk = 1;
+  for (int i = 0; i < n; ++i) maxblk = std::max(maxblk, mem[i].nblk);
+  std::vector<VecMember> vm(n);
+  auto send = [&]() {
+    const size_t bytes = (size_t)n * sizeof(VecMember);
+    HIP_OK(hipMemcpyAsync(dmem, stage_pinned(vm.data(), bytes, s), bytes, hipMemcpyHostToDevice, s));
+  };
+  // partial squared norms of b_i - ax_i (ax_i null: of b_i; r_i null: not stored)
+  auto resid = [&](const std::vector<const double2*>& b, const std::vector<const double2*>& ax,
+                   const std::vector<double2*>& r) {
+    for (int i = 0; i < n; ++i) {
+      vm[i] = VecMember{};
+      vm[i].b = b[i];
+      vm[i].ax = ax.empty() ? nullptr : ax[i];
+      vm[i].r = r.empty() ? nullptr : r[i];
+      vm[i].n = v[i].N;
+      vm[i].part = dpart + mem[i].poff;
+    }
+    send();
+    launch_resid_batch(n, maxblk, dmem, s);
+  };
+  // partial dots of w_i against V_i[0..nv)
+  auto multidot = [&](const std::vector<double2*>& w, int nv) {
+    for (int i = 0; i < n; ++i) {
+      vm[i] = VecMember{};
+      vm[i].V = mem[i].V;
+      vm[i].vstride = v[i].N;
+      vm[i].w = w[i];
+      vm[i].n = v[i].N;
+      vm[i].part = dpart + mem[i].poff;
+      vm[i].nv = nv;
+    }
+    send();
+    launch_multidot_batch(n, maxblk, dmem, s);
+  };
+  // w_i = dbeta[i] w_i + sum_q dcoef[i (m + 1) + q] B_i[q] (B_i = V_i or Z_i)
+  auto lincomb_dev = [&](const std::vector<double2*>& w, const std::vector<int>& nv, bool from_z) {
+    for (int i = 0; i < n; ++i) {
+      vm[i] = VecMember{};
+      vm[i].V = nv[i] ? (from_z ? mem[i].Z : mem[i].V) : nullptr;
+      vm[i].vstride = v[i].N;
+      vm[i].w = w[i];
+      vm[i].n = v[i].N;
+      vm[i].coef = dcoef + (size_t)i * (m + 1);
+      vm[i].beta = dbeta + i;
+      vm[i].nv = nv[i];
+    }
+    send();
+    launch_lincomb_batch(n, maxblk, dmem, s);
+  };
   std::vector<double2> sums((size_t)n * (m + 2));
   // the members' partial sums reduced on the device, then nv values each to
   // the host: out[i][0..nv)
@@ -308,20 +362,24 @@ std::vector<RefineResult> BatchRefiner::run(const std::vector<BatchVec>& v, cons
       for (int q = 0; q < nv[i]; ++q) hc[(size_t)i * (m + 1) + q] = coef[i][q];
     HIP_OK(hipMemcpyAsync(dcoef, stage_pinned(hc.data(), hc.size() * 16, s), hc.size() * 16, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(dbeta, stage_pinned(beta.data(), n * 16, s), n * 16, hipMemcpyHostToDevice, s));
-    for (int i = 0; i < n; ++i)
-      launch_lincomb(w[i], nv[i] ? mem[i].V : nullptr, v[i].N, nv[i], dcoef + (size_t)i * (m + 1), dbeta + i,
-                     v[i].N, 1, s);
+    lincomb_dev(w, nv, false);
   };
   std::vector<double> bn(n), rn(n);
-  for (int i = 0; i < n; ++i) launch_resid(v[i].b, nullptr, nullptr, v[i].N, 1, (double*)(dpart + mem[i].poff), s);
+  std::vector<const double2*> bs(n);
+  for (int i = 0; i < n; ++i) bs[i] = v[i].b;
+  resid(bs, {}, {});
   fetch_norms(bn);
   int steps = 0;
   for (;;) {
     // true residuals r = b - A x
     for (int i = 0; i < n; ++i) src[i] = v[i].x, dst[i] = mem[i].r;
     A(src, dst);
-    for (int i = 0; i < n; ++i)
-      launch_resid(v[i].b, mem[i].r, mem[i].r, v[i].N, 1, (double*)(dpart + mem[i].poff), s);
+    {
+      std::vector<const double2*> ax(n);
+      std::vector<double2*> rs(n);
+      for (int i = 0; i < n; ++i) ax[i] = rs[i] = mem[i].r;
+      resid(bs, ax, rs);
+    }
     fetch_norms(rn);
     double worst = 0;
     for (int i = 0; i < n; ++i) {
@@ -353,10 +411,8 @@ std::vector<RefineResult> BatchRefiner::run(const std::vector<BatchVec>& v, cons
       ++steps;
       std::vector<int> nvj(n, j + 1);
       std::vector<double2*> w(n);
-      for (int i = 0; i < n; ++i) {
-        w[i] = mem[i].V + (size_t)(j + 1) * v[i].N;
-        launch_multidot(mem[i].V, v[i].N, j + 1, w[i], v[i].N, 1, dpart + mem[i].poff, s);
-      }
+      for (int i = 0; i < n; ++i) w[i] = mem[i].V + (size_t)(j + 1) * v[i].N;
+      multidot(w, j + 1);
       std::vector<std::vector<double2>> h, h2w, neg(n);
       fetch(j + 1, h);
       std::vector<double2> one(n, make_double2(1, 0));
@@ -366,7 +422,7 @@ std::vector<RefineResult> BatchRefiner::run(const std::vector<BatchVec>& v, cons
       }
       lincomb(w, nvj, neg, one);
       // second pass with |w|^2 (w is basis slot j + 1)
-      for (int i = 0; i < n; ++i) launch_multidot(mem[i].V, v[i].N, j + 2, w[i], v[i].N, 1, dpart + mem[i].poff, s);
+      multidot(w, j + 2);
       fetch(j + 2, h2w);
       std::vector<double> hn(n);
       bool exact = false;
@@ -380,7 +436,7 @@ std::vector<RefineResult> BatchRefiner::run(const std::vector<BatchVec>& v, cons
       }
       if (exact) {
         lincomb(w, nvj, neg, one);
-        for (int i = 0; i < n; ++i) launch_resid(w[i], nullptr, nullptr, v[i].N, 1, (double*)(dpart + mem[i].poff), s);
+        resid(std::vector<const double2*>(w.begin(), w.end()), {}, {});
         fetch_norms(hn);
         for (int i = 0; i < n; ++i) sc[i] = make_double2(hn[i] > 0 ? 1.0 / hn[i] : 0.0, 0);
         lincomb(w, nv0, none, sc);
@@ -444,8 +500,11 @@ std::vector<RefineResult> BatchRefiner::run(const std::vector<BatchVec>& v, cons
       for (int q = 0; q < k; ++q) hc[(size_t)i * (m + 1) + q] = y[i][q];
     HIP_OK(hipMemcpyAsync(dcoef, stage_pinned(hc.data(), hc.size() * 16, s), hc.size() * 16, hipMemcpyHostToDevice, s));
     HIP_OK(hipMemcpyAsync(dbeta, stage_pinned(one.data(), n * 16, s), n * 16, hipMemcpyHostToDevice, s));
-    for (int i = 0; i < n; ++i)
-      launch_lincomb(v[i].x, mem[i].Z, v[i].N, k, dcoef + (size_t)i * (m + 1), dbeta + i, v[i].N, 1, s);
+    {
+      std::vector<double2*> xs(n);
+      for (int i = 0; i < n; ++i) xs[i] = v[i].x;
+      lincomb_dev(xs, nvk, true);
+    }
   }
   HIP_OK(hipStreamSynchronize(s));
   return R;

@@ -30,6 +30,8 @@ struct aifmm_ls_problem {
     std::unique_ptr<Arena> arena;
     std::unique_ptr<Program<double2>> s64, mat;
     std::unique_ptr<Program<float2>> s32;
+    NearDesc* near = nullptr;  // the members' near fields (one launch)
+    int near_boxes = 0;
   };
   std::unique_ptr<BatchPlan> bplan;
   int refine_method = 0;
@@ -295,6 +297,12 @@ int aifmm_ls_solve_batch(aifmm_ls_problem* const* hs, int n, const double* const
       bp->mat.reset(new Program<double2>());
       bp->mat->merge_from(pm);
       bp->mat->finalize(*bp->arena, s);
+      std::vector<NearDesc> nd(n);
+      for (int i = 0; i < n; ++i) {
+        nd[i] = hs[i]->h2->near_desc(1);
+        bp->near_boxes = std::max(bp->near_boxes, nd[i].nbox);
+      }
+      bp->near = upload(*bp->arena, nd, s);
       if (prec == 0) {
         bp->s64.reset(new Program<double2>());
         bp->s64->merge_from(ps64);
@@ -331,6 +339,7 @@ int aifmm_ls_solve_batch(aifmm_ls_problem* const* hs, int n, const double* const
       };
       load();
       mat.run(1, s, load);
+      launch_near(n, bp->near_boxes, bp->near, s);
       for (int i = 0; i < n; ++i) hs[i]->h2->store(dst[i], 1, s);
     };
     std::vector<BatchVec> bv(n);

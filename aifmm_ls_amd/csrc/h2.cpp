@@ -5,7 +5,10 @@
 //   downward, level j = 2..L:  lo_j[k] = sum_{b in IL(k)} M2L[k][b] mu_j[b]
 //                                        + (L2P_parent lo_{j-1}[parent]) rows of k
 //   leaves:                    y_k = L2P_k lo_L[k] + sum_{nn in N(k) and k} P2P[k][nn] x_nn
+// (the near-field sum by k_near after the program, entries from the tables)
 #include "h2.h"
+#include "batch.h"
+#include "kernels.h"
 
 #include <vector>
 
@@ -24,6 +27,8 @@ void H2Op::build(int nrhs) {
   yout = ws.get<double2>((size_t)N * nrhs);
   const double2* x = xin;
   double2* y = yout;
+  if (R > 64) throw std::runtime_error("near field kernel takes at most 64 points per leaf");
+  std::vector<int> nbr((size_t)9 << (2 * L), -1);
   std::vector<std::vector<int>> off(L + 1);
   std::vector<Blk<double2>> Mu(L + 1), Lo(L + 1);
   for (int j = 2; j <= L; ++j) {
@@ -92,19 +97,63 @@ void H2Op::build(int nrhs) {
           need(B.L2P.r == R && B.L2P.c == B.r, "leaf L2P");
           out_term<double2>(out, B.L2P.p, B.L2P.ld, B.r, nullptr, 0, ONE);
         }
+        // the near field (P2P) is added by k_near after the program, from
+        // the tables: the same entries without reading the assembled blocks
         for (int slot = 0; slot < 9; ++slot) {
           int nn = slot == 8 ? k : P.lists[L][k].nb[slot];
+          nbr[(size_t)k * 9 + slot] = -1;
           if (nn < 0) continue;
           auto it = P.P2P[k].find(nn);
           if (it == P.P2P[k].end() || !it->second.size()) continue;
           need(it->second.r == R && it->second.c == R, "P2P");
-          out_term<double2>(out, it->second.p, it->second.ld, R, x + (size_t)nn * R, N, ONE);
+          nbr[(size_t)k * 9 + slot] = nn;
         }
       }
       G.add(op);
     }
   }
   G.finalize(ws, P.stream);
+  // kappa^2 T, each entry scaled as the assembled entries are (T kappa kappa)
+  {
+    const double kap = P.ctx.kappa;
+    std::vector<double2> tk(P.T_h.size());
+    for (size_t e = 0; e < tk.size(); ++e) {
+      double2 v = P.T_h[e];
+      v.x *= kap, v.y *= kap;
+      v.x *= kap, v.y *= kap;
+      tk[e] = v;
+    }
+    dtk = ws.get<double2>(tk.size());
+    HIP_OK(hipMemcpyAsync(dtk, stage_pinned(tk.data(), tk.size() * sizeof(double2), P.stream),
+                          tk.size() * sizeof(double2), hipMemcpyHostToDevice, P.stream));
+  }
+  dnbr = ws.get<int>(nbr.size());
+  HIP_OK(hipMemcpyAsync(dnbr, stage_pinned(nbr.data(), nbr.size() * sizeof(int), P.stream), nbr.size() * sizeof(int),
+                        hipMemcpyHostToDevice, P.stream));
+}
+
+NearDesc H2Op::near_desc(int nrhs) const {
+  NearDesc d{};
+  d.T = dtk;
+  d.q = pb.ctx.q;
+  d.nbr = dnbr;
+  d.x = xin;
+  d.y = yout;
+  d.kappa = pb.ctx.kappa;
+  d.L = pb.L;
+  d.R = pb.R;
+  d.nbox = 1 << (2 * pb.L);
+  d.nrhs = nrhs;
+  d.N = pb.N;
+  return d;
+}
+
+void H2Op::near(int nrhs, hipStream_t s) {
+  const NearDesc d = near_desc(nrhs);
+  NearDesc* dd = (NearDesc*)stage_dev.alloc(sizeof(NearDesc));
+  HIP_OK(hipMemcpyAsync(dd, stage_pinned(&d, sizeof(d), s), sizeof(d), hipMemcpyHostToDevice, s));
+  launch_near(1, d.nbox, dd, s);
+  stage_dev.reset();
 }
 
 Program<double2>& H2Op::program(int nrhs) {
@@ -125,6 +174,7 @@ void H2Op::apply(const double2* x, double2* y, int nrhs) {
   Program<double2>& P = program(nrhs);
   load(x, nrhs, pb.stream);
   P.run(nrhs, pb.stream, [&] { load(x, nrhs, pb.stream); });
+  near(nrhs, pb.stream);
   store(y, nrhs, pb.stream);
 }
 

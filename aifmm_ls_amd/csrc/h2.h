@@ -5,6 +5,7 @@
 #pragma once
 #include "problem.h"
 #include "prog.h"
+#include "kernels.h"
 
 #include <memory>
 
@@ -27,6 +28,10 @@ class H2Op {
   Program<double2>& program(int nrhs);
   void load(const double2* x, int nrhs, hipStream_t s);
   void store(double2* y, int nrhs, hipStream_t s);
+  // The near field, added to the program's output after it ran: launched
+  // here (near), or described for a launch over several problems.
+  void near(int nrhs, hipStream_t s);
+  NearDesc near_desc(int nrhs) const;
 
  private:
   Problem& pb;
@@ -34,6 +39,9 @@ class H2Op {
   std::unique_ptr<Program<double2>> plan;
   double2* xin = nullptr;
   double2* yout = nullptr;
+  int* dnbr = nullptr;      // leaf boxes x 9 near-field neighbours
+  double2* dtk = nullptr;   // kappa^2 x the near-field tables
+  Arena stage_dev{size_t(1) << 20};  // the near-field descriptor
   int cap = 0;
   void build(int nrhs);
 };

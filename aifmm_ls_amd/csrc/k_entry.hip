@@ -129,6 +129,82 @@ __device__ double2 ls_entry(const EntryCtx& C, int64_t i, int64_t j) {
   return s;
 }
 
+// One workgroup per (leaf box, problem); wave w takes neighbour slots w,
+// w + 4, w + 8; lane = row of the box (R <= 64).  The near entries are
+// kappa^2 q_i T_nb[ii, jj] (+ 1 on the diagonal): with D.T = kappa^2 T, each
+// row sums T_nb x_nn over its slots and is scaled by q_i once, plus x_i.
+// (The assembled blocks round each entry and zero those below 1e-16; here
+// the product differs from theirs at rounding level only.)
+__device__ __forceinline__ double2 near_fma(double2 a, double2 b, double2 c) {
+  c.x = fma(a.x, b.x, c.x);
+  c.x = fma(-a.y, b.y, c.x);
+  c.y = fma(a.x, b.y, c.y);
+  c.y = fma(a.y, b.x, c.y);
+  return c;
+}
+
+__global__ __launch_bounds__(256) void k_near(const NearDesc* __restrict__ ds) {
+  const NearDesc D = ds[blockIdx.y];
+  const int k = blockIdx.x;
+  if (k >= D.nbox) return;
+  const int R = D.R, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  __shared__ double2 red[4][64];
+  int xk, yk;
+  box_xy(D.L, k, xk, yk);
+  const int64_t i = (int64_t)k * R + lane;
+  // the table of each slot this wave takes
+  int nn_s[3], t_s[3];
+  for (int q = 0; q < 3; ++q) {
+    const int sl = w + 4 * q;
+    nn_s[q] = sl < 9 ? D.nbr[(size_t)k * 9 + sl] : -1;
+    t_s[q] = 8;
+    if (nn_s[q] >= 0) {
+      int xn, yn;
+      box_xy(D.L, nn_s[q], xn, yn);
+      const int dx = xn - xk, dy = yn - yk;
+      if (dx || dy)
+        for (int o = 0; o < 8; ++o)
+          if (c_nb_off[o][0] == dx && c_nb_off[o][1] == dy) t_s[q] = o;
+    }
+  }
+  for (int c = 0; c < D.nrhs; ++c) {
+    double2 a0 = mk(0.0, 0.0), a1 = mk(0.0, 0.0);
+    if (lane < R)
+      for (int q = 0; q < 3; ++q) {
+        if (nn_s[q] < 0) continue;
+        const double2* Tt = D.T + (size_t)t_s[q] * R * R + lane;
+        const double2* xs = D.x + (size_t)D.N * c + (size_t)nn_s[q] * R;
+        int jj = 0;
+        for (; jj + 1 < R; jj += 2) {
+          const double2 e0 = Tt[(size_t)R * jj], e1 = Tt[(size_t)R * (jj + 1)];
+          const double2 v0 = xs[jj], v1 = xs[jj + 1];
+          a0 = near_fma(e0, v0, a0);
+          a1 = near_fma(e1, v1, a1);
+        }
+        if (jj < R) a0 = near_fma(Tt[(size_t)R * jj], xs[jj], a0);
+      }
+    red[w][lane] = cadd(a0, a1);
+    __syncthreads();
+    if (w == 0 && lane < R) {
+      double2 t = red[0][lane];
+      t = cadd(t, red[1][lane]);
+      t = cadd(t, red[2][lane]);
+      t = cadd(t, red[3][lane]);
+      const size_t o = (size_t)D.N * c + i;
+      double2 v = cscale(t, D.q[i]);
+      v = cadd(v, D.x[o]);  // the identity
+      D.y[o] = cadd(D.y[o], v);
+    }
+    __syncthreads();
+  }
+}
+
+void launch_near(int nmem, int max_box, const NearDesc* d, hipStream_t s) {
+  if (!nmem || max_box < 1) return;
+  hipLaunchKernelGGL(k_near, dim3(max_box, nmem), dim3(256), 0, s, d);
+  HIP_OK(hipGetLastError());
+}
+
 __global__ void k_entries(EntryCtx C, int64_t n, const int64_t* __restrict__ I, const int64_t* __restrict__ Jv,
                           double2* __restrict__ out) {
   int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;

@@ -191,6 +191,23 @@ void launch_sum_partials(const double2* part, const int64_t* off, const int* nbl
 void launch_d2f(const double2* src, float2* dst, int64_t n, hipStream_t s);
 void launch_f2d(const float2* src, double2* dst, int64_t n, hipStream_t s);
 
+// Near field of the FP64 operator, rebuilt from the translation-invariant
+// tables instead of read from the assembled blocks: for every leaf box k and
+// each of its neighbour slots (8 neighbours and the box itself),
+// y_k += P2P(k, nn) x_nn with P2P(k, nn)[ii, jj] = kappa^2 q_i T_nb[ii, jj]
+// (+ 1 on the diagonal), at rounding level from the assembled product.
+struct NearDesc {
+  const double2* T;  // kappa^2 x the 9 tables R x R (column-major; slot 8: the box itself)
+  const double* q;   // contrast at the N points
+  const int* nbr;    // nbox x 9 neighbour boxes (-1: none; slot 8: the box)
+  const double2* x;  // N x nrhs, ld N
+  double2* y;        // accumulated into
+  double kappa;
+  int L, R, nbox, nrhs;
+  int64_t N;
+};
+void launch_near(int nmem, int max_box, const NearDesc* d, hipStream_t s);
+
 // The same vector operations for several independent vectors in one launch
 // (one single-column problem each; member i on grid row i).  Every member
 // uses its own block count resid_blocks(n) and row stride, so its partial

@@ -15,6 +15,9 @@
 namespace aifmm {
 
 static const double2 ONE = {1.0, 0.0};
+// offsets (dx, dy) of the near-field tables 0..7 (table 8: the box itself),
+// the order of the neighbour slots and of the entry kernel's tables
+static const int NEAR_OFF[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {1, 0}, {1, 1}, {0, 1}, {-1, 1}, {-1, 0}};
 
 void H2Op::build(int nrhs) {
   Problem& P = pb;
@@ -106,6 +109,15 @@ void H2Op::build(int nrhs) {
           auto it = P.P2P[k].find(nn);
           if (it == P.P2P[k].end() || !it->second.size()) continue;
           need(it->second.r == R && it->second.c == R, "P2P");
+          // k_near takes table `slot` for this neighbour: its offset must be
+          // the table's (the neighbour lists and the tables share the order)
+          if (slot < 8) {
+            int xk, yk, xn, yn;
+            box_xy(L, k, xk, yk);
+            box_xy(L, nn, xn, yn);
+            if (xn - xk != NEAR_OFF[slot][0] || yn - yk != NEAR_OFF[slot][1])
+              throw std::runtime_error("neighbour slot does not match its near-field table");
+          }
           nbr[(size_t)k * 9 + slot] = nn;
         }
       }

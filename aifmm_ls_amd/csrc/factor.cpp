@@ -369,6 +369,16 @@ void Factor<T>::recompress(int j) {
     if (touched[u]) tl.push_back(u);
   int nt = (int)tl.size();
   st.recompressions += nt;
+  std::vector<int> ti(nb, -1);
+  for (int i = 0; i < nt; ++i) ti[tl[i]] = i;
+  // each touched box's row fills (it is the target) and column fills (it is
+  // the source), in fill order: the column order of its two stacks
+  std::vector<std::vector<int>> rowf(nt), colf(nt);
+  for (size_t q = 0; q < fills.size(); ++q) {
+    const Fill& f = fills[q];
+    if (f.kind == 0 || f.kind == 1) rowf[ti[f.u]].push_back((int)q);
+    if (f.kind == 0 || f.kind == 2) colf[ti[f.p]].push_back((int)q);
+  }
   // stacked matrices
   std::vector<Blk<T>> Arow(nt), Acol(nt);
   CopyBatch<T> cp;
@@ -377,29 +387,23 @@ void Factor<T>::recompress(int j) {
     FBox& U = Bx[u];
     int m = U.m;
     int nr = U.r, ncl = U.r;
-    for (auto& f : fills) {
-      if (f.u == u && f.kind == 0) nr += Bx[f.p].m;
-      if (f.u == u && f.kind == 1) nr += Bx[f.p].r;
-      if (f.p == u && f.kind == 0) ncl += Bx[f.u].m;
-      if (f.p == u && f.kind == 2) ncl += Bx[f.u].r;
-    }
+    for (int q : rowf[i]) nr += fills[q].kind == 0 ? Bx[fills[q].p].m : Bx[fills[q].p].r;
+    for (int q : colf[i]) ncl += fills[q].kind == 0 ? Bx[fills[q].u].m : Bx[fills[q].u].r;
     Arow[i] = new_blk<T>(ar, m, nr);
     Acol[i] = new_blk<T>(ar, m, ncl);
     if (!m) continue;
     int oR = 0, oC = 0;
     if (U.r) cp.add(U.L2P, Arow[i].sub(0, 0, m, U.r)), cp.add(U.P2M, Acol[i].sub(0, 0, m, U.r), 1);
     oR = oC = U.r;
-    for (auto& f : fills) {
-      if (f.u == u && (f.kind == 0 || f.kind == 1)) {
-        Blk<T>& F = blk_of(f);
-        if (F.c) cp.add(F, Arow[i].sub(0, oR, m, F.c));
-        oR += F.c;
-      }
-      if (f.p == u && (f.kind == 0 || f.kind == 2)) {
-        Blk<T>& F = blk_of(f);
-        if (F.r) cp.add(F, Acol[i].sub(0, oC, m, F.r), 1);
-        oC += F.r;
-      }
+    for (int q : rowf[i]) {
+      Blk<T>& F = blk_of(fills[q]);
+      if (F.c) cp.add(F, Arow[i].sub(0, oR, m, F.c));
+      oR += F.c;
+    }
+    for (int q : colf[i]) {
+      Blk<T>& F = blk_of(fills[q]);
+      if (F.r) cp.add(F, Acol[i].sub(0, oC, m, F.r), 1);
+      oC += F.r;
     }
   }
   cp.run(sc, pb.stream);
@@ -475,8 +479,6 @@ void Factor<T>::recompress(int j) {
     }
   }
   g.run(sc, pb.stream);
-  std::vector<int> ti(nb, -1);
-  for (int i = 0; i < nt; ++i) ti[tl[i]] = i;
   auto il_of = [&](int u, std::vector<int>& v) {
     v.clear();
     for (int s = 0; s < 16; ++s)

@@ -630,7 +630,9 @@ void Factor<T>::factor() {
   }
   base_factor();
   sweep_operators();
+  dense_top_alloc();
   build_plan(1);
+  dense_top_fill();
   HIP_OK(hipStreamSynchronize(pb.stream));
   sc.release();  // factorisation scratch; later users allocate afresh
   st.t_factor = now_s() - t0;
@@ -773,22 +775,31 @@ void Factor<T>::sweep_operators() {
 // A pulled update equals the reference's pushed one: each neighbour update
 // is applied before the value it changes is next read.
 template <class T>
-void Factor<T>::build_plan(int cap) {
-  plan.reset(new Program<T>(cap));
-  pv.reset();
-  plan_cap = cap;
-  const int64_t N = pb.N;
-  Bt = new_blk<T>(pv, (int)N, cap);
-  Xt = new_blk<T>(pv, (int)N, cap);
+static void need_shape(const Blk<T>& b, int rows, int cols, const char* what) {
+  if (b.r != rows || b.c != cols) throw std::runtime_error(std::string("sweep block shape mismatch: ") + what);
+}
+
+// Vectors of the sweep at levels 2..jhi for cap right-hand sides, from arena
+// a.  The leaves (jhi == L) are slices of Bt / Xt.  With b2 / x2 given, the
+// level-2 boxes' b and x are slices of them (n2 x cap, boxes stacked in
+// order), so the level-2 part of the sweep can run as one dense product.
+template <class T>
+void Factor<T>::alloc_vectors(Arena& a, int cap, int jhi, const Blk<T>* b2, const Blk<T>* x2) {
+  if (jhi == L) {
+    Bt = new_blk<T>(a, (int)pb.N, cap);
+    Xt = new_blk<T>(a, (int)pb.N, cap);
+  }
   const int nbase = boff.empty() ? 0 : boff[16];
-  yb = new_blk<T>(pv, nbase, cap);
-  xb = new_blk<T>(pv, nbase, cap);
+  yb = new_blk<T>(a, nbase, cap);
+  xb = new_blk<T>(a, nbase, cap);
   bk.assign(L + 1, {});
   bs.assign(L + 1, {});
   mr.assign(L + 1, {});
   xk.assign(L + 1, {});
   yk.assign(L + 1, {});
-  for (int j = L; j >= 2; --j) {
+  yt.assign(L + 1, {});
+  vk.assign(L + 1, {});
+  for (int j = jhi; j >= 2; --j) {
     int nb = 1 << (2 * j);
     bk[j].resize(nb);
     bs[j].resize(nb);
@@ -799,14 +810,17 @@ void Factor<T>::build_plan(int cap) {
         if (F.m != R) throw std::runtime_error("leaf block size mismatch");
         bk[j][k] = Bt.sub(k * R, 0, R, cap);
         xk[j][k] = Xt.sub(k * R, 0, R, cap);
+      } else if (j == 2 && b2) {
+        bk[j][k] = b2->sub(off2[k], 0, F.m, cap);
+        xk[j][k] = x2->sub(off2[k], 0, F.m, cap);
       } else {
-        bk[j][k] = new_blk<T>(pv, F.m, cap);
-        xk[j][k] = new_blk<T>(pv, F.m, cap);
+        bk[j][k] = new_blk<T>(a, F.m, cap);
+        xk[j][k] = new_blk<T>(a, F.m, cap);
       }
-      bs[j][k] = new_blk<T>(pv, F.m, cap);
+      bs[j][k] = new_blk<T>(a, F.m, cap);
     }
   }
-  for (int j = L; j >= 2; --j) {
+  for (int j = jhi; j >= 2; --j) {
     int nb = 1 << (2 * j);
     mr[j].resize(nb);
     yk[j].resize(nb);
@@ -824,22 +838,21 @@ void Factor<T>::build_plan(int cap) {
     }
   }
   // phase-1 vectors of tall boxes in the back-substitution (no save target)
-  std::vector<std::vector<Blk<T>>> yt(L + 1);
-  vk.assign(L + 1, {});
-  for (int j = 2; j <= L; ++j) {
+  for (int j = 2; j <= jhi; ++j) {
     yt[j].resize(1 << (2 * j));
     vk[j].resize(1 << (2 * j));
     for (int k = 0; k < (1 << (2 * j)); ++k) {
-      if (B[j][k].m > Program<T>::tile_m(cap)) yt[j][k] = new_blk<T>(pv, B[j][k].m, cap);
-      if (B[j][k].ill) vk[j][k] = new_blk<T>(pv, B[j][k].r, cap);
+      if (B[j][k].m > Program<T>::tile_m(cap)) yt[j][k] = new_blk<T>(a, B[j][k].m, cap);
+      if (B[j][k].ill) vk[j][k] = new_blk<T>(a, B[j][k].r, cap);
     }
   }
-  Program<T>& P = *plan;
-  auto need = [](const Blk<T>& b, int rows, int cols, const char* what) {
-    if (b.r != rows || b.c != cols) throw std::runtime_error(std::string("sweep block shape mismatch: ") + what);
-  };
-  // forward
-  for (int j = L; j >= 2; --j) {
+}
+
+// forward sweep of levels jhi..jlo (each level's colours, then its deferred
+// P2L updates)
+template <class T>
+void Factor<T>::emit_forward(Program<T>& P, int jlo, int jhi) {
+  for (int j = jhi; j >= jlo; --j) {
     const auto& LS = pb.lists[j];
     for (int c = 0; c < 9; ++c) {
       P.stage();
@@ -857,7 +870,7 @@ void Factor<T>::build_plan(int cap) {
           if (nn < 0 || B[j][nn].color >= c) continue;
           auto it = F.P2P.find(nn);
           if (it == F.P2P.end() || !it->second.size()) continue;
-          need(it->second, F.m, B[j][nn].m, "P2P (forward)");
+          need_shape(it->second, F.m, B[j][nn].m, "P2P (forward)");
           op.term(it->second.p, it->second.ld, it->second.c, bk[j][nn].p, bk[j][nn].ld, MONE);
         }
         const Blk<T>& Zf = F.ill ? F.Ai : F.Z;
@@ -901,13 +914,17 @@ void Factor<T>::build_plan(int cap) {
         if (k < 0 || B[j][k].color <= Fn.color || !B[j][k].m) continue;
         auto it = Fn.P2L.find(k);
         if (it == Fn.P2L.end() || !it->second.size()) continue;
-        need(it->second, Fn.r, B[j][k].m, "P2L");
+        need_shape(it->second, Fn.r, B[j][k].m, "P2L");
         op.term(it->second.p, it->second.ld, it->second.c, bk[j][k].p, bk[j][k].ld, MONE);
       }
       if (!op.terms.empty()) P.add(op);
     }
   }
-  // base
+}
+
+template <class T>
+void Factor<T>::emit_base(Program<T>& P) {
+  const int nbase = boff.empty() ? 0 : boff[16];
   if (nbase) {
     P.stage();
     BoxOp<T> op;
@@ -915,8 +932,12 @@ void Factor<T>::build_plan(int cap) {
     out_term<T>(op.out(xb.p, xb.ld, nbase), binv.p, binv.ld, nbase, yb.p, yb.ld, ONE);
     P.add(op);
   }
-  // backward
-  for (int j = 2; j <= L; ++j) {
+}
+
+// back-substitution of levels jlo..jhi
+template <class T>
+void Factor<T>::emit_backward(Program<T>& P, int jlo, int jhi) {
+  for (int j = jlo; j <= jhi; ++j) {
     const auto& LS = pb.lists[j];
     for (int c = 8; c >= 0; --c) {
       P.stage();
@@ -937,7 +958,7 @@ void Factor<T>::build_plan(int cap) {
           auto it = rm.find(nn);
           if (it == rm.end() || !it->second.size()) continue;
           const Blk<T>& v = after ? xk[j][nn] : yk[j][nn];
-          need(it->second, F.m, v.r, after ? "P2P (backward)" : "M2P");
+          need_shape(it->second, F.m, v.r, after ? "P2P (backward)" : "M2P");
           op.term(it->second.p, it->second.ld, it->second.c, v.p, v.ld, MONE);
         }
         auto& o = op.out(xk[j][k].p, xk[j][k].ld, F.m);
@@ -968,6 +989,83 @@ void Factor<T>::build_plan(int cap) {
         P.add(op);
       }
     }
+  }
+}
+
+// The level-2 part of the sweep (forward at level 2, the base solve and the
+// back-substitution at level 2) maps the level-2 right-hand sides b_k
+// (stacked over the 16 boxes) linearly to the level-2 solutions x_k and reads
+// nothing else.  Its matrix D2 (n2 x n2, n2 = sum of the level-2 box sizes)
+// is formed once per factorisation, by running that part of the sweep on the
+// n2 columns of the identity; the solve sweeps then apply it as one dense
+// product instead of about 38 short dependent launches over 16 boxes.
+// dense_top_alloc sizes and allocates D2 (build_plan only needs where it
+// lives); dense_top_fill forms it after build_plan, whose host work overlaps
+// the sweep-operator kernels.  AIFMM_DENSE_TOP=0 keeps the staged form.
+template <class T>
+void Factor<T>::dense_top_alloc() {
+  d2 = Blk<T>{};
+  n2 = 0;
+  static const bool off = [] {
+    const char* e = getenv("AIFMM_DENSE_TOP");
+    return e && e[0] == '0';
+  }();
+  if (off || L < 4) return;
+  off2.assign(17, 0);
+  for (int k = 0; k < 16; ++k) off2[k + 1] = off2[k] + B[2][k].m;
+  const int n = off2[16];
+  if (n == 0 || n > kDenseTopMax) return;
+  d2 = new_blk<T>(ar, n, n);
+  n2 = n;
+}
+
+template <class T>
+void Factor<T>::dense_top_fill() {
+  if (!n2) return;
+  const int n = n2;
+  Blk<T> b2 = new_blk<T>(sc, n, n), x2 = new_blk<T>(sc, n, n);
+  alloc_vectors(sc, n, 2, &b2, &x2);  // the sweep program already holds its own
+  Program<T> P(n);
+  emit_forward(P, 2, 2);
+  emit_base(P);
+  emit_backward(P, 2, 2);
+  P.finalize(sc, pb.stream);
+  // 256 threads, 2 columns per pass: the fastest launch shape for these
+  // ~800-column runs on MI355X (0.8 ms against 1.5 ms at the default shape)
+  P.force_shape(n, 256 + 2 * PROG_SHAPE_CH);
+  CopyBatch<T> cp;
+  cp.add(b2, b2, 3);
+  cp.run(sc, pb.stream);
+  P.run(n, pb.stream);
+  cp.add(x2, d2);
+  cp.run(sc, pb.stream);
+  HIP_OK(hipStreamSynchronize(pb.stream));  // before P and its launches go
+}
+
+template <class T>
+void Factor<T>::build_plan(int cap) {
+  plan.reset(new Program<T>(cap));
+  pv.reset();
+  plan_cap = cap;
+  Program<T>& P = *plan;
+  if (n2) {
+    Blk<T> b2 = new_blk<T>(pv, n2, cap), x2 = new_blk<T>(pv, n2, cap);
+    alloc_vectors(pv, cap, L, &b2, &x2);
+    emit_forward(P, 3, L);
+    // levels 2 and base: x2 = D2 b2, in row tiles
+    P.stage();
+    BoxOp<T> op;
+    op.m = n2;
+    op.save = x2.p;
+    op.ldsave = x2.ld;
+    op.term(d2.p, d2.ld, n2, b2.p, b2.ld, ONE);
+    P.add(op);
+    emit_backward(P, 3, L);
+  } else {
+    alloc_vectors(pv, cap, L, nullptr, nullptr);
+    emit_forward(P, 2, L);
+    emit_base(P);
+    emit_backward(P, 2, L);
   }
   P.finalize(pv, pb.stream);
 }

@@ -164,9 +164,22 @@ class Factor {
   int plan_cap = 0;
   Blk<T> Bt, Xt, yb, xb;
   std::vector<std::vector<Blk<T>>> bk, bs, mr, xk, yk;
+  std::vector<std::vector<Blk<T>>> yt;  // phase-1 vectors of tall boxes (back-substitution)
   std::vector<std::vector<Blk<T>>> vk;  // y_k - P2M A^{-1} x0 of the ill boxes
   Blk<T> eye;                           // identity, rank x rank of the largest box
+  // the level-2 part of the sweep as one dense matrix (n2 x n2; n2 = 0: the
+  // staged form), level-2 box offsets in its stacked vectors
+  static constexpr int kDenseTopMax = 6144;
+  Blk<T> d2;
+  int n2 = 0;
+  std::vector<int> off2;
   void sweep_operators();
+  void dense_top_alloc();
+  void dense_top_fill();
+  void alloc_vectors(Arena& a, int cap, int jhi, const Blk<T>* b2, const Blk<T>* x2);
+  void emit_forward(Program<T>& P, int jlo, int jhi);
+  void emit_base(Program<T>& P);
+  void emit_backward(Program<T>& P, int jlo, int jhi);
   void build_plan(int cap);
 
   void init_level(int j);

@@ -781,8 +781,9 @@ static void need_shape(const Blk<T>& b, int rows, int cols, const char* what) {
 
 // Vectors of the sweep at levels 2..jhi for cap right-hand sides, from arena
 // a.  The leaves (jhi == L) are slices of Bt / Xt.  With b2 / x2 given, the
-// level-2 boxes' b and x are slices of them (n2 x cap, boxes stacked in
-// order), so the level-2 part of the sweep can run as one dense product.
+// boxes of the dense level jd have their b and x as slices of them (n2 x cap,
+// boxes stacked in order), so the sweep at levels jd..2 can run as one dense
+// product.
 template <class T>
 void Factor<T>::alloc_vectors(Arena& a, int cap, int jhi, const Blk<T>* b2, const Blk<T>* x2) {
   if (jhi == L) {
@@ -810,7 +811,7 @@ void Factor<T>::alloc_vectors(Arena& a, int cap, int jhi, const Blk<T>* b2, cons
         if (F.m != R) throw std::runtime_error("leaf block size mismatch");
         bk[j][k] = Bt.sub(k * R, 0, R, cap);
         xk[j][k] = Xt.sub(k * R, 0, R, cap);
-      } else if (j == 2 && b2) {
+      } else if (j == jd && b2) {
         bk[j][k] = b2->sub(off2[k], 0, F.m, cap);
         xk[j][k] = x2->sub(off2[k], 0, F.m, cap);
       } else {
@@ -992,31 +993,38 @@ void Factor<T>::emit_backward(Program<T>& P, int jlo, int jhi) {
   }
 }
 
-// The level-2 part of the sweep (forward at level 2, the base solve and the
-// back-substitution at level 2) maps the level-2 right-hand sides b_k
-// (stacked over the 16 boxes) linearly to the level-2 solutions x_k and reads
-// nothing else.  Its matrix D2 (n2 x n2, n2 = sum of the level-2 box sizes)
-// is formed once per factorisation, by running that part of the sweep on the
-// n2 columns of the identity; the solve sweeps then apply it as one dense
-// product instead of about 38 short dependent launches over 16 boxes.
-// dense_top_alloc sizes and allocates D2 (build_plan only needs where it
-// lives); dense_top_fill forms it after build_plan, whose host work overlaps
-// the sweep-operator kernels.  AIFMM_DENSE_TOP=0 keeps the staged form.
+// The top of the sweep (the forward sweep at levels jd..2, the base solve and
+// the back-substitution at levels 2..jd) maps the stacked right-hand sides b_k
+// of the level-jd boxes linearly to their solutions x_k and reads nothing
+// else.  Its matrix D (n2 x n2, n2 = sum of the level-jd box sizes) is formed
+// once per factorisation, by running that part of the sweep on the n2 columns
+// of the identity; the solve sweeps then apply it as one dense product
+// instead of about 38 short dependent launches per level.  jd is 3 when its
+// n2 is at most kDenseTopMax, else 2 under the same bound, else the sweep
+// stays staged.  dense_top_alloc sizes and allocates D (build_plan only needs
+// where it lives); dense_top_fill forms it after build_plan, whose host work
+// overlaps the sweep-operator kernels.  AIFMM_DENSE_TOP=0 keeps the staged
+// form, AIFMM_DENSE_TOP=2 stops at level 2.
 template <class T>
 void Factor<T>::dense_top_alloc() {
   d2 = Blk<T>{};
   n2 = 0;
-  static const bool off = [] {
+  static const int top = [] {
     const char* e = getenv("AIFMM_DENSE_TOP");
-    return e && e[0] == '0';
+    return e && e[0] == '0' ? 0 : e && e[0] == '2' ? 2 : 3;
   }();
-  if (off || L < 4) return;
-  off2.assign(17, 0);
-  for (int k = 0; k < 16; ++k) off2[k + 1] = off2[k] + B[2][k].m;
-  const int n = off2[16];
-  if (n == 0 || n > kDenseTopMax) return;
-  d2 = new_blk<T>(ar, n, n);
-  n2 = n;
+  for (int j = top; j >= 2; --j) {
+    if (L < j + 2) continue;
+    const int nb = 1 << (2 * j);
+    std::vector<int> off(nb + 1, 0);
+    for (int k = 0; k < nb; ++k) off[k + 1] = off[k] + B[j][k].m;
+    if (off[nb] == 0 || off[nb] > kDenseTopMax) continue;
+    jd = j;
+    off2 = off;
+    n2 = off[nb];
+    d2 = new_blk<T>(ar, n2, n2);
+    return;
+  }
 }
 
 template <class T>
@@ -1024,14 +1032,15 @@ void Factor<T>::dense_top_fill() {
   if (!n2) return;
   const int n = n2;
   Blk<T> b2 = new_blk<T>(sc, n, n), x2 = new_blk<T>(sc, n, n);
-  alloc_vectors(sc, n, 2, &b2, &x2);  // the sweep program already holds its own
+  alloc_vectors(sc, n, jd, &b2, &x2);  // the sweep program already holds its own
   Program<T> P(n);
-  emit_forward(P, 2, 2);
+  emit_forward(P, 2, jd);
   emit_base(P);
-  emit_backward(P, 2, 2);
+  emit_backward(P, 2, jd);
   P.finalize(sc, pb.stream);
-  // 256 threads, 2 columns per pass: the fastest launch shape for these
-  // ~800-column runs on MI355X (0.8 ms against 1.5 ms at the default shape)
+  // 256 threads, 2 columns per pass: the fastest launch shape for these runs
+  // on MI355X (N = 147,456: level 3, 2,485 columns, 7.3 ms against 16.1 ms at
+  // the default shape; level 2, 790 columns, 0.8 against 1.5 ms)
   P.force_shape(n, 256 + 2 * PROG_SHAPE_CH);
   CopyBatch<T> cp;
   cp.add(b2, b2, 3);
@@ -1051,8 +1060,8 @@ void Factor<T>::build_plan(int cap) {
   if (n2) {
     Blk<T> b2 = new_blk<T>(pv, n2, cap), x2 = new_blk<T>(pv, n2, cap);
     alloc_vectors(pv, cap, L, &b2, &x2);
-    emit_forward(P, 3, L);
-    // levels 2 and base: x2 = D2 b2, in row tiles
+    emit_forward(P, jd + 1, L);
+    // levels jd..2 and the base solve: x2 = D b2, in row tiles
     P.stage();
     BoxOp<T> op;
     op.m = n2;
@@ -1060,7 +1069,7 @@ void Factor<T>::build_plan(int cap) {
     op.ldsave = x2.ld;
     op.term(d2.p, d2.ld, n2, b2.p, b2.ld, ONE);
     P.add(op);
-    emit_backward(P, 3, L);
+    emit_backward(P, jd + 1, L);
   } else {
     alloc_vectors(pv, cap, L, nullptr, nullptr);
     emit_forward(P, 2, L);

@@ -167,11 +167,11 @@ class Factor {
   std::vector<std::vector<Blk<T>>> yt;  // phase-1 vectors of tall boxes (back-substitution)
   std::vector<std::vector<Blk<T>>> vk;  // y_k - P2M A^{-1} x0 of the ill boxes
   Blk<T> eye;                           // identity, rank x rank of the largest box
-  // the level-2 part of the sweep as one dense matrix (n2 x n2; n2 = 0: the
-  // staged form), level-2 box offsets in its stacked vectors
+  // the sweep at levels jd..2 and the base solve as one dense matrix (n2 x
+  // n2; n2 = 0: the staged form), level-jd box offsets in its stacked vectors
   static constexpr int kDenseTopMax = 6144;
   Blk<T> d2;
-  int n2 = 0;
+  int n2 = 0, jd = 2;
   std::vector<int> off2;
   void sweep_operators();
   void dense_top_alloc();

@@ -1057,7 +1057,11 @@ void Factor<T>::build_plan(int cap) {
   pv.reset();
   plan_cap = cap;
   Program<T>& P = *plan;
-  if (n2) {
+  // The dense top serves one-column sweeps only: applying the explicit
+  // matrix rounds differently from the staged factors, and with 16 columns
+  // the worst column then needed a fourth refinement step (718 -> 621 RHS
+  // solves/s at N = 147,456), more than the saved launches return there.
+  if (n2 && cap == 1) {
     Blk<T> b2 = new_blk<T>(pv, n2, cap), x2 = new_blk<T>(pv, n2, cap);
     alloc_vectors(pv, cap, L, &b2, &x2);
     emit_forward(P, jd + 1, L);

@@ -134,7 +134,7 @@ def profile_read(reset=True, max_kernels=64):
 
 STAT_NAMES = ["t_tables", "t_assemble", "t_factor", "factor_flops", "factor_bytes", "base_n", "max_rank",
               "solve_bytes_1rhs", "recompressed_boxes", "operator_bytes", "t_solve", "t_nca", "ill_boxes",
-              "sweep_tuned_time_ratio", "sweep_tuned_stages"]
+              "sweep_tuned_time_ratio", "sweep_tuned_stages", "dense_top_level", "dense_top_n"]
 
 
 class LSProblem:

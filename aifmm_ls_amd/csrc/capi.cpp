@@ -407,7 +407,7 @@ int aifmm_ls_device_bytes(aifmm_ls_problem* h, int64_t* bytes) {
 
 int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
   GUARD({
-    double v[15] = {0};
+    double v[17] = {0};
     Problem& P = *h->pb;
     v[0] = P.t_tables;
     v[1] = P.t_assemble;
@@ -420,6 +420,8 @@ int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
       v[6] = st->max_rank;
       v[8] = st->recompressions;
       v[12] = st->ill_boxes;
+      v[15] = st->dense_level;
+      v[16] = st->dense_n;
     }
     if (h->f64) v[7] = (double)h->f64->solve_bytes(1);
     if (h->f32) v[7] = (double)h->f32->solve_bytes(1);
@@ -434,7 +436,7 @@ int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
     };
     if (h->precision == 0 && h->f64) tuned(h->f64->sweep_tune());
     if (h->precision == 1 && h->f32) tuned(h->f32->sweep_tune());
-    for (int i = 0; i < n && i < 15; ++i) out[i] = v[i];
+    for (int i = 0; i < n && i < 17; ++i) out[i] = v[i];
   })
 }
 

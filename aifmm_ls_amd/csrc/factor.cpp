@@ -1009,6 +1009,7 @@ template <class T>
 void Factor<T>::dense_top_alloc() {
   d2 = Blk<T>{};
   n2 = 0;
+  st.dense_level = st.dense_n = 0;
   static const int top = [] {
     const char* e = getenv("AIFMM_DENSE_TOP");
     return e && e[0] == '0' ? 0 : e && e[0] == '2' ? 2 : 3;
@@ -1023,6 +1024,8 @@ void Factor<T>::dense_top_alloc() {
     off2 = off;
     n2 = off[nb];
     d2 = new_blk<T>(ar, n2, n2);
+    st.dense_level = jd;
+    st.dense_n = n2;
     return;
   }
 }

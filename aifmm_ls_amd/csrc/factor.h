@@ -19,6 +19,8 @@ struct FactorStats {
   size_t bytes = 0;        // factor storage
   int recompressions = 0;  // boxes re-based after fill-in
   int ill_boxes = 0;       // boxes whose sweeps keep the reference's association
+  int dense_level = 0;     // top of the one-column sweep as one dense product: its level (0: staged)
+  int dense_n = 0;         // and its size
 };
 
 // Factorisation of a square block for repeated solves.  complex128: column-

@@ -100,7 +100,9 @@ int aifmm_ls_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y);
  * (badly conditioned L2M_f), and the latest launch tuning of the active
  * factorisation's sweep (see aifmm_ls_set_tuning): run time with the tuned
  * workgroup sizes over the default's (1 when the default was kept; 0 before
- * the first solve), stages whose size was changed. */
+ * the first solve), stages whose size was changed; then the top of the
+ * one-column sweep applied as one dense matrix: its tree level (0 = every
+ * level staged) and its size. */
 int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n);
 /* Launch tuning (no reference counterpart): the first solve at each RHS count
  * times the sweep's and the FP64 operator's stages at 256, 512 and 1024

@@ -52,6 +52,9 @@ def lib():
         L.aifmm_ls_skeleton.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)]
         L.aifmm_ls_factor.argtypes = [P, ctypes.c_int]
+        L.aifmm_ls_set_pivot_eps.argtypes = [P, ctypes.c_double, ctypes.c_double]
+        L.aifmm_ls_factor_trace.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_int)]
         L.aifmm_ls_solve.argtypes = [P, ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_int, dp]
         L.aifmm_ls_set_rhs.argtypes = [P, ctypes.c_int, dp]
         L.aifmm_ls_solve_resident.argtypes = [P, ctypes.c_double, ctypes.c_int, dp]
@@ -134,7 +137,8 @@ def profile_read(reset=True, max_kernels=64):
 
 STAT_NAMES = ["t_tables", "t_assemble", "t_factor", "factor_flops", "factor_bytes", "base_n", "max_rank",
               "solve_bytes_1rhs", "recompressed_boxes", "operator_bytes", "t_solve", "t_nca", "ill_boxes",
-              "sweep_tuned_time_ratio", "sweep_tuned_stages", "dense_top_level", "dense_top_n"]
+              "sweep_tuned_time_ratio", "sweep_tuned_stages", "dense_top_level", "dense_top_n",
+              "base_rdiag_ratio", "zeroed_pivot_blocks"]
 
 
 class LSProblem:
@@ -216,6 +220,24 @@ class LSProblem:
 
     def factor(self, precision=PREC_C128):
         _chk(lib().aifmm_ls_factor(self.h, precision))
+
+    def set_pivot_eps(self, l2mf_eps=1e-12, base_eps=0.0):
+        """Zero-pivot epsilons of the next complex128 factorisation: the L2M_f
+        pivot blocks (default 1e-12) and the level-2 base system (default 0 =
+        Eigen's rule, the reference's)."""
+        _chk(lib().aifmm_ls_set_pivot_eps(self.h, float(l2mf_eps), float(base_eps)))
+
+    def factor_trace(self, which=0):
+        """which 0: recompressions of the factorisation (rows: level, colour
+        step, box, row-stack rank, column-stack rank, new rank); which 1:
+        L2M_f factorisations with zeroed pivots (rows: level, box, nonzero
+        pivots, size; complex128 only)."""
+        n = ctypes.c_int()
+        ip = ctypes.POINTER(ctypes.c_int)
+        _chk(lib().aifmm_ls_factor_trace(self.h, which, None, 0, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1), np.int32)
+        _chk(lib().aifmm_ls_factor_trace(self.h, which, out.ctypes.data_as(ip), n.value, ctypes.byref(n)))
+        return out[:n.value].reshape(-1, 6 if which == 0 else 4).copy()
 
     def solve(self, b, tol=1e-6, max_refine=0):
         b = np.asarray(b, np.complex128)

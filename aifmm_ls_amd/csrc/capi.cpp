@@ -36,6 +36,7 @@ struct aifmm_ls_problem {
   std::unique_ptr<BatchPlan> bplan;
   int refine_method = 0;
   int precision = -1;
+  double l2mf_eps = 1e-12, base_eps = 0;  // zero-pivot epsilons of the next factorisation
   Arena sc{size_t(16) << 20};
   Arena io{size_t(64) << 20};
   double2* rhs = nullptr;
@@ -179,16 +180,48 @@ int aifmm_ls_factor(aifmm_ls_problem* h, int precision) {
   GUARD({
     if (precision != 0 && precision != 1) throw std::runtime_error("precision must be 0 (c128) or 1 (c64)");
     h->pb->assemble();
+    // no usable factor until the new one is complete: a failed factorisation
+    // leaves the handle unfactored (solves then report "factor first")
+    h->precision = -1;
     h->f64.reset();
     h->f32.reset();
     if (precision == 0) {
-      h->f64.reset(new Factor<double2>(*h->pb));
-      h->f64->factor();
+      std::unique_ptr<Factor<double2>> f(new Factor<double2>(*h->pb));
+      f->l2mf_eps = h->l2mf_eps;
+      f->base_eps = h->base_eps;
+      f->factor();
+      h->f64 = std::move(f);
     } else {
-      h->f32.reset(new Factor<float2>(*h->pb));
-      h->f32->factor();
+      std::unique_ptr<Factor<float2>> f(new Factor<float2>(*h->pb));
+      f->l2mf_eps = h->l2mf_eps;
+      f->base_eps = h->base_eps;
+      f->factor();
+      h->f32 = std::move(f);
     }
     h->precision = precision;
+  })
+}
+
+int aifmm_ls_set_pivot_eps(aifmm_ls_problem* h, double l2mf_eps, double base_eps) {
+  GUARD({
+    if (!h) throw std::runtime_error("null handle");
+    if (!(l2mf_eps >= 0 && l2mf_eps < 1) || !(base_eps >= 0 && base_eps < 1))
+      throw std::runtime_error("zero-pivot epsilons must lie in [0, 1)");
+    h->l2mf_eps = l2mf_eps;
+    h->base_eps = base_eps;
+  })
+}
+
+int aifmm_ls_factor_trace(aifmm_ls_problem* h, int which, int* out, int cap, int* n) {
+  GUARD({
+    if (!h || !n) throw std::runtime_error("null argument");
+    if (h->precision < 0) throw std::runtime_error("factor first");
+    const FactorStats& st = h->precision == 0 ? h->f64->st : h->f32->st;
+    if (which != 0 && which != 1) throw std::runtime_error("which must be 0 (recompressions) or 1 (zeroed pivots)");
+    const std::vector<int>& v = which == 0 ? st.rc_trace : st.zero_piv;
+    *n = (int)v.size();
+    if (out)
+      for (int i = 0; i < (int)v.size() && i < cap; ++i) out[i] = v[i];
   })
 }
 
@@ -407,7 +440,7 @@ int aifmm_ls_device_bytes(aifmm_ls_problem* h, int64_t* bytes) {
 
 int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
   GUARD({
-    double v[17] = {0};
+    double v[19] = {0};
     Problem& P = *h->pb;
     v[0] = P.t_tables;
     v[1] = P.t_assemble;
@@ -422,6 +455,8 @@ int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
       v[12] = st->ill_boxes;
       v[15] = st->dense_level;
       v[16] = st->dense_n;
+      v[17] = st->base_rdiag_ratio;
+      v[18] = (double)(st->zero_piv.size() / 4);
     }
     if (h->f64) v[7] = (double)h->f64->solve_bytes(1);
     if (h->f32) v[7] = (double)h->f32->solve_bytes(1);
@@ -436,7 +471,7 @@ int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
     };
     if (h->precision == 0 && h->f64) tuned(h->f64->sweep_tune());
     if (h->precision == 1 && h->f32) tuned(h->f32->sweep_tune());
-    for (int i = 0; i < n && i < 17; ++i) out[i] = v[i];
+    for (int i = 0; i < n && i < 19; ++i) out[i] = v[i];
   })
 }
 

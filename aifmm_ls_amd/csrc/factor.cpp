@@ -30,14 +30,14 @@ static double now_s() {
 }
 static const double2 ONE = {1.0, 0.0}, MONE = {-1.0, 0.0}, ZERO = {0.0, 0.0};
 
-// Zero-pivot epsilon of the L2M_f factorisations (complex128).  Eigen's
-// ColPivHouseholderQR counts a pivot as nonzero down to 2.2e-16 of the largest
-// column norm; an L2M_f whose trailing pivot sits below 1e-12 of its leading
-// one (a low-contrast box whose two skeleton directions coincide to rounding)
-// has no correct digit left in that pivot, and its inverse would carry that
-// rounding into every fill-in by up to 1e15.  Such pivots are treated as zero:
-// the solve is Eigen's basic solution, as for an exactly singular block.
-static constexpr double kPivEps = 1e-12;
+// Zero-pivot epsilon of the L2M_f factorisations (complex128, Factor::l2mf_eps,
+// 1e-12 by default).  Eigen's ColPivHouseholderQR counts a pivot as nonzero
+// down to 2.2e-16 of the largest column norm; an L2M_f whose trailing pivot
+// sits below 1e-12 of its leading one (a low-contrast box whose two skeleton
+// directions coincide to rounding) has no correct digit left in that pivot,
+// and its inverse would carry that rounding into every fill-in by up to 1e15.
+// Such pivots are treated as zero: the solve is Eigen's basic solution, as for
+// an exactly singular block.
 
 template <class T>
 Factor<T>::Factor(Problem& p) : pb(p), L(p.L), R(p.R) {
@@ -190,7 +190,7 @@ void Factor<T>::eliminate_color(int j, int c) {
     int r = K.r;
     X[t] = new_blk<T>(ar, r, ncols[t]);
     if (!r) continue;
-    dn.factor(ar, sc, K.fg, K.G.sub(0, 0, r, r), kPivEps);
+    dn.factor(ar, sc, K.fg, K.G.sub(0, 0, r, r), l2mf_eps);
     cp.add(X[t], X[t].sub(0, 0, r, r), 3, MONE);
     if (ncols[t] > r) cp.add(K.G.sub(0, r, r, ncols[t] - r), X[t].sub(0, r, r, ncols[t] - r));
     fl += 16.0 * r * r * r / 3.0 + 16.0 * r * r * ncols[t];
@@ -295,7 +295,7 @@ void Factor<T>::eliminate_color(int j, int c) {
 // basis = CPQR([P2M_u^H | column fill-ins^H]); rank = min of the two
 // thresholded ranks (the reference's equalisation).
 template <class T>
-void Factor<T>::recompress(int j) {
+void Factor<T>::recompress(int j, int c) {
   using Rl = typename CplxOf<T>::real;
   auto& Bx = B[j];
   const auto& LS = pb.lists[j];
@@ -441,6 +441,11 @@ void Factor<T>::recompress(int j) {
   HIP_OK(hipMemcpyAsync(rk.data(), dr, 8 * nt, hipMemcpyDeviceToHost, pb.stream));
   HIP_OK(hipStreamSynchronize(pb.stream));
   std::vector<int> rnew(nb, -1);
+  for (int i = 0; i < nt; ++i) {
+    const int u = tl[i];
+    const int r = Bx[u].m ? std::min(rk[2 * i], rk[2 * i + 1]) : 0;
+    st.rc_trace.insert(st.rc_trace.end(), {j, c, u, rk[2 * i], rk[2 * i + 1], r});
+  }
   static const bool dbg = getenv("AIFMM_DEBUG") != nullptr;
   if (dbg) {
     fprintf(stderr, "[gpu] level %d: %zu candidates, %zu fills, %d touched:", j, cand.size(), fills.size(), nt);
@@ -607,9 +612,23 @@ void Factor<T>::base_factor() {
     }
   cp.run(sc, pb.stream);
   DenseOps<T> dn;
-  dn.factor(ar, sc, fb, base);
+  dn.factor(ar, sc, fb, base, base_eps);
   dn.run_factor(sc, pb.stream);
   st.flops += 16.0 * n * (double)n * n / 3.0;
+  {
+    // smallest over largest |diagonal| of the factor (reported in the stats)
+    Blk<double2> f = DenseOps<T>::fp64_factor(fb);
+    Blk<double2> dd = new_blk<double2>(sc, n, 1);
+    CopyBatch<double2> dg;
+    dg.add(Blk<double2>{f.p, 1, n, f.ld + 1}, dd, 2);
+    dg.run(sc, pb.stream);
+    std::vector<double2> hd(n);
+    HIP_OK(hipMemcpyAsync(hd.data(), dd.p, n * sizeof(double2), hipMemcpyDeviceToHost, pb.stream));
+    HIP_OK(hipStreamSynchronize(pb.stream));
+    double mx = 0, mn = HUGE_VAL;
+    for (auto& v : hd) mx = std::max(mx, std::hypot(v.x, v.y)), mn = std::min(mn, std::hypot(v.x, v.y));
+    st.base_rdiag_ratio = mx > 0 ? mn / mx : 0;
+  }
   sc.reset();  // scratch reuse is ordered on the stream
 }
 
@@ -625,10 +644,28 @@ void Factor<T>::factor() {
     for (int c = 0; c < 9; ++c) {
       if (colors[j][c].empty()) continue;
       eliminate_color(j, c);
-      recompress(j);
+      recompress(j, c);
     }
   }
   base_factor();
+  if constexpr (std::is_same<T, double2>::value) {
+    // nonzero-pivot counts of every L2M_f factorisation (one gather, one readback)
+    std::vector<const int*> src;
+    std::vector<std::array<int, 3>> who;
+    for (int j = 2; j <= L; ++j)
+      for (int k = 0; k < (int)B[j].size(); ++k)
+        if (B[j][k].m && B[j][k].r) src.push_back(B[j][k].fg.q.nz), who.push_back({j, k, B[j][k].r});
+    if (!src.empty()) {
+      auto* ds = upload(sc, src, pb.stream);
+      int* dn = sc.get<int>(src.size());
+      launch_gather_i32(ds, dn, (int)src.size(), pb.stream);
+      std::vector<int> nz(src.size());
+      HIP_OK(hipMemcpyAsync(nz.data(), dn, nz.size() * 4, hipMemcpyDeviceToHost, pb.stream));
+      HIP_OK(hipStreamSynchronize(pb.stream));
+      for (size_t i = 0; i < nz.size(); ++i)
+        if (nz[i] < who[i][2]) st.zero_piv.insert(st.zero_piv.end(), {who[i][0], who[i][1], nz[i], who[i][2]});
+    }
+  }
   sweep_operators();
   dense_top_alloc();
   build_plan(1);

@@ -21,6 +21,13 @@ struct FactorStats {
   int ill_boxes = 0;       // boxes whose sweeps keep the reference's association
   int dense_level = 0;     // top of the one-column sweep as one dense product: its level (0: staged)
   int dense_n = 0;         // and its size
+  // one record per recompressed box, in order: level, colour step, box,
+  // row-stack rank, column-stack rank, new rank
+  std::vector<int> rc_trace;
+  // complex128 only: every L2M_f factorisation whose nonzero-pivot count fell
+  // below its size: level, box, nonzero pivots, size
+  std::vector<int> zero_piv;
+  double base_rdiag_ratio = 0;  // smallest over largest |R_ii| (complex128) or |U_ii| (complex64) of the base factor
 };
 
 // Factorisation of a square block for repeated solves.  complex128: column-
@@ -146,6 +153,11 @@ class Factor {
     Blk<T> PA;  // -P2M A^{-1}           (r x m)
   };
   static constexpr double kIllCond = 1e4;
+  // Zero-pivot epsilons of the complex128 factorisations (Eigen's
+  // ColPivHouseholderQR "nonzero pivot" test with this epsilon in place of the
+  // FP64 one; 0 = the FP64 epsilon, the reference's rule): the L2M_f pivot
+  // blocks, and the level-2 base system.  Set before factor().
+  double l2mf_eps = 1e-12, base_eps = 0;
   std::vector<std::vector<FBox>> B;
   std::vector<std::vector<std::vector<int>>> colors;
 
@@ -187,7 +199,7 @@ class Factor {
   void init_level(int j);
   void nonleaf_p2p(int j);
   void eliminate_color(int j, int c);
-  void recompress(int j);
+  void recompress(int j, int c);
   void base_factor();
   Blk<T> nb_blk(int j, int row, int col);  // coupling block (row eq, col unknown)
 };

@@ -268,7 +268,14 @@ __global__ __launch_bounds__(NT) void k_lu_solve(const SolveDesc<T>* __restrict_
     __syncthreads();
   }
   for (int k = n - 1; k >= 0; --k) {
-    if (threadIdx.x < nc) B[k + (size_t)lb * threadIdx.x] = cdiv(B[k + (size_t)lb * threadIdx.x], A[k + (size_t)la * k]);
+    // an exactly zero pivot (k_lu skips its column) gives that unknown 0, as
+    // the basic solution of a singular block, instead of Inf / NaN
+    const T dk = A[k + (size_t)la * k];
+    const bool nzp = dk.x != 0 || dk.y != 0;
+    if (threadIdx.x < nc) {
+      T& bk = B[k + (size_t)lb * threadIdx.x];
+      bk = nzp ? cdiv(bk, dk) : T{0, 0};
+    }
     __syncthreads();
     for (int e = threadIdx.x; e < k * nc; e += NT) {
       int i = e % k, c = e / k;

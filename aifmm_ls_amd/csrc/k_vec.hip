@@ -248,6 +248,15 @@ void launch_resid(const double2* b, const double2* ax, double2* r, int64_t n, in
   HIP_OK(hipGetLastError());
 }
 
+__global__ __launch_bounds__(VNT) void k_gather_i32(const int* const* __restrict__ src, int* __restrict__ dst, int n) {
+  const int i = blockIdx.x * VNT + threadIdx.x;
+  if (i < n) dst[i] = *src[i];
+}
+
+void launch_gather_i32(const int* const* src, int* dst, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_gather_i32, dim3((n + VNT - 1) / VNT), dim3(VNT), 0, s, src, dst, n);
+}
+
 void launch_axpy(double2* y, const double2* x, int64_t n, hipStream_t s) {
   if (!n) return;
   int64_t b = (n + VNT - 1) / VNT;

@@ -62,6 +62,21 @@ int aifmm_ls_skeleton(aifmm_ls_problem* h, int level, int box, int which, int64_
 /* FMM2DTree::eliminate_phase_efficient (FMM2DTree_find_error.hpp:2407-2451), here the
  * level-synchronous colour schedule.  precision: 0 = complex128, 1 = complex64. */
 int aifmm_ls_factor(aifmm_ls_problem* h, int precision);
+/* Zero-pivot epsilons of the next complex128 factorisation, in Eigen's
+ * ColPivHouseholderQR "nonzero pivot" test (a pivot counts as zero below
+ * eps x the largest column norm; the reference uses the FP64 epsilon,
+ * FMM2DTree_find_error.hpp:2092 for L2M_f and :2477 for the base system):
+ * l2mf_eps for the L2M_f pivot blocks (default 1e-12), base_eps for the
+ * level-2 base system (default 0 = the FP64 epsilon).  Both in [0, 1). */
+int aifmm_ls_set_pivot_eps(aifmm_ls_problem* h, double l2mf_eps, double base_eps);
+/* Decisions of the last factorisation, for comparison with the reference's
+ * (FMM2DTree_find_error.hpp:1424-1699 recompression ranks, :2092 pivots).
+ * which 0: one record per recompressed box, 6 ints: level, colour step, box,
+ * rank of [L2P | row fill-ins], rank of [P2M^H | column fill-ins^H], new rank.
+ * which 1 (complex128): one record per L2M_f factorisation with zeroed
+ * pivots, 4 ints: level, box, nonzero pivots, size.  *n gets the number of
+ * ints; up to cap are written to out (out may be null). */
+int aifmm_ls_factor_trace(aifmm_ls_problem* h, int which, int* out, int cap, int* n);
 /* forward sweep + base solve + back_substitution_phase (FMM2DTree_find_error.hpp:2487-2532)
  * for nrhs right-hand sides (N x nrhs).  stats (>= 8 doubles, optional): solve seconds,
  * refinement steps, final relative residual max_c |b_c - A_H x_c| / |b_c|, ...
@@ -102,7 +117,8 @@ int aifmm_ls_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y);
  * workgroup sizes over the default's (1 when the default was kept; 0 before
  * the first solve), stages whose size was changed; then the top of the
  * one-column sweep applied as one dense matrix: its tree level (0 = every
- * level staged) and its size. */
+ * level staged) and its size; the base factor's smallest over largest
+ * |diagonal|; the number of L2M_f blocks with zeroed pivots (complex128). */
 int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n);
 /* Launch tuning (no reference counterpart): the first solve at each RHS count
  * times the sweep's and the FP64 operator's stages at 256, 512 and 1024

@@ -14,6 +14,8 @@ struct Handle {
   Tree T;
   std::vector<cd> b, x;
   double t_tables = 0, t_assemble = 0, t_factor = 0, t_solve = 0;
+  double piv_eps = 0;       // Tree::piv_eps of every later solve
+  double base_piv_eps = 0;  // Tree::base_piv_eps of every later solve
   bool assembled = false;
 };
 double now() {
@@ -50,6 +52,30 @@ void orc_set_perturb(void* p, double eps, unsigned long long seed) {
   Handle* h = (Handle*)p;
   h->D.perturb = eps;
   h->D.perturb_seed = seed;
+}
+
+// Zero-pivot epsilon of the L2M_f factorisations in later solves (Tree::piv_eps;
+// 0 = Eigen's rule, the reference's).
+void orc_set_piv_eps(void* p, double eps, double base_eps) {
+  ((Handle*)p)->piv_eps = eps;
+  ((Handle*)p)->base_piv_eps = base_eps;
+}
+
+// L2M_f factorisations of the last solve with zeroed pivots (Tree::zero_piv,
+// 4 ints each); writes up to cap ints, returns the number of ints.
+int orc_zero_pivots(void* p, int* out, int cap) {
+  const std::vector<int>& t = ((Handle*)p)->T.zero_piv;
+  for (int i = 0; i < (int)t.size() && i < cap; ++i) out[i] = t[i];
+  return (int)t.size();
+}
+
+// Recompression records of the last colour-schedule solve (Tree::rc_trace,
+// 6 ints each); writes up to cap ints, returns the number of ints.
+int orc_recompress_trace(void* p, int* out, int cap) {
+  Handle* h = (Handle*)p;
+  const std::vector<int>& t = h->T.rc_trace;
+  for (int i = 0; i < (int)t.size() && i < cap; ++i) out[i] = t[i];
+  return (int)t.size();
 }
 
 long long orc_n(void* p) { return ((Handle*)p)->D.N; }
@@ -108,7 +134,7 @@ void orc_rhs(void* p, double theta, double* b) {
 // Full hierarchical solve: NCA, M2L/P2P assembly, elimination with the given
 // schedule, base solve, back-substitution.  stats (optional, >= 8 doubles):
 // t_tables, t_assemble, t_factor, t_solve, sum of leaf ranks, max rank,
-// base-system size, 0.
+// base-system size, smallest over largest |R_ii| of the base factorisation.
 int orc_solve(void* p, const double* b, double* x, int schedule, double* stats) {
   Handle* h = (Handle*)p;
   int64_t N = h->D.N;
@@ -116,6 +142,8 @@ int orc_solve(void* p, const double* b, double* x, int schedule, double* stats) 
   for (int64_t i = 0; i < N; ++i) bb[i] = cd(b[2 * i], b[2 * i + 1]);
   Tree& T = h->T;
   T = Tree();
+  T.piv_eps = h->piv_eps;
+  T.base_piv_eps = h->base_piv_eps;
   double t0 = now();
   T.create(&h->D, h->D.P.tol_pow);
   T.assign_rhs(bb.data());
@@ -142,7 +170,7 @@ int orc_solve(void* p, const double* b, double* x, int schedule, double* stats) 
     int nb = 0;
     for (auto& B : T.T[2]) nb += B.NumMultipoles;
     stats[6] = nb;
-    stats[7] = 0;
+    stats[7] = T.base_rdiag_ratio;
   }
   return 0;
 }

@@ -131,7 +131,7 @@ Mat LU::solve(const Mat& B) const {
 }
 
 // ------------------------------------------------------------------ CPQR
-void CPQR::compute(const Mat& A) {
+void CPQR::compute(const Mat& A, double nz_eps) {
   m = A.r;
   n = A.c;
   qr = A;
@@ -147,7 +147,7 @@ void CPQR::compute(const Mat& A) {
     upd[j] = dir[j] = std::sqrt(s);
     maxnorm = std::max(maxnorm, upd[j]);
   }
-  double eps = DBL_EPSILON;
+  double eps = nz_eps > 0 ? nz_eps : DBL_EPSILON;
   double th_helper = (maxnorm * eps) * (maxnorm * eps) / (m > 0 ? m : 1);
   double norm_downdate_threshold = std::sqrt(eps);
   nonzero = size;

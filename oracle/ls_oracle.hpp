@@ -62,7 +62,8 @@ struct CPQR {
   std::vector<int> perm;  // perm[k] = original column placed at position k
   double maxpivot = 0;
   int nonzero = 0;
-  void compute(const Mat& A);
+  // nz_eps: epsilon of Eigen's "nonzero pivot" test (0: the FP64 epsilon)
+  void compute(const Mat& A, double nz_eps = 0);
   int rank(double threshold) const;
   Mat solve(const Mat& B) const;
   Mat Q(int k) const;  // first k columns of the unitary factor
@@ -150,6 +151,22 @@ struct Tree {
   int L = 0;
   int tol_pow = 8;
   double rrqr_thr = 1e-8;
+  // Zero-pivot epsilon of the L2M_f factorisations (eliminate_z and the
+  // back-substitution's Atemp).  0 keeps Eigen's rule (the FP64 epsilon), as
+  // the reference; a positive value counts pivots below nz_eps times the
+  // largest column norm as zero, the rule of the GPU's complex128 factor.
+  double piv_eps = 0;
+  // The same rule for the level-2 base system (0: Eigen's rule, the reference's).
+  double base_piv_eps = 0;
+  // Every L2M_f factorisation of the elimination whose nonzero-pivot count
+  // fell below its size: level, box, nonzero pivots, size.
+  std::vector<int> zero_piv;
+  // smallest over largest |R_ii| of the base system's factorisation (0 if empty)
+  double base_rdiag_ratio = 0;
+  // Colour schedule only: one record per recompressed box, in order:
+  // level, colour step, box, row-stack rank, column-stack rank, new rank.
+  std::vector<int> rc_trace;
+  int cur_color = -1;
   std::vector<std::vector<Box>> T;
   std::vector<std::vector<int>> order;  // elimination order per level
   std::vector<std::pair<int, int>> P2Plist, P2L_M2Plist;

@@ -577,7 +577,10 @@ void Tree::eliminate_z(int j, int k) {
   Box& K = T[j][k];
   bool have = K.L2M_f.size() > 0;
   CPQR qr;
-  if (have) qr.compute(K.L2M_f);
+  if (have) {
+    qr.compute(K.L2M_f, piv_eps);
+    if (qr.nonzero < std::min(qr.m, qr.n)) zero_piv.insert(zero_piv.end(), {j, k, qr.nonzero, std::min(qr.m, qr.n)});
+  }
   int r = K.NumMultipoles;
   if (have) {
     K.M2L[k] = qr.solve(neg(eye(r)));
@@ -920,6 +923,7 @@ void Tree::colored_recompress(int j) {
     // does (FMM2DTree_find_error.hpp:1445-1460, 1505-1512)
     int r = std::min(q1.rank(rrqr_thr), q2.rank(rrqr_thr));
     rnew[u] = r;
+    rc_trace.insert(rc_trace.end(), {j, cur_color, u, q1.rank(rrqr_thr), q2.rank(rrqr_thr), r});
     if (getenv("AIFMM_DEBUG")) dbg_line += " " + std::to_string(u) + "(" + std::to_string(q1.rank(rrqr_thr)) + "," +
                                             std::to_string(q2.rank(rrqr_thr)) + "|r" + std::to_string(U.L2P.c) + ")";
     QR_[u] = q1.Q(r);
@@ -1027,6 +1031,7 @@ void Tree::eliminate(Schedule s) {
         }
         P2Plist.clear();
         P2L_M2Plist.clear();
+        cur_color = c;
         colored_recompress(j);
       }
       continue;
@@ -1113,7 +1118,10 @@ void Tree::base_solve() {
     ri += T[j][r].NumLocals;
   }
   CPQR qr;
-  qr.compute(A);
+  qr.compute(A, base_piv_eps);
+  double mx = 0, mn = HUGE_VAL;
+  for (int i = 0; i < std::min(qr.m, qr.n); ++i) mx = std::max(mx, std::abs(qr.qr(i, i))), mn = std::min(mn, std::abs(qr.qr(i, i)));
+  base_rdiag_ratio = mx > 0 ? mn / mx : 0;
   Mat y = qr.solve(b);
   int idx = 0;
   for (int k = 0; k < nb; ++k) {
@@ -1146,7 +1154,7 @@ void Tree::back_substitution() {
         Mat Atemp = neg(mul(K.P2M, qr.solve(K.L2P)));
         if (Atemp.size() != 0) {
           CPQR q2;
-          q2.compute(Atemp);
+          q2.compute(Atemp, piv_eps);
           K.locals = q2.solve(rhs2);
         } else {
           K.locals = Mat(0, 1);

@@ -36,6 +36,11 @@ def lib():
         L.orc_create.argtypes = [ip, ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_int]
         L.orc_destroy.argtypes = [P]
         L.orc_set_perturb.argtypes = [P, ctypes.c_double, ctypes.c_ulonglong]
+        L.orc_set_piv_eps.argtypes = [P, ctypes.c_double, ctypes.c_double]
+        L.orc_zero_pivots.restype = ctypes.c_int
+        L.orc_zero_pivots.argtypes = [P, ip, ctypes.c_int]
+        L.orc_recompress_trace.restype = ctypes.c_int
+        L.orc_recompress_trace.argtypes = [P, ip, ctypes.c_int]
         L.orc_n.restype = ctypes.c_longlong
         L.orc_n.argtypes = [P]
         L.orc_points.argtypes = [P, dp, dp]
@@ -70,7 +75,8 @@ class Problem:
     """One discretised LS problem (tables, points, contrast) in the oracle."""
 
     def __init__(self, n_levels, kappa, tol_pow=8, qchoice=0, n_cheb=6, degree=6, L=1.0,
-                 bumps=None, quantize=True, exact_bessel=False, perturb=0.0, perturb_seed=0):
+                 bumps=None, quantize=True, exact_bessel=False, perturb=0.0, perturb_seed=0, piv_eps=0.0,
+                 base_piv_eps=0.0):
         ip = np.array([n_cheb, degree, n_levels, tol_pow, qchoice, int(quantize), int(exact_bessel)], np.int32)
         nb = 0
         self._bumps = None
@@ -86,6 +92,26 @@ class Problem:
         if perturb:
             # sensitivity probe: far-field entries scaled by 1 + perturb * u(i, j)
             lib().orc_set_perturb(self.h, float(perturb), int(perturb_seed))
+        if piv_eps or base_piv_eps:
+            # zero-pivot rule of the L2M_f factorisations and of the base system
+            # (0: Eigen's, the reference's)
+            lib().orc_set_piv_eps(self.h, float(piv_eps), float(base_piv_eps))
+
+    def zero_pivots(self):
+        """L2M_f factorisations of the last solve whose nonzero-pivot count fell
+        below their size, one row each: level, box, nonzero pivots, size."""
+        n = lib().orc_zero_pivots(self.h, np.zeros(1, np.int32), 0)
+        out = np.zeros(max(n, 1), np.int32)
+        lib().orc_zero_pivots(self.h, out, n)
+        return out[:n].reshape(-1, 4).copy()
+
+    def recompress_trace(self):
+        """Recompressions of the last colour-schedule solve, one row each:
+        level, colour step, box, row-stack rank, column-stack rank, new rank."""
+        n = lib().orc_recompress_trace(self.h, np.zeros(1, np.int32), 0)
+        out = np.zeros(max(n, 1), np.int32)
+        lib().orc_recompress_trace(self.h, out, n)
+        return out[:n].reshape(-1, 6).copy()
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -61,6 +61,7 @@ def lib():
         L.aifmm_ls_get_solution.argtypes = [P, dp]
         L.aifmm_ls_stats.argtypes = [P, dp, ctypes.c_int]
         L.aifmm_ls_matvec.argtypes = [P, ctypes.c_int, dp, dp]
+        L.aifmm_ls_dense_matvec.argtypes = [P, ctypes.c_int, dp, dp]
         ip64 = ctypes.POINTER(ctypes.c_int64)
         ip = ctypes.POINTER(ctypes.c_int)
         L.aifmm_ls_aca.argtypes = [P, ctypes.c_int, ctypes.c_int, ip64, ctypes.c_int, ip64, ip, ip, dp, dp, ip]
@@ -280,6 +281,17 @@ class LSProblem:
         y = y.view(np.complex128).reshape(nrhs, self.N).T
         return y[:, 0] if x.ndim == 1 else y
 
+    def dense_matvec(self, x):
+        """y = A x with the discretised operator itself (every entry as
+        getMatrixEntry), as the generator's dense rhs = A x."""
+        x = np.asarray(x, np.complex128)
+        nrhs = 1 if x.ndim == 1 else x.shape[1]
+        xx = np.ascontiguousarray(x.reshape(self.N, nrhs).T).view(np.float64)
+        y = np.zeros(2 * self.N * nrhs)
+        _chk(lib().aifmm_ls_dense_matvec(self.h, nrhs, _dp(xx), _dp(y)))
+        y = y.view(np.complex128).reshape(nrhs, self.N).T
+        return y[:, 0] if x.ndim == 1 else y
+
     def aca(self, rows, cols, tol_pow):
         rows = np.ascontiguousarray(rows, np.int64)
         cols = np.ascontiguousarray(cols, np.int64)
@@ -403,7 +415,7 @@ def solve_batch(problems, rhs, tol=1e-9, max_refine=30):
 
 
 SERVER_STAT_NAMES = ["resident_problems", "resident_bytes", "budget", "evictions", "refactorisations", "completed",
-                     "batches", "merged_requests", "peak_running", "queued"]
+                     "batches", "merged_requests", "peak_running", "queued", "cross_problem_batches"]
 
 
 class Server:
@@ -453,13 +465,13 @@ class Server:
         return t.value
 
     def wait(self, ticket):
-        info = np.zeros(8)
+        info = np.zeros(9)
         rc = lib().aifmm_ls_server_wait(self.s, int(ticket), _dp(info))
         out = self._out.pop(ticket, None)
         if rc != 0:
             raise AifmmError(lib().aifmm_ls_last_error().decode())
         d = dict(latency=info[1], queue=info[2], run=info[3], steps=int(info[4]), relres=info[5],
-                 batch_rhs=int(info[6]), refactored=bool(info[7]))
+                 batch_rhs=int(info[6]), refactored=bool(info[7]), batch_problems=int(info[8]))
         if out is None:
             return None, d
         x, N, nrhs, vec = out

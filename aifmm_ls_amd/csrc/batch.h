@@ -41,11 +41,12 @@ Blk<T> new_blk(Arena& a, int r, int c) {
   return b;
 }
 
-// Pinned staging for descriptor uploads: the host copies a batch's
-// descriptors into a pinned ring and queues an asynchronous copy to the
-// device, so it can build the next batch while the GPU runs.  A region of the
-// ring is reused only after the device has been synchronised (on wrap-around).
-void* stage_pinned(const void* src, size_t bytes, hipStream_t s);
+// Host <-> device copies through a per-thread ring of pinned memory, ordered
+// on stream s.  h2d returns once the source has been copied into the ring
+// (the source may be released then); d2h returns when the data is in dst and
+// synchronises stream s only.  Neither waits for other streams.
+void h2d(void* dst, const void* src, size_t bytes, hipStream_t s);
+void d2h(void* dst, const void* src, size_t bytes, hipStream_t s);
 
 // Upload a host vector into a transient device buffer in the given scratch
 // arena (stream-ordered; the host vector may be released on return).
@@ -54,8 +55,7 @@ D* upload(Arena& a, const std::vector<D>& v, hipStream_t s) {
   if (v.empty()) return nullptr;
   const size_t bytes = v.size() * sizeof(D);
   D* d = (D*)a.alloc(bytes);
-  void* h = stage_pinned(v.data(), bytes, s);
-  HIP_OK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s));
+  h2d(d, v.data(), bytes, s);
   return d;
 }
 

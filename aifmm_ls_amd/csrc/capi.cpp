@@ -2,6 +2,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <string>
 #include <vector>
@@ -51,8 +52,23 @@ namespace aifmm {
 void set_last_error(const std::string& msg) { g_err = msg; }
 }  // namespace aifmm
 
+extern "C" {
+static int solve_batch_impl(aifmm_ls_problem* const* hs, int n, const double* const* b, double* const* x, double tol,
+                            int max_refine, double* relres, double* stats, bool tune);
+}
+
+namespace aifmm {
+// The batch solve without launch tuning of its merged sequences (serving loop:
+// its batches change members from call to call).
+int solve_batch_untuned(aifmm_ls_problem* const* hs, int n, const double* const* b, double* const* x, double tol,
+                        int max_refine, double* relres, double* stats) {
+  return solve_batch_impl(hs, n, b, x, tol, max_refine, relres, stats, false);
+}
+}  // namespace aifmm
+
 #define GUARD(...)                          \
   try {                                     \
+    aifmm::ApiScope api_scope_;             \
     __VA_ARGS__;                            \
     return 0;                               \
   } catch (const std::exception& e) {      \
@@ -101,7 +117,10 @@ int aifmm_ls_create(const aifmm_ls_params* p, aifmm_ls_problem** out) {
   })
 }
 
-void aifmm_ls_destroy(aifmm_ls_problem* h) { delete h; }
+void aifmm_ls_destroy(aifmm_ls_problem* h) {
+  aifmm::ApiScope api_scope_;
+  delete h;
+}
 
 int64_t aifmm_ls_size(const aifmm_ls_problem* h) { return h ? h->pb->N : -1; }
 
@@ -233,7 +252,7 @@ int aifmm_ls_set_rhs(aifmm_ls_problem* h, int nrhs, const double* b) {
     h->rhs = h->io.get<double2>(N * nrhs);
     h->sol = h->io.get<double2>(N * nrhs);
     h->nrhs = nrhs;
-    HIP_OK(hipMemcpy(h->rhs, b, N * nrhs * 16, hipMemcpyHostToDevice));
+    h2d(h->rhs, b, N * nrhs * 16, h->pb->stream);
   })
 }
 
@@ -275,12 +294,14 @@ int aifmm_ls_set_refinement(aifmm_ls_problem* h, int method) {
 int aifmm_ls_get_solution(aifmm_ls_problem* h, double* x) {
   GUARD({
     if (!h->sol) throw std::runtime_error("no solution");
-    HIP_OK(hipMemcpy(x, h->sol, h->pb->N * h->nrhs * 16, hipMemcpyDeviceToHost));
+    d2h(x, h->sol, h->pb->N * h->nrhs * 16, h->pb->stream);
   })
 }
 
-int aifmm_ls_solve_batch(aifmm_ls_problem* const* hs, int n, const double* const* b, double* const* x, double tol,
-                         int max_refine, double* relres, double* stats) {
+// The batch solve; tune: launch-tune newly merged launch sequences on their
+// first run (worth it for batches that repeat, not for one-off ones)
+static int solve_batch_impl(aifmm_ls_problem* const* hs, int n, const double* const* b, double* const* x, double tol,
+                            int max_refine, double* relres, double* stats, bool tune) {
   GUARD({
     if (n < 1 || !hs || !b || !x) throw std::runtime_error("empty batch");
     const int prec = hs[0]->precision;
@@ -307,7 +328,7 @@ int aifmm_ls_solve_batch(aifmm_ls_problem* const* hs, int n, const double* const
       h->rhs = v[i].b;
       h->sol = v[i].x;
       h->nrhs = 1;
-      HIP_OK(hipMemcpyAsync(v[i].b, b[i], N * 16, hipMemcpyHostToDevice, s));
+      h2d(v[i].b, b[i], N * 16, s);
     }
     // merged sweep and matvec launch sequences (cached on the leading handle)
     std::vector<const Program<double2>*> ps64, pm;
@@ -359,8 +380,9 @@ int aifmm_ls_solve_batch(aifmm_ls_problem* const* hs, int n, const double* const
         }
       };
       load();
-      if (prec == 0) bp->s64->run(1, s, load);
-      else bp->s32->run(1, s, load);
+      const std::function<void()> reload = tune ? std::function<void()>(load) : std::function<void()>();
+      if (prec == 0) bp->s64->run(1, s, reload);
+      else bp->s32->run(1, s, reload);
       for (int i = 0; i < n; ++i) {
         if (prec == 0) hs[i]->f64->sweep_store(dst[i], 1, s);
         else hs[i]->f32->sweep_store(dst[i], 1, s);
@@ -371,7 +393,7 @@ int aifmm_ls_solve_batch(aifmm_ls_problem* const* hs, int n, const double* const
         for (int i = 0; i < n; ++i) hs[i]->h2->load(src[i], 1, s);
       };
       load();
-      mat.run(1, s, load);
+      mat.run(1, s, tune ? std::function<void()>(load) : std::function<void()>());
       launch_near(n, bp->near_boxes, bp->near, s);
       for (int i = 0; i < n; ++i) hs[i]->h2->store(dst[i], 1, s);
     };
@@ -385,7 +407,7 @@ int aifmm_ls_solve_batch(aifmm_ls_problem* const* hs, int n, const double* const
       rel[i] = R[i].relres;
       steps = std::max(steps, R[i].steps);
     }
-    for (int i = 0; i < n; ++i) HIP_OK(hipMemcpyAsync(x[i], v[i].x, v[i].N * 16, hipMemcpyDeviceToHost, s));
+    for (int i = 0; i < n; ++i) d2h(x[i], v[i].x, v[i].N * 16, s);
     HIP_OK(hipStreamSynchronize(s));
     const double t = now_s() - t0;
     double worst = 0;
@@ -400,6 +422,12 @@ int aifmm_ls_solve_batch(aifmm_ls_problem* const* hs, int n, const double* const
       stats[2] = worst;
     }
   })
+}
+
+
+int aifmm_ls_solve_batch(aifmm_ls_problem* const* hs, int n, const double* const* b, double* const* x, double tol,
+                         int max_refine, double* relres, double* stats) {
+  return solve_batch_impl(hs, n, b, x, tol, max_refine, relres, stats, true);
 }
 
 int aifmm_ls_solve(aifmm_ls_problem* h, int nrhs, const double* b, double* x, double tol, int max_refine,
@@ -419,10 +447,30 @@ int aifmm_ls_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y) {
     Arena a{size_t(64) << 20};
     double2* dx = a.get<double2>(N * nrhs);
     double2* dy = a.get<double2>(N * nrhs);
-    HIP_OK(hipMemcpy(dx, x, N * nrhs * 16, hipMemcpyHostToDevice));
+    h2d(dx, x, N * nrhs * 16, h->pb->stream);
     h->h2->apply(dx, dy, nrhs);
     HIP_OK(hipStreamSynchronize(h->pb->stream));
-    HIP_OK(hipMemcpy(y, dy, N * nrhs * 16, hipMemcpyDeviceToHost));
+    d2h(y, dy, N * nrhs * 16, h->pb->stream);
+  })
+}
+
+int aifmm_ls_dense_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y) {
+  GUARD({
+    if (!h || !x || !y) throw std::runtime_error("null argument");
+    if (nrhs < 1) throw std::runtime_error("nrhs must be >= 1");
+    Problem& P = *h->pb;
+    const int64_t N = P.N;
+    const int nch = dmv_chunks(N, P.nleaf);
+    const size_t ns = dmv_scratch(N, P.nleaf, nch);
+    Arena a{(2 * (size_t)N + ns) * sizeof(double2) + (size_t(1) << 20)};
+    double2* dx = a.get<double2>(N);
+    double2* dy = a.get<double2>(N);
+    double2* sc = a.get<double2>(ns);
+    for (int c = 0; c < nrhs; ++c) {
+      h2d(dx, x + 2 * (size_t)N * c, (size_t)N * 16, P.stream);
+      launch_dense_matvec(P.ctx, N, dx, dy, sc, nch, P.stream);
+      d2h(y + 2 * (size_t)N * c, dy, (size_t)N * 16, P.stream);
+    }
   })
 }
 

@@ -343,7 +343,7 @@ void Factor<T>::recompress(int j, int c) {
     double* dn = sc.get<double>(cand.size());
     ProfScope ps(pb.stream, KNAME(T, "k_fro"), 0, 0);
     launch_fro<T>((int)nd.size(), dd, dn, pb.stream);
-    HIP_OK(hipMemcpyAsync(norms.data(), dn, 8 * cand.size(), hipMemcpyDeviceToHost, pb.stream));
+    d2h(norms.data(), dn, 8 * cand.size(), pb.stream);
     HIP_OK(hipStreamSynchronize(pb.stream));
   }
   std::vector<Fill> fills;
@@ -438,7 +438,7 @@ void Factor<T>::recompress(int j, int c) {
     launch_cpqr<T>((int)qd.size(), dq, pb.stream);
   }
   std::vector<int> rk(2 * nt);
-  HIP_OK(hipMemcpyAsync(rk.data(), dr, 8 * nt, hipMemcpyDeviceToHost, pb.stream));
+  d2h(rk.data(), dr, 8 * nt, pb.stream);
   HIP_OK(hipStreamSynchronize(pb.stream));
   std::vector<int> rnew(nb, -1);
   for (int i = 0; i < nt; ++i) {
@@ -623,7 +623,7 @@ void Factor<T>::base_factor() {
     dg.add(Blk<double2>{f.p, 1, n, f.ld + 1}, dd, 2);
     dg.run(sc, pb.stream);
     std::vector<double2> hd(n);
-    HIP_OK(hipMemcpyAsync(hd.data(), dd.p, n * sizeof(double2), hipMemcpyDeviceToHost, pb.stream));
+    d2h(hd.data(), dd.p, n * sizeof(double2), pb.stream);
     HIP_OK(hipStreamSynchronize(pb.stream));
     double mx = 0, mn = HUGE_VAL;
     for (auto& v : hd) mx = std::max(mx, std::hypot(v.x, v.y)), mn = std::min(mn, std::hypot(v.x, v.y));
@@ -660,7 +660,7 @@ void Factor<T>::factor() {
       int* dn = sc.get<int>(src.size());
       launch_gather_i32(ds, dn, (int)src.size(), pb.stream);
       std::vector<int> nz(src.size());
-      HIP_OK(hipMemcpyAsync(nz.data(), dn, nz.size() * 4, hipMemcpyDeviceToHost, pb.stream));
+      d2h(nz.data(), dn, nz.size() * 4, pb.stream);
       HIP_OK(hipStreamSynchronize(pb.stream));
       for (size_t i = 0; i < nz.size(); ++i)
         if (nz[i] < who[i][2]) st.zero_piv.insert(st.zero_piv.end(), {who[i][0], who[i][1], nz[i], who[i][2]});
@@ -708,7 +708,7 @@ void Factor<T>::sweep_operators() {
       }
     dg.run(sc, pb.stream);
     std::vector<double2> hd(nd);
-    if (nd) HIP_OK(hipMemcpyAsync(hd.data(), dd.p, nd * sizeof(double2), hipMemcpyDeviceToHost, pb.stream));
+    if (nd) d2h(hd.data(), dd.p, nd * sizeof(double2), pb.stream);
     HIP_OK(hipStreamSynchronize(pb.stream));
     o = 0;
     st.ill_boxes = 0;

@@ -136,12 +136,10 @@ void H2Op::build(int nrhs) {
       tk[e] = v;
     }
     dtk = ws.get<double2>(tk.size());
-    HIP_OK(hipMemcpyAsync(dtk, stage_pinned(tk.data(), tk.size() * sizeof(double2), P.stream),
-                          tk.size() * sizeof(double2), hipMemcpyHostToDevice, P.stream));
+    h2d(dtk, tk.data(), tk.size() * sizeof(double2), P.stream);
   }
   dnbr = ws.get<int>(nbr.size());
-  HIP_OK(hipMemcpyAsync(dnbr, stage_pinned(nbr.data(), nbr.size() * sizeof(int), P.stream), nbr.size() * sizeof(int),
-                        hipMemcpyHostToDevice, P.stream));
+  h2d(dnbr, nbr.data(), nbr.size() * sizeof(int), P.stream);
 }
 
 NearDesc H2Op::near_desc(int nrhs) const {
@@ -163,7 +161,7 @@ NearDesc H2Op::near_desc(int nrhs) const {
 void H2Op::near(int nrhs, hipStream_t s) {
   const NearDesc d = near_desc(nrhs);
   NearDesc* dd = (NearDesc*)stage_dev.alloc(sizeof(NearDesc));
-  HIP_OK(hipMemcpyAsync(dd, stage_pinned(&d, sizeof(d), s), sizeof(d), hipMemcpyHostToDevice, s));
+  h2d(dd, &d, sizeof(d), s);
   launch_near(1, d.nbox, dd, s);
   stage_dev.reset();
 }

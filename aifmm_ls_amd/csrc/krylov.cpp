@@ -16,7 +16,7 @@ void Refiner::norms(const double2* v, int nrhs, std::vector<double>& out) {
   std::vector<double> part((size_t)nblk * nrhs);
   double* dp = (double*)dpart_;
   launch_resid(v, nullptr, nullptr, N, nrhs, dp, pb_.stream);
-  HIP_OK(hipMemcpyAsync(part.data(), dp, part.size() * 8, hipMemcpyDeviceToHost, pb_.stream));
+  d2h(part.data(), dp, part.size() * 8, pb_.stream);
   HIP_OK(hipStreamSynchronize(pb_.stream));
   out.assign(nrhs, 0);
   for (int c = 0; c < nrhs; ++c) {
@@ -31,7 +31,7 @@ void Refiner::dots(const double2* V, int64_t vstride, int nv, const double2* w, 
   const int nblk = resid_blocks(N);
   std::vector<double2> part((size_t)nblk * nv * nrhs);
   launch_multidot(V, vstride, nv, w, N, nrhs, dpart_, pb_.stream);
-  HIP_OK(hipMemcpyAsync(part.data(), dpart_, part.size() * 16, hipMemcpyDeviceToHost, pb_.stream));
+  d2h(part.data(), dpart_, part.size() * 16, pb_.stream);
   HIP_OK(hipStreamSynchronize(pb_.stream));
   out.assign((size_t)nv * nrhs, make_double2(0, 0));
   for (int c = 0; c < nrhs; ++c)
@@ -52,10 +52,10 @@ void Refiner::lincomb(double2* w, const double2* V, int64_t vstride, int nv, con
   // without waiting, and the host vectors may go out of scope on return
   if (nv) {
     const size_t b = (size_t)nv * nrhs * 16;
-    HIP_OK(hipMemcpyAsync(dcoef_, stage_pinned(coef.data(), b, pb_.stream), b, hipMemcpyHostToDevice, pb_.stream));
+    h2d(dcoef_, coef.data(), b, pb_.stream);
   }
   const size_t bb = (size_t)nrhs * 16;
-  HIP_OK(hipMemcpyAsync(dbeta_, stage_pinned(beta.data(), bb, pb_.stream), bb, hipMemcpyHostToDevice, pb_.stream));
+  h2d(dbeta_, beta.data(), bb, pb_.stream);
   launch_lincomb(w, V, vstride, nv, dcoef_, dbeta_, pb_.N, nrhs, pb_.stream);
 }
 
@@ -95,7 +95,7 @@ RefineResult Refiner::run(const Precond& M, const double2* b, double2* x, int nr
       const int nblk = resid_blocks(N);
       std::vector<double> part((size_t)nblk * nr);
       launch_resid(b, r, r, N, nr, (double*)dpart_, s);
-      HIP_OK(hipMemcpyAsync(part.data(), dpart_, part.size() * 8, hipMemcpyDeviceToHost, s));
+      d2h(part.data(), dpart_, part.size() * 8, s);
       HIP_OK(hipStreamSynchronize(s));
       rn.assign(nr, 0);
       for (int c = 0; c < nr; ++c) {
@@ -278,8 +278,8 @@ std::vector<RefineResult> BatchRefiner::run(const std::vector<BatchVec>& v, cons
     std::vector<int64_t> off(n);
     std::vector<int> nb(n);
     for (int i = 0; i < n; ++i) off[i] = (int64_t)mem[i].poff, nb[i] = mem[i].nblk;
-    HIP_OK(hipMemcpyAsync(doff, stage_pinned(off.data(), n * 8, s), n * 8, hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemcpyAsync(dnb, stage_pinned(nb.data(), n * 4, s), n * 4, hipMemcpyHostToDevice, s));
+    h2d(doff, off.data(), n * 8, s);
+    h2d(dnb, nb.data(), n * 4, s);
   }
   // member descriptors of the batched vector launches (one launch per
   // operation for all members; uploads and launches share the stream, so the
@@ -290,7 +290,7 @@ std::vector<RefineResult> BatchRefiner::run(const std::vector<BatchVec>& v, cons
   std::vector<VecMember> vm(n);
   auto send = [&]() {
     const size_t bytes = (size_t)n * sizeof(VecMember);
-    HIP_OK(hipMemcpyAsync(dmem, stage_pinned(vm.data(), bytes, s), bytes, hipMemcpyHostToDevice, s));
+    h2d(dmem, vm.data(), bytes, s);
   };
   // partial squared norms of b_i - ax_i (ax_i null: of b_i; r_i null: not stored)
   auto resid = [&](const std::vector<const double2*>& b, const std::vector<const double2*>& ax,
@@ -340,7 +340,7 @@ std::vector<RefineResult> BatchRefiner::run(const std::vector<BatchVec>& v, cons
   // the host: out[i][0..nv)
   auto fetch = [&](int nv, std::vector<std::vector<double2>>& out) {
     launch_sum_partials(dpart, doff, dnb, n, nv, 0, dsum, s);
-    HIP_OK(hipMemcpyAsync(sums.data(), dsum, (size_t)n * nv * 16, hipMemcpyDeviceToHost, s));
+    d2h(sums.data(), dsum, (size_t)n * nv * 16, s);
     HIP_OK(hipStreamSynchronize(s));
     out.assign(n, {});
     for (int i = 0; i < n; ++i) out[i].assign(sums.begin() + (size_t)i * nv, sums.begin() + (size_t)(i + 1) * nv);
@@ -348,7 +348,7 @@ std::vector<RefineResult> BatchRefiner::run(const std::vector<BatchVec>& v, cons
   // 2-norms from the real partials of launch_resid
   auto fetch_norms = [&](std::vector<double>& out) {
     launch_sum_partials(dpart, doff, dnb, n, 1, 1, dsum, s);
-    HIP_OK(hipMemcpyAsync(sums.data(), dsum, (size_t)n * 16, hipMemcpyDeviceToHost, s));
+    d2h(sums.data(), dsum, (size_t)n * 16, s);
     HIP_OK(hipStreamSynchronize(s));
     out.resize(n);
     for (int i = 0; i < n; ++i) out[i] = std::sqrt(sums[i].x);
@@ -360,8 +360,8 @@ std::vector<RefineResult> BatchRefiner::run(const std::vector<BatchVec>& v, cons
     std::vector<double2> hc((size_t)n * (m + 1), make_double2(0, 0));
     for (int i = 0; i < n; ++i)
       for (int q = 0; q < nv[i]; ++q) hc[(size_t)i * (m + 1) + q] = coef[i][q];
-    HIP_OK(hipMemcpyAsync(dcoef, stage_pinned(hc.data(), hc.size() * 16, s), hc.size() * 16, hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemcpyAsync(dbeta, stage_pinned(beta.data(), n * 16, s), n * 16, hipMemcpyHostToDevice, s));
+    h2d(dcoef, hc.data(), hc.size() * 16, s);
+    h2d(dbeta, beta.data(), n * 16, s);
     lincomb_dev(w, nv, false);
   };
   std::vector<double> bn(n), rn(n);
@@ -498,8 +498,8 @@ std::vector<RefineResult> BatchRefiner::run(const std::vector<BatchVec>& v, cons
     std::vector<double2> hc((size_t)n * (m + 1), make_double2(0, 0));
     for (int i = 0; i < n; ++i)
       for (int q = 0; q < k; ++q) hc[(size_t)i * (m + 1) + q] = y[i][q];
-    HIP_OK(hipMemcpyAsync(dcoef, stage_pinned(hc.data(), hc.size() * 16, s), hc.size() * 16, hipMemcpyHostToDevice, s));
-    HIP_OK(hipMemcpyAsync(dbeta, stage_pinned(one.data(), n * 16, s), n * 16, hipMemcpyHostToDevice, s));
+    h2d(dcoef, hc.data(), hc.size() * 16, s);
+    h2d(dbeta, one.data(), n * 16, s);
     {
       std::vector<double2*> xs(n);
       for (int i = 0; i < n; ++i) xs[i] = v[i].x;

@@ -174,19 +174,21 @@ Problem::Problem(const Params& Pin) : P(Pin) {
   dpx = persist.get<double>(N);
   dpy = persist.get<double>(N);
   dq = persist.get<double>(N);
-  HIP_OK(hipMemcpy(dpx, px.data(), N * 8, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(dpy, py.data(), N * 8, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(dq, q.data(), N * 8, hipMemcpyHostToDevice));
+  h2d(dpx, px.data(), N * 8, stream);
+  h2d(dpy, py.data(), N * 8, stream);
+  h2d(dq, q.data(), N * 8, stream);
   std::vector<int64_t> iota(N);
   for (int64_t i = 0; i < N; ++i) iota[i] = i;
   d_iota = persist.get<int64_t>(N);
-  HIP_OK(hipMemcpy(d_iota, iota.data(), N * 8, hipMemcpyHostToDevice));
+  h2d(d_iota, iota.data(), N * 8, stream);
   build_lists();
   build_tables();
 }
 
 Problem::~Problem() {
-  if (stream) (void)hipStreamDestroy(stream);
+  if (!stream) return;
+  (void)hipStreamSynchronize(stream);  // queued work still reads this problem's memory
+  (void)hipStreamDestroy(stream);
 }
 
 void Problem::build_lists() {
@@ -327,9 +329,9 @@ void Problem::build_tables() {
   g.run(scratch, stream);
   M_h.resize((size_t)NTERM * R);
   T_h.resize((size_t)9 * R * R);
-  HIP_OK(hipMemcpyAsync(M_h.data(), Md.p, M_h.size() * 16, hipMemcpyDeviceToHost, stream));
-  HIP_OK(hipMemcpyAsync(T_h.data(), Td.p, T_h.size() * 16, hipMemcpyDeviceToHost, stream));
-  HIP_OK(hipMemcpyAsync(&quad_capped, dflag, 4, hipMemcpyDeviceToHost, stream));
+  d2h(M_h.data(), Md.p, M_h.size() * 16, stream);
+  d2h(T_h.data(), Td.p, T_h.size() * 16, stream);
+  d2h(&quad_capped, dflag, 4, stream);
   HIP_OK(hipStreamSynchronize(stream));
   if (P.quantize) {
     for (auto& v : M_h) v = make_double2(round6(v.x), round6(v.y));
@@ -337,8 +339,8 @@ void Problem::build_tables() {
   }
   dM = persist.get<double2>(M_h.size());
   dT = persist.get<double2>(T_h.size());
-  HIP_OK(hipMemcpy(dM, M_h.data(), M_h.size() * 16, hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(dT, T_h.data(), T_h.size() * 16, hipMemcpyHostToDevice));
+  h2d(dM, M_h.data(), M_h.size() * 16, stream);
+  h2d(dT, T_h.data(), T_h.size() * 16, stream);
   scratch.reset();
   ctx.px = dpx;
   ctx.py = dpy;
@@ -359,7 +361,7 @@ void Problem::entries(int64_t n, const int64_t* I, const int64_t* J, double2* ou
   int64_t* dJ = upload(scratch, Jv, stream);
   double2* dout = scratch.get<double2>(n);
   launch_entries(ctx, n, dI, dJ, dout, stream);
-  HIP_OK(hipMemcpyAsync(out, dout, n * 16, hipMemcpyDeviceToHost, stream));
+  d2h(out, dout, n * 16, stream);
   HIP_OK(hipStreamSynchronize(stream));
   scratch.reset();
 }
@@ -466,7 +468,7 @@ void Problem::nca_level(int j) {
     launch_aca(ctx, (int)probs.size(), dp, tol, tol_robust, stream);
     // every problem's rank and pivots in one copy
     std::vector<int> hres(rtot);
-    HIP_OK(hipMemcpyAsync(hres.data(), dres, rtot * sizeof(int), hipMemcpyDeviceToHost, stream));
+    d2h(hres.data(), dres, rtot * sizeof(int), stream);
     HIP_OK(hipStreamSynchronize(stream));
     CopyBatch<double2> keepcp;
     for (int i = p0; i < p1; ++i) {

@@ -94,15 +94,22 @@ ProfScope::~ProfScope() {
   if (pending.size() > 8192) fold();
 }
 
-// One staging ring per host thread.  A thread queues the copy out of its
-// region before it asks for the next one, so after a device synchronise every
-// copy the thread queued from its ring has completed: a ring that wraps
-// synchronises and starts over.  Threads that end hand their ring back; the
-// next owner starts with a wrap, since the old owner's copies may be queued.
+// One staging ring of pinned host memory per host thread.  Every host-to-
+// device copy staged through it records an event on its stream right after the
+// copy is queued (one event per stream, re-recorded); a ring that wraps waits
+// for those events only, never for the whole device, so threads that drive
+// other streams are not held up.  Device-to-host copies land in the ring and
+// wait for their own stream.  A ring whose buffer is too small for one copy
+// moves to a larger buffer and keeps the old one (copies may still read it)
+// until the process ends.  Threads that end hand their ring back with its
+// pending events; the next owner waits for them before it writes.
 namespace {
 struct Ring {
   char* base = nullptr;
   size_t cap = 0, off = 0;
+  std::vector<std::pair<hipStream_t, hipEvent_t>> marks;  // last copy per stream since the last wrap
+  std::vector<hipEvent_t> spare_ev;
+  std::vector<char*> retired;
 };
 std::mutex ring_mu;
 std::vector<Ring*> spare_rings;
@@ -115,39 +122,90 @@ struct RingSlot {
   }
 };
 thread_local RingSlot my_ring;
-}  // namespace
 
-void* stage_pinned(const void* src, size_t bytes, hipStream_t s) {
-  (void)s;
+Ring& ring() {
   if (!my_ring.r) {
     std::lock_guard<std::mutex> g(ring_mu);
     if (!spare_rings.empty()) {
       my_ring.r = spare_rings.back();
       spare_rings.pop_back();
-      my_ring.r->off = my_ring.r->cap;  // force a synchronised wrap first
+      my_ring.r->off = my_ring.r->cap;  // wait for the old owner's copies first
     } else {
       my_ring.r = new Ring();
     }
   }
-  Ring& R = *my_ring.r;
+  return *my_ring.r;
+}
+
+void wait_marks(Ring& R) {
+  for (auto& m : R.marks) {
+    HIP_OK(hipEventSynchronize(m.second));
+    R.spare_ev.push_back(m.second);
+  }
+  R.marks.clear();
+}
+
+// a region of `bytes` of pinned memory that no queued copy still reads
+char* take(Ring& R, size_t bytes) {
   const size_t need = (bytes + 255) & ~size_t(255);
   if (!R.base || need > R.cap) {
-    if (R.base) {
-      HIP_OK(hipDeviceSynchronize());
-      HIP_OK(hipHostFree(R.base));
-    }
+    if (R.base) R.retired.push_back(R.base);
+    for (auto& m : R.marks) R.spare_ev.push_back(m.second);  // their copies read the old buffer
+    R.marks.clear();
     R.cap = std::max<size_t>(size_t(64) << 20, need);
     HIP_OK(hipHostMalloc((void**)&R.base, R.cap, hipHostMallocDefault));
     R.off = 0;
   }
   if (R.off + need > R.cap) {
-    HIP_OK(hipDeviceSynchronize());
+    wait_marks(R);
     R.off = 0;
   }
-  void* p = R.base + R.off;
-  std::memcpy(p, src, bytes);
+  char* p = R.base + R.off;
   R.off += need;
   return p;
+}
+
+void mark(Ring& R, hipStream_t s) {
+  for (auto& m : R.marks)
+    if (m.first == s) {
+      HIP_OK(hipEventRecord(m.second, s));
+      return;
+    }
+  hipEvent_t e;
+  if (!R.spare_ev.empty()) {
+    e = R.spare_ev.back();
+    R.spare_ev.pop_back();
+  } else {
+    HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  HIP_OK(hipEventRecord(e, s));
+  R.marks.emplace_back(s, e);
+}
+
+constexpr size_t kChunk = size_t(16) << 20;
+}  // namespace
+
+void h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  Ring& R = ring();
+  for (size_t o = 0; o < bytes; o += kChunk) {
+    const size_t n = std::min(kChunk, bytes - o);
+    char* p = take(R, n);
+    std::memcpy(p, (const char*)src + o, n);
+    HIP_OK(hipMemcpyAsync((char*)dst + o, p, n, hipMemcpyHostToDevice, s));
+    mark(R, s);
+  }
+}
+
+void d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
+  Ring& R = ring();
+  for (size_t o = 0; o < bytes; o += kChunk) {
+    const size_t n = std::min(kChunk, bytes - o);
+    char* p = take(R, n);
+    HIP_OK(hipMemcpyAsync(p, (const char*)src + o, n, hipMemcpyDeviceToHost, s));
+    HIP_OK(hipStreamSynchronize(s));
+    std::memcpy((char*)dst + o, p, n);
+  }
+  if (!bytes) HIP_OK(hipStreamSynchronize(s));
 }
 
 }  // namespace aifmm

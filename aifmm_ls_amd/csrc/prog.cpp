@@ -2,6 +2,9 @@
 // algorithmic byte / flop count of one run.
 #include "prog.h"
 
+#include <condition_variable>
+#include <mutex>
+
 #include <algorithm>
 #include <cstddef>
 #include <atomic>
@@ -9,6 +12,7 @@
 #include <string>
 #include <type_traits>
 
+#include "batch.h"
 #include "prof.h"
 
 namespace aifmm {
@@ -119,7 +123,7 @@ void Program<T>::finalize(Arena& a, hipStream_t s) {
   const int lim = prog_max_m(sizeof(T));
   if (!terms_.empty()) {
     dterms_ = a.get<PTerm<T>>(terms_.size());
-    HIP_OK(hipMemcpyAsync(dterms_, terms_.data(), terms_.size() * sizeof(PTerm<T>), hipMemcpyHostToDevice, s));
+    h2d(dterms_, terms_.data(), terms_.size() * sizeof(PTerm<T>), s);
   }
   for (auto& g : st_) {
     if (g.maxm > lim)
@@ -150,7 +154,7 @@ void Program<T>::finalize(Arena& a, hipStream_t s) {
     }
     g.slot_words = (int)((offsetof(PSlot<T>, t) + inl * sizeof(PTerm<T>) + 15) / 16);
     g.dslots = a.get<PSlot<T>>(sl.size());
-    HIP_OK(hipMemcpyAsync(g.dslots, sl.data(), sl.size() * sizeof(PSlot<T>), hipMemcpyHostToDevice, s));
+    h2d(g.dslots, sl.data(), sl.size() * sizeof(PSlot<T>), s);
   }
   HIP_OK(hipStreamSynchronize(s));
 }
@@ -330,6 +334,43 @@ void Program<T>::tune(int nrhs, hipStream_t s, const std::function<void()>& relo
 // whose threads free memory and synchronise while others would capture
 static std::atomic<int> graph_holds{0};
 void prog_graphs_hold(bool on) { graph_holds += on ? 1 : -1; }
+
+namespace {
+std::mutex gate_mu;
+std::condition_variable gate_cv;
+int gate_inside = 0;      // threads inside the library
+bool gate_capturing = false;
+thread_local int gate_depth = 0;
+}  // namespace
+
+ApiScope::ApiScope() {
+  if (gate_depth++) return;
+  std::unique_lock<std::mutex> lk(gate_mu);
+  gate_cv.wait(lk, [] { return !gate_capturing; });
+  ++gate_inside;
+}
+
+ApiScope::~ApiScope() {
+  if (--gate_depth) return;
+  std::lock_guard<std::mutex> g(gate_mu);
+  --gate_inside;
+  gate_cv.notify_all();
+}
+
+// true: the calling thread is alone inside the library and others are held
+// out until capture_end()
+static bool capture_begin() {
+  std::lock_guard<std::mutex> g(gate_mu);
+  if (gate_inside > 1 || gate_capturing) return false;
+  gate_capturing = true;
+  return true;
+}
+
+static void capture_end() {
+  std::lock_guard<std::mutex> g(gate_mu);
+  gate_capturing = false;
+  gate_cv.notify_all();
+}
 static bool graphs_enabled() {
   static const bool on = [] {
     const char* e = getenv("AIFMM_GRAPH");
@@ -363,20 +404,31 @@ void Program<T>::run(int nrhs, hipStream_t s, const std::function<void()>& reloa
     return;
   }
   if (!gexec_ || gnrhs_ != nrhs) {
-    if (gexec_) HIP_OK(hipGraphExecDestroy(gexec_));
-    gexec_ = nullptr;
-    hipGraph_t g = nullptr;
-    HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-    try {
+    if (!capture_begin()) {  // other threads are inside: launch directly this time
       launch_all(nrhs, s, false);
+      return;
+    }
+    hipGraph_t g = nullptr;
+    try {
+      if (gexec_) HIP_OK(hipGraphExecDestroy(gexec_));
+      gexec_ = nullptr;
+      HIP_OK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        launch_all(nrhs, s, false);
+      } catch (...) {
+        (void)hipStreamEndCapture(s, &g);
+        throw;
+      }
+      HIP_OK(hipStreamEndCapture(s, &g));
+      HIP_OK(hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0));
+      HIP_OK(hipGraphDestroy(g));
+      g = nullptr;
     } catch (...) {
-      (void)hipStreamEndCapture(s, &g);
       if (g) (void)hipGraphDestroy(g);
+      capture_end();
       throw;
     }
-    HIP_OK(hipStreamEndCapture(s, &g));
-    HIP_OK(hipGraphInstantiate(&gexec_, g, nullptr, nullptr, 0));
-    HIP_OK(hipGraphDestroy(g));
+    capture_end();
     gnrhs_ = nrhs;
   }
   HIP_OK(hipGraphLaunch(gexec_, s));

@@ -82,6 +82,17 @@ bool prog_shape_fits(int nrhs, int ldy, int shape);
 int prog_max_m(int elem_bytes);
 // while held (counted), programs launch directly instead of through graphs
 void prog_graphs_hold(bool on);
+// Threads inside the library (one ApiScope per C ABI call, reentrant per
+// thread).  A sweep is captured into a HIP graph only while its thread is the
+// only one inside, and threads that enter during a capture wait for it to end:
+// another thread's frees, synchronisations or copies would otherwise meet the
+// capture.
+struct ApiScope {
+  ApiScope();
+  ~ApiScope();
+  ApiScope(const ApiScope&) = delete;
+  ApiScope& operator=(const ApiScope&) = delete;
+};
 
 // One box operation before lowering.
 template <class T>

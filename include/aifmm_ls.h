@@ -106,6 +106,14 @@ int aifmm_ls_solve_batch(aifmm_ls_problem* const* hs, int n, const double* const
  * factorisation as right preconditioner (default), 1 = plain iterative
  * refinement x += M (b - A_H x).  Both use the FP64 operator of aifmm_ls_matvec. */
 int aifmm_ls_set_refinement(aifmm_ls_problem* h, int method);
+/* Dense FP64 product y = A x with the discretised operator itself, every
+ * entry as getMatrixEntry over all leaf pairs: the generator's rhs = A x
+ * (FMM2DTree_gen_rhs_x.hpp:2808-2903, getMatrix blocks at :2864-2871).  Far
+ * entries are summed per source leaf through its moments (one Hankel series
+ * per point and leaf); near entries are formed and zeroed below 1e-16 one by
+ * one, as getMatrixEntry; unzeroed far entries below 1e-16 change a row sum
+ * by less than 1e-16 |x_j| each.  Host buffers, N x nrhs column-major. */
+int aifmm_ls_dense_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y);
 /* FP64 product y = A_H x with the assembled hierarchical operator (leaf P2P plus the
  * nested L2P M2L P2M far field), the operator the factorisation inverts; N x nrhs. */
 int aifmm_ls_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y);
@@ -182,7 +190,9 @@ int aifmm_ls_profile_launches(int reset, int max, double* out, char* names, int 
  * worker threads, each on its problem's own HIP stream, so requests for
  * different problems overlap on the device; queued solves for one problem
  * with equal tol / max_refine are merged into one multi-RHS solve (at most 64
- * right-hand sides).  New factorisations are admitted by device memory against
+ * right-hand sides), and queued one-column solves of different problems with
+ * equal tol / max_refine and factor precision into one batch solve (as
+ * aifmm_ls_solve_batch, at most 32 problems).  New factorisations are admitted by device memory against
  * budget_bytes, evicting idle problems least-recently used first; a solve for
  * an evicted problem factorises it again.  Submission copies the inputs and
  * returns at once; x must stay valid until the request is waited for.  While a
@@ -196,16 +206,19 @@ int aifmm_ls_server_factor(aifmm_ls_server* s, const aifmm_ls_params* p, int pre
 /* queue a solve of nrhs right-hand sides (N x nrhs interleaved complex, column-major) */
 int aifmm_ls_server_solve(aifmm_ls_server* s, int64_t problem, int nrhs, const double* b, double* x, double tol,
                           int max_refine, int64_t* ticket);
-/* block until the request ends; info[0..7] = status (0 ok, -1 failed), latency
- * (submit to end, s), queue wait (s), run time (s), refinement steps, relative
- * residual, right-hand sides in the merged solve, 1 if the problem was
- * factorised again for this request.  Each ticket is waited for once. */
+/* block until the request ends; info (optional, 9 doubles) = status (0 ok,
+ * -1 failed), latency (submit to end, s), queue wait (s), run time (s),
+ * refinement steps, relative residual, right-hand sides in the merged solve,
+ * 1 if the problem was factorised again for this request, problems in the
+ * batch it ran in (1 unless merged with other problems' one-column solves).
+ * Each ticket is waited for once. */
 int aifmm_ls_server_wait(aifmm_ls_server* s, int64_t ticket, double* info);
 /* drop a problem: its memory is freed now, or when its running request ends */
 int aifmm_ls_server_release(aifmm_ls_server* s, int64_t problem);
-/* out[0..9] = resident problems, resident bytes, budget, evictions,
+/* out[0..10] = resident problems, resident bytes, budget, evictions,
  * re-factorisations, completed requests, executed batches, requests merged
- * into another's solve, most batches running at once, queued requests */
+ * into another's solve, most batches running at once, queued requests,
+ * batches of several problems */
 int aifmm_ls_server_stats(aifmm_ls_server* s, double* out, int n);
 /* finish every queued request, then stop the slots and free all problems */
 int aifmm_ls_server_destroy(aifmm_ls_server* s);

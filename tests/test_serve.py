@@ -1,5 +1,6 @@
 """Serving loop (aifmm_ls_server_*): each result is the same alone and in a mix
-of concurrent requests, queued solves of one problem are merged, problems are
+of concurrent requests, queued solves of one problem are merged, queued
+one-column solves of different problems run as one batch, problems are
 admitted by device memory with least-recently-used eviction, and a solve for
 an evicted problem factorises it again.
 
@@ -93,6 +94,38 @@ def test_same_problem_solves_are_merged():
     assert s.stats()["merged_requests"] >= 5
     s.close()
     ref.close()
+
+
+def test_one_column_solves_of_different_problems_batch():
+    """One slot, busy with a 64-column solve while one-column solves of the
+    three problems queue (the one of problem 0 cannot join the 64-column
+    solve: 64 columns is the merge limit): when the slot frees, they run as
+    one batch solve (merged launch sequences, lockstep refinement), and each
+    result equals the same solve alone."""
+    print("\n[serve] cross-problem batch", flush=True)
+    ref = [alone(c) for c in PROBLEMS]
+    s = A.Server(64 << 30, slots=1)
+    pids = [s.factor(precision=A.PREC_C64, **c)[0] for c in PROBLEMS]
+    B0 = np.stack([ref[0].rhs(0.1 * i) for i in range(64)], axis=1)
+    t0 = s.solve(pids[0], B0, tol=TOL, max_refine=40)
+    bs = [ref[p].rhs(1.0 + p) for p in (1, 2, 0)]
+    ts = [s.solve(pids[p], b, tol=TOL, max_refine=40) for p, b in zip((1, 2, 0), bs)]
+    s.wait(t0)
+    worst, sizes = 0.0, []
+    for p, b, t in zip((1, 2, 0), bs, ts):
+        x, info = s.wait(t)
+        sizes.append(info["batch_problems"])
+        assert info["relres"] <= TOL
+        xa, _ = ref[p].solve(b, tol=TOL, max_refine=40)
+        worst = max(worst, rel(x, xa))
+    st = s.stats()
+    print(f"  batch sizes {sizes}, cross-problem batches {st['cross_problem_batches']:.0f}, "
+          f"worst batch vs alone {worst:.2e}", flush=True)
+    assert max(sizes) == 3 and st["cross_problem_batches"] >= 1
+    assert worst < 1e3 * TOL
+    s.close()
+    for g in ref:
+        g.close()
 
 
 def test_admission_by_memory_evicts_lru_and_refactors():

@@ -46,7 +46,7 @@ def build(verbose=True):
     return LIB
 
 
-DRIVERS = ["testFMM2D_find_error"]
+DRIVERS = ["testFMM2D_find_error", "testFMM2D_gen_rhs_x"]
 ROOT = os.path.dirname(HERE)
 
 

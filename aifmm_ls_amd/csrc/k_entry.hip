@@ -9,6 +9,7 @@
 // Entries below 1e-16 in modulus are zeroed.
 #include "common.h"
 #include "kernels.h"
+#include "prof.h"
 
 namespace aifmm {
 
@@ -465,6 +466,261 @@ __global__ __launch_bounds__(ACA_THREADS) void k_aca(EntryCtx C, const AcaProble
 }
 
 // ------------------------------------------------------------ launchers
+// ---------------------------------------------------------------- dense product
+// y = A x with every entry of A as ls_entry, over all leaf pairs: the
+// generator's dense rhs = A x (FMM2DTree_gen_rhs_x.hpp:2808-2903).  A far
+// entry from source leaf l is (i/4) kappa^2 q_i sum_n M(n, jj) H_n e^{i n phi}
+// with (rho, phi) taken from the leaf's centre, so the leaf's contribution to
+// row i is (i/4) kappa^2 q_i sum_n m_l(n) H_n e^{i n phi}, with the leaf
+// moments m_l = M x_l (121 values, the layout of a table column).  One Graf
+// series per (target point, source leaf) instead of one per entry.  The near
+// 3 x 3 neighbourhood is summed entry by entry, each entry formed and zeroed
+// below 1e-16 as ls_entry does; far entries are summed unzeroed (an entry
+// below 1e-16 changes its row sum by less than 1e-16 |x_j|).
+constexpr int DMV_T = 256;
+
+// m[l * NTERM + n] = sum_jj M(n, jj) x[l R + jj]
+__global__ __launch_bounds__(128) void k_dmv_moments(EntryCtx C, const double2* __restrict__ x,
+                                                     double2* __restrict__ m) {
+  const int l = blockIdx.x, n = threadIdx.x;
+  if (n >= NTERM) return;
+  const double2* xl = x + (size_t)l * C.R;
+  double2 acc = mk(0.0, 0.0);
+  for (int jj = 0; jj < C.R; ++jj) acc = cadd(acc, cmul(C.M[n + (size_t)NTERM * jj], xl[jj]));
+  m[(size_t)l * NTERM + n] = acc;
+}
+
+// The Graf sum of graf_sum for the dense product, on cheaper arithmetic and
+// few registers.  sum_n H_n c_n with c_n = m(60+n) w^n +- m(60-n) w^-n,
+// w = e^{i phi}, splits into sum_n J_n c_n and i sum_n Y_n c_n:
+//  * one backward pass of Miller's recurrence yields the unnormalised J_n,
+//    from which it accumulates sum_n J_n c_n (n <= 60), the normalisation
+//    J_0 + 2 sum_k J_2k, and the two Neumann sums that give Y_0 and Y_1
+//    (Abramowitz & Stegun 9.1.88, and its term-by-term derivative with
+//    Y_1 = -Y_0'):
+//      Y_0 = (2/pi) (ln(x/2) + gamma) J_0 - (4/pi) sum_k (-1)^k J_2k / k
+//      Y_1 = -(2/(pi x)) J_0 + (2/pi) (ln(x/2) + gamma) J_1
+//            + (2/pi) sum_k (-1)^k (J_2k-1 - J_2k+1) / k
+//  * one forward pass runs Y's recurrence and accumulates sum_n Y_n c_n.
+// The recurrences multiply by 1/x instead of dividing by x, and w is the
+// direction (dx, dy) / rho itself instead of sincos(atan2(dy, dx)); each
+// changes a term by rounding only (the series agree with scipy's Y_0, Y_1 to
+// 3e-14 relative for x up to 170).
+struct InvTable {
+  double v[1024];
+  constexpr InvTable() : v() {
+    for (int i = 1; i < 1024; ++i) v[i] = 1.0 / i;
+  }
+};
+__constant__ InvTable c_inv = InvTable();
+
+__device__ __forceinline__ double inv_int(int m) { return m < 1024 ? c_inv.v[m] : 1.0 / m; }
+
+__device__ __forceinline__ double2 graf_c(const double2* __restrict__ Mcol, int n, double2 pw) {
+  double2 c = cmul(Mcol[SERIES + n], pw);
+  if (n > 0) {
+    const double2 t = cmul(Mcol[SERIES - n], cconj(pw));
+    c = (n & 1) ? csub(c, t) : cadd(c, t);
+  }
+  return c;
+}
+
+__device__ __forceinline__ double2 graf_sum_dir(const double2* __restrict__ Mcol, double x, double2 e1) {
+  constexpr double kPi = 3.14159265358979323846, kGamma = 0.57721566490153286061;
+  const double rx = 1.0 / x;
+  // w^60 by squaring: w^4, w^8, w^16, w^32 and their product
+  double2 w60;
+  {
+    const double2 w2 = cmul(e1, e1), w4 = cmul(w2, w2), w8 = cmul(w4, w4), w16 = cmul(w8, w8),
+                  w32 = cmul(w16, w16);
+    w60 = cmul(cmul(w32, w16), cmul(w8, w4));
+  }
+  const double2 e1c = cconj(e1);
+  int big = x > (double)SERIES ? (int)x : SERIES;
+  int Mst = big + 30 + (int)sqrt(40.0 * big);
+  if (Mst & 1) ++Mst;
+  // jc holds the unnormalised J_n of the step's n = k - 1, jp1 J_{n+1}
+  double jp1 = 0.0, jc = 1e-300;
+  double sum = 0.0, s0 = 0.0, s1 = 0.0;  // J_0 + 2 sum J_2k;  Neumann sums of Y_0, Y_1
+  for (int k = Mst; k > SERIES + 1; --k) {
+    const double jm1 = (2.0 * k * rx) * jc - jp1;
+    jp1 = jc;
+    jc = jm1;
+    if (fabs(jc) > 0x1p800) {
+      jc *= 0x1p-800;
+      jp1 *= 0x1p-800;
+      sum *= 0x1p-800;
+      s0 *= 0x1p-800;
+      s1 *= 0x1p-800;
+    }
+    const int n = k - 1;
+    if ((n & 1) == 0) {
+      const int m = n >> 1;
+      sum += 2.0 * jc;
+      s0 += ((m & 1) ? -inv_int(m) : inv_int(m)) * jc;
+    } else {
+      const int a = (n + 1) >> 1;
+      const double w = inv_int(a) + inv_int(a - 1);
+      s1 += ((a & 1) ? -w : w) * jc;
+    }
+  }
+  {
+    const int e = ilogb(jc);
+    const double f = ldexp(1.0, -500 - e);
+    jc *= f;
+    jp1 *= f;
+    sum *= f;
+    s0 *= f;
+    s1 *= f;
+  }
+  double2 pw = w60;
+  double accj_re = 0.0, accj_im = 0.0;  // sum J~_n c_n (unnormalised)
+#pragma unroll 2
+  for (int k = SERIES + 1; k >= 1; --k) {
+    const double jm1 = (2.0 * k * rx) * jc - jp1;
+    jp1 = jc;
+    jc = jm1;
+    const int n = k - 1;
+    if ((n & 1) == 0) {
+      if (n > 0) {
+        const int m = n >> 1;
+        sum += 2.0 * jc;
+        s0 += ((m & 1) ? -inv_int(m) : inv_int(m)) * jc;
+      }
+    } else {
+      const int a = (n + 1) >> 1;
+      const double w = a > 1 ? inv_int(a) + inv_int(a - 1) : 1.0;
+      s1 += ((a & 1) ? -w : w) * jc;
+    }
+    const double2 c = graf_c(Mcol, n, pw);
+    accj_re += jc * c.x;
+    accj_im += jc * c.y;
+    if (n > 0) pw = cmul(pw, e1c);
+  }
+  sum += jc;
+  const double rs = 1.0 / sum;
+  const double j0 = jc * rs, j1 = jp1 * rs;
+  const double lg = log(0.5 * x) + kGamma;
+  const double y0v = (2.0 / kPi) * lg * j0 - (4.0 / kPi) * (s0 * rs);
+  const double y1v = -(2.0 / kPi) * rx * j0 + (2.0 / kPi) * lg * j1 + (2.0 / kPi) * (s1 * rs);
+  // forward: Y_n and w^n
+  pw = mk(1.0, 0.0);
+  double accy_re = 0.0, accy_im = 0.0;  // sum Y_n c_n
+  double ym1 = 0.0, yc = y0v;
+#pragma unroll 2
+  for (int n = 0; n <= SERIES; ++n) {
+    if (n == 1) {
+      ym1 = y0v;
+      yc = y1v;
+    } else if (n > 1) {
+      const double yn = (2.0 * (n - 1) * rx) * yc - ym1;
+      ym1 = yc;
+      yc = yn;
+    }
+    const double2 c = graf_c(Mcol, n, pw);
+    accy_re += yc * c.x;
+    accy_im += yc * c.y;
+    pw = cmul(pw, e1);
+  }
+  // sum H_n c_n = sum J_n c_n + i sum Y_n c_n
+  return mk(accj_re * rs - accy_im, accj_im * rs + accy_re);
+}
+
+// grid (row tiles of DMV_T target points, leaf chunks): part[c N + i] = the
+// Graf sums of target i over the far leaves of chunk c
+__global__ __launch_bounds__(DMV_T) void k_dmv_far(EntryCtx C, const double2* __restrict__ m,
+                                                   double2* __restrict__ part, int64_t N, int nleaf, int lpc) {
+  const int64_t i = (int64_t)blockIdx.x * DMV_T + threadIdx.x;
+  const int c = blockIdx.y;
+  if (i >= N) return;
+  const int l0 = c * lpc, l1 = min(nleaf, l0 + lpc);
+  int xi, yi;
+  box_xy(C.L, (int)(i / C.R), xi, yi);
+  const double px = C.px[i], py = C.py[i];
+  double2 acc = mk(0.0, 0.0);
+  for (int l = l0; l < l1; ++l) {
+    int xj, yj;
+    box_xy(C.L, l, xj, yj);
+    if (xj - xi >= -1 && xj - xi <= 1 && yj - yi >= -1 && yj - yi <= 1) continue;
+    const double cx = -C.Ldom + (2 * xj + 1) * C.rleaf, cy = -C.Ldom + (2 * yj + 1) * C.rleaf;
+    const double ddy = py - cy, ddx = px - cx;
+    const double rho = sqrt(ddy * ddy + ddx * ddx);
+    const double ir = 1.0 / rho;
+    acc = cadd(acc, graf_sum_dir(m + (size_t)NTERM * l, C.kappa * rho, mk(ddx * ir, ddy * ir)));
+  }
+  part[(size_t)c * N + i] = acc;
+}
+
+// y_i = (i/4) kappa^2 q_i sum_c part[c N + i] (chunks in order) + the near
+// entries of row i times x
+__global__ __launch_bounds__(DMV_T) void k_dmv_sum(EntryCtx C, const double2* __restrict__ x,
+                                                   const double2* __restrict__ part, int nchunk, double2* __restrict__ y,
+                                                   int64_t N) {
+  const int64_t i = (int64_t)blockIdx.x * DMV_T + threadIdx.x;
+  if (i >= N) return;
+  const int R = C.R, li = (int)(i / R), ii = (int)(i - (int64_t)li * R);
+  double2 far = mk(0.0, 0.0);
+  for (int c = 0; c < nchunk; ++c) far = cadd(far, part[(size_t)c * N + i]);
+  const double qi = C.q[i];
+  double2 acc = cscale(cmul(far, mk(0.0, 0.25)), C.kappa * C.kappa * qi);
+  int xi, yi;
+  box_xy(C.L, li, xi, yi);
+  const int side = 1 << C.L;
+  for (int nb = 0; nb < 9; ++nb) {
+    const int dx = nb < 8 ? c_nb_off[nb][0] : 0, dy = nb < 8 ? c_nb_off[nb][1] : 0;
+    if (xi + dx < 0 || xi + dx >= side || yi + dy < 0 || yi + dy >= side) continue;
+    const int lj = xy_box(C.L, xi + dx, yi + dy);
+    const double2* Tt = C.T + (size_t)nb * R * R + ii;
+    const double2* xl = x + (size_t)lj * R;
+    for (int jj = 0; jj < R; ++jj) {
+      double2 e = Tt[(size_t)R * jj];
+      e = cscale(e, C.kappa);
+      e = cscale(e, C.kappa);
+      e = cscale(e, qi);
+      if (nb == 8 && jj == ii) e.x = 1.0 + e.x;
+      if (hypot(e.x, e.y) < 1e-16) e = mk(0.0, 0.0);
+      acc = cadd(acc, cmul(e, xl[jj]));
+    }
+  }
+  y[i] = acc;
+}
+
+int dmv_chunks(int64_t N, int nleaf) {
+  const int64_t tiles = (N + DMV_T - 1) / DMV_T;
+  int64_t c = (4096 + tiles - 1) / tiles;
+  return (int)(c < 1 ? 1 : c > nleaf ? nleaf : c);
+}
+
+void launch_dense_matvec(const EntryCtx& C, int64_t N, const double2* x, double2* y, double2* scratch, int nchunk,
+                         hipStream_t s) {
+  const int nleaf = (int)(N / C.R);
+  if (nchunk < 1 || nchunk > nleaf) throw std::runtime_error("dense product: bad leaf-chunk count");
+  double2* m = scratch;
+  double2* part = scratch + (size_t)NTERM * nleaf;
+  const int lpc = (nleaf + nchunk - 1) / nchunk;
+  const unsigned tiles = (unsigned)((N + DMV_T - 1) / DMV_T);
+  {
+    ProfScope ps(s, "k_dmv_moments", 16.0 * ((double)NTERM * C.R + N + (double)NTERM * nleaf),
+                 8.0 * NTERM * (double)N);
+    hipLaunchKernelGGL(k_dmv_moments, dim3(nleaf), dim3(128), 0, s, C, x, m);
+    HIP_OK(hipGetLastError());
+  }
+  {
+    // flops: an estimate per (point, far leaf) pair of the Graf series (61
+    // combined terms of about 29 flops, Y recurrence included) and Miller's
+    // recurrence for J (about 150 steps of 3 flops): 2,200
+    ProfScope ps(s, "k_dmv_far", 16.0 * ((double)NTERM * nleaf + (double)nchunk * N),
+                 2200.0 * (double)N * (nleaf - 9));
+    hipLaunchKernelGGL(k_dmv_far, dim3(tiles, nchunk), dim3(DMV_T), 0, s, C, m, part, N, nleaf, lpc);
+    HIP_OK(hipGetLastError());
+  }
+  {
+    ProfScope ps(s, "k_dmv_sum", 16.0 * ((double)nchunk * N + 10.0 * N), 8.0 * 9 * C.R * (double)N);
+    hipLaunchKernelGGL(k_dmv_sum, dim3(tiles), dim3(DMV_T), 0, s, C, x, part, nchunk, y, N);
+    HIP_OK(hipGetLastError());
+  }
+}
+
 void launch_entries(const EntryCtx& C, int64_t n, const int64_t* I, const int64_t* J, double2* out, hipStream_t s) {
   if (n == 0) return;
   int th = 128;

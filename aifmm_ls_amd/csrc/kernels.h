@@ -35,6 +35,17 @@ struct AcaProblem {
 void launch_entries(const EntryCtx& C, int64_t n, const int64_t* I, const int64_t* J, double2* out, hipStream_t s);
 void launch_assemble(const EntryCtx& C, int nblk, const BlockDesc* d, hipStream_t s);
 void launch_aca(const EntryCtx& C, int nprob, const AcaProblem* d, double tol, double tol_robust, hipStream_t s);
+// Dense product y = A x (N values each, one column) with every entry as
+// getMatrixEntry: leaf moments, far Graf sums per (point, leaf chunk), then the
+// chunk sums in order plus the near entries.  scratch holds
+// dmv_scratch(N, nleaf, nchunk) double2: the moments (NTERM x nleaf) and the
+// partial sums (nchunk x N).  dmv_chunks: the default chunk count.
+int dmv_chunks(int64_t N, int nleaf);
+inline size_t dmv_scratch(int64_t N, int nleaf, int nchunk) {
+  return (size_t)NTERM * nleaf + (size_t)nchunk * N;
+}
+void launch_dense_matvec(const EntryCtx& C, int64_t N, const double2* x, double2* y, double2* scratch, int nchunk,
+                         hipStream_t s);
 
 // ----------------------------------------------------------- quadrature
 struct QuadTask {

@@ -12,6 +12,7 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "aifmm_ls_amd", "_lib", "testFMM2D_find_error")
+GEN = os.path.join(ROOT, "aifmm_ls_amd", "_lib", "testFMM2D_gen_rhs_x")
 
 
 def write_vec(path, v):
@@ -45,10 +46,11 @@ def test_vector_file_layout(tmp_path):
 
 
 def test_driver_built_and_usage():
-    if not os.path.exists(DRIVER):
-        pytest.skip("driver not built")
-    r = subprocess.run([DRIVER], capture_output=True, text=True)
-    assert r.returncode == 2 and "usage" in r.stderr
+    for exe in (DRIVER, GEN):
+        if not os.path.exists(exe):
+            pytest.skip("drivers not built")
+        r = subprocess.run([exe], capture_output=True, text=True)
+        assert r.returncode == 2 and "usage" in r.stderr
 
 
 @pytest.mark.gpu
@@ -87,3 +89,43 @@ def test_find_error_protocol(tmp_path):
     assert r.returncode == 0, r.stderr
     err2 = float(r.stdout.strip().splitlines()[-1].split(":")[1])
     assert err2 < 1.5 * ref_err + 1e-9
+
+
+@pytest.mark.gpu
+def test_generator_then_find_error(tmp_path):
+    """The reference's two-run protocol with this build's own binaries only
+    (Readme.md: run the generator, then find_error with the same arguments):
+    testFMM2D_gen_rhs_x solves for the incident field, writes x and the dense
+    rhs = A x to rhs_x/ and prints |A x - b| / |b|; testFMM2D_find_error then
+    solves A y = rhs and reports |y - x| / |x|.  Checked: the files against
+    the oracle's dense matrix (rhs = A x to 1e-12), the printed residual
+    against numpy's, and the forward error against the reference schedule's
+    own on the same right-hand side."""
+    sys.path.insert(0, ROOT)
+    from oracle import oracle as O
+    args = ["6", "10", "6", "1", "0", "8", "3"]
+    print("\n[driver] generator L=3", flush=True)
+    r = subprocess.run([GEN, *args], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "gen rhs x" in r.stdout
+    gen_err = float(r.stdout.strip().splitlines()[-1].split(":")[1])
+    o = O.Problem(3, 10.0, 8, 0)
+    N = o.N
+    tag = f"0_10_{N}.bin"
+    d = tmp_path / "rhs_x"
+    x = read_vec(d / f"x_real_{tag}") + 1j * read_vec(d / f"x_imag_{tag}")
+    rhs = read_vec(d / f"rhs_real_{tag}") + 1j * read_vec(d / f"rhs_imag_{tag}")
+    Ad = o.dense()
+    b0 = o.rhs(0.0)
+    assert np.linalg.norm(rhs - Ad @ x) / np.linalg.norm(Ad @ x) < 1e-12
+    res = np.linalg.norm(Ad @ x - b0) / np.linalg.norm(b0)
+    assert abs(gen_err - res) <= 1e-5 * res + 1e-15  # printed to 6 significant digits, as the reference does
+    print(f"[driver] find_error on the generator's files (residual of x: {res:.3e})", flush=True)
+    r = subprocess.run([DRIVER, *args], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "rhs from rhs_x/rhs_real_" in r.stdout
+    fwd = float(r.stdout.strip().splitlines()[-1].split(":")[1])
+    xe, _ = o.solve(rhs, O.SCHED_EFFICIENT)
+    ref = np.linalg.norm(xe - x) / np.linalg.norm(x)
+    print(f"  forward error {fwd:.3e}, reference schedule {ref:.3e}", flush=True)
+    assert fwd < 1.5 * ref + 1e-9

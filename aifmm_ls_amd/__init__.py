@@ -62,6 +62,8 @@ def lib():
         L.aifmm_ls_stats.argtypes = [P, dp, ctypes.c_int]
         L.aifmm_ls_matvec.argtypes = [P, ctypes.c_int, dp, dp]
         L.aifmm_ls_dense_matvec.argtypes = [P, ctypes.c_int, dp, dp]
+        L.aifmm_ls_dense_matvec_noalloc.argtypes = [P, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                                     ctypes.c_void_p]
         ip64 = ctypes.POINTER(ctypes.c_int64)
         ip = ctypes.POINTER(ctypes.c_int)
         L.aifmm_ls_aca.argtypes = [P, ctypes.c_int, ctypes.c_int, ip64, ctypes.c_int, ip64, ip, ip, dp, dp, ip]

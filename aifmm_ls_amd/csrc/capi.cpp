@@ -58,6 +58,9 @@ static int solve_batch_impl(aifmm_ls_problem* const* hs, int n, const double* co
 }
 
 namespace aifmm {
+// The problem behind a handle (for entry points defined in other files).
+Problem* problem_of(aifmm_ls_problem* h) { return h ? h->pb.get() : nullptr; }
+
 // The batch solve without launch tuning of its merged sequences (serving loop:
 // its batches change members from call to call).
 int solve_batch_untuned(aifmm_ls_problem* const* hs, int n, const double* const* b, double* const* x, double tol,

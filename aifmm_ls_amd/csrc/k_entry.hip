@@ -692,7 +692,7 @@ int dmv_chunks(int64_t N, int nleaf) {
 }
 
 void launch_dense_matvec(const EntryCtx& C, int64_t N, const double2* x, double2* y, double2* scratch, int nchunk,
-                         hipStream_t s) {
+                         hipStream_t s, bool profile) {
   const int nleaf = (int)(N / C.R);
   if (nchunk < 1 || nchunk > nleaf) throw std::runtime_error("dense product: bad leaf-chunk count");
   double2* m = scratch;
@@ -700,8 +700,8 @@ void launch_dense_matvec(const EntryCtx& C, int64_t N, const double2* x, double2
   const int lpc = (nleaf + nchunk - 1) / nchunk;
   const unsigned tiles = (unsigned)((N + DMV_T - 1) / DMV_T);
   {
-    ProfScope ps(s, "k_dmv_moments", 16.0 * ((double)NTERM * C.R + N + (double)NTERM * nleaf),
-                 8.0 * NTERM * (double)N);
+    ProfScope ps(profile ? s : nullptr, profile ? "k_dmv_moments" : nullptr,
+                 16.0 * ((double)NTERM * C.R + N + (double)NTERM * nleaf), 8.0 * NTERM * (double)N);
     hipLaunchKernelGGL(k_dmv_moments, dim3(nleaf), dim3(128), 0, s, C, x, m);
     HIP_OK(hipGetLastError());
   }
@@ -709,13 +709,14 @@ void launch_dense_matvec(const EntryCtx& C, int64_t N, const double2* x, double2
     // flops: an estimate per (point, far leaf) pair of the Graf series (61
     // combined terms of about 29 flops, Y recurrence included) and Miller's
     // recurrence for J (about 150 steps of 3 flops): 2,200
-    ProfScope ps(s, "k_dmv_far", 16.0 * ((double)NTERM * nleaf + (double)nchunk * N),
-                 2200.0 * (double)N * (nleaf - 9));
+    ProfScope ps(profile ? s : nullptr, profile ? "k_dmv_far" : nullptr,
+                 16.0 * ((double)NTERM * nleaf + (double)nchunk * N), 2200.0 * (double)N * (nleaf - 9));
     hipLaunchKernelGGL(k_dmv_far, dim3(tiles, nchunk), dim3(DMV_T), 0, s, C, m, part, N, nleaf, lpc);
     HIP_OK(hipGetLastError());
   }
   {
-    ProfScope ps(s, "k_dmv_sum", 16.0 * ((double)nchunk * N + 10.0 * N), 8.0 * 9 * C.R * (double)N);
+    ProfScope ps(profile ? s : nullptr, profile ? "k_dmv_sum" : nullptr, 16.0 * ((double)nchunk * N + 10.0 * N),
+                 8.0 * 9 * C.R * (double)N);
     hipLaunchKernelGGL(k_dmv_sum, dim3(tiles), dim3(DMV_T), 0, s, C, x, part, nchunk, y, N);
     HIP_OK(hipGetLastError());
   }

@@ -44,8 +44,9 @@ int dmv_chunks(int64_t N, int nleaf);
 inline size_t dmv_scratch(int64_t N, int nleaf, int nchunk) {
   return (size_t)NTERM * nleaf + (size_t)nchunk * N;
 }
+// profile = false: no profiler events (nothing is created or allocated)
 void launch_dense_matvec(const EntryCtx& C, int64_t N, const double2* x, double2* y, double2* scratch, int nchunk,
-                         hipStream_t s);
+                         hipStream_t s, bool profile = true);
 
 // ----------------------------------------------------------- quadrature
 struct QuadTask {

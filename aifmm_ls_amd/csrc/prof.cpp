@@ -78,7 +78,7 @@ std::vector<LaunchRecord> prof_launches(bool reset) {
 
 ProfScope::ProfScope(hipStream_t s, const char* name, double bytes, double flops)
     : s_(s), name_(name), bytes_(bytes), flops_(flops) {
-  if (!enabled) return;
+  if (!enabled || !name) return;  // a null name records nothing
   std::lock_guard<std::mutex> g(mu);
   e0_ = take();
   HIP_OK(hipEventRecord(e0_, s_));

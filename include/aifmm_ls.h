@@ -114,6 +114,18 @@ int aifmm_ls_set_refinement(aifmm_ls_problem* h, int method);
  * one, as getMatrixEntry; unzeroed far entries below 1e-16 change a row sum
  * by less than 1e-16 |x_j| each.  Host buffers, N x nrhs column-major. */
 int aifmm_ls_dense_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y);
+/* The dense product of aifmm_ls_dense_matvec without device allocation.  No
+ * counterpart in the reference: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * it).  x and y are the caller's device buffers (N x nrhs, column-major
+ * interleaved complex); the scratch is a static 64 MiB device array of the
+ * library (held on each GPU it runs on until the process ends), so N must
+ * satisfy 121 x 4^L + N <= 4,194,304 complex values (up to 7 levels at 6 x 6
+ * points per leaf); larger problems are refused.  stream: a hipStream_t to
+ * launch on, returning at once, or null for the problem's own stream, waited
+ * for before returning.  Calls that overlap on one GPU share the array and
+ * race; consecutive calls need no clearing. */
+int aifmm_ls_dense_matvec_noalloc(aifmm_ls_problem* h, int nrhs, const double* x, double* y, void* stream);
 /* FP64 product y = A_H x with the assembled hierarchical operator (leaf P2P plus the
  * nested L2P M2L P2M far field), the operator the factorisation inverts; N x nrhs. */
 int aifmm_ls_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y);

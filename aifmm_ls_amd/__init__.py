@@ -70,6 +70,7 @@ def lib():
         L.aifmm_ls_selftest.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp, dp, dp,
                                         ctypes.POINTER(ctypes.c_int), ctypes.c_double]
         L.aifmm_ls_profile_enable.argtypes = [ctypes.c_int]
+        L.aifmm_ls_selftest_stream.argtypes = [ctypes.c_int64, ctypes.c_int, dp]
         L.aifmm_ls_set_refinement.argtypes = [P, ctypes.c_int]
         L.aifmm_ls_profile_read.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_char_p, ctypes.c_int]
         L.aifmm_ls_profile_launches.argtypes = [ctypes.c_int, ctypes.c_int, dp, ctypes.c_char_p, ctypes.c_int]
@@ -326,6 +327,14 @@ class LSProblem:
 
 def _cm(a):
     return np.ascontiguousarray(np.asarray(a, np.complex128).T).view(np.float64)
+
+
+def selftest_stream(nbytes, reps=3):
+    """Copy nbytes between two fresh device buffers reps times with the plain
+    copy kernel (k_stream_copy); returns the mean milliseconds per copy."""
+    ms = ctypes.c_double()
+    _chk(lib().aifmm_ls_selftest_stream(int(nbytes), int(reps), ctypes.byref(ms)))
+    return ms.value
 
 
 def selftest_gemm(A, B, c64=False):

@@ -765,7 +765,37 @@ extern "C" int aifmm_ls_selftest(int op, int m, int n, int k, const double* A, c
 
 
 
-extern "C" int aifmm_ls_profile_enable(int on) {
+extern "C" int aifmm_ls_selftest_stream(int64_t bytes, int reps, double* ms) {
+  GUARD({
+    if (bytes < 16 || reps < 1) throw std::runtime_error("stream self-test: bytes >= 16 and reps >= 1");
+    const int64_t n = bytes / 16;
+    Arena a{size_t(2 * n) * 16 + 4096};
+    double2* src = a.get<double2>(n);
+    double2* dst = a.get<double2>(n);
+    hipStream_t s;
+    HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    HIP_OK(hipMemsetAsync(src, 0, (size_t)n * 16, s));
+    double tot = 0;
+    for (int r = 0; r < reps; ++r) {
+      HIP_OK(hipEventRecord(e0, s));
+      launch_stream_copy(r & 1 ? dst : src, r & 1 ? src : dst, n, s);
+      HIP_OK(hipEventRecord(e1, s));
+      HIP_OK(hipEventSynchronize(e1));
+      float t = 0;
+      HIP_OK(hipEventElapsedTime(&t, e0, e1));
+      tot += t;
+    }
+    if (ms) *ms = tot / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+  })
+}
+
+int aifmm_ls_profile_enable(int on) {
   GUARD({
     if (on < 0 || on > 2) throw std::runtime_error("profile mode must be 0, 1 or 2");
     prof_enable(on != 0, on == 2);

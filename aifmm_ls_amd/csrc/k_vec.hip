@@ -253,6 +253,26 @@ __global__ __launch_bounds__(VNT) void k_gather_i32(const int* const* __restrict
   if (i < n) dst[i] = *src[i];
 }
 
+// dst[i] = src[i] over n 16-byte elements (grid-stride), for calibrating
+// hardware traffic counters against a stream of known size
+__global__ __launch_bounds__(VNT) void k_stream_copy(const double2* __restrict__ src, double2* __restrict__ dst,
+                                                     int64_t n) {
+  const int64_t st = (int64_t)gridDim.x * VNT;
+  int64_t i = (int64_t)blockIdx.x * VNT + threadIdx.x;
+  for (; i + 3 * st < n; i += 4 * st) {  // four loads in flight per thread
+    const double2 a = src[i], b = src[i + st], c = src[i + 2 * st], d = src[i + 3 * st];
+    dst[i] = a;
+    dst[i + st] = b;
+    dst[i + 2 * st] = c;
+    dst[i + 3 * st] = d;
+  }
+  for (; i < n; i += st) dst[i] = src[i];
+}
+
+void launch_stream_copy(const double2* src, double2* dst, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_stream_copy, dim3(8192), dim3(VNT), 0, s, src, dst, n);
+}
+
 void launch_gather_i32(const int* const* src, int* dst, int n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_gather_i32, dim3((n + VNT - 1) / VNT), dim3(VNT), 0, s, src, dst, n);
 }

@@ -201,6 +201,9 @@ void launch_lincomb(double2* w, const double2* V, int64_t vstride, int nv, const
 void launch_sum_partials(const double2* part, const int64_t* off, const int* nblk, int n, int nv, int real,
                          double2* out, hipStream_t s);
 void launch_d2f(const double2* src, float2* dst, int64_t n, hipStream_t s);
+// dst[i] = src[i] over n 16-byte elements: a plain stream for calibrating
+// hardware traffic counters
+void launch_stream_copy(const double2* src, double2* dst, int64_t n, hipStream_t s);
 // dst[i] = *src[i], i < n (scattered device ints, e.g. per-factor pivot counts)
 void launch_gather_i32(const int* const* src, int* dst, int n, hipStream_t s);
 void launch_f2d(const float2* src, double2* dst, int64_t n, hipStream_t s);

@@ -8,6 +8,10 @@ profiler sees that part on its own:
   sweep   a complex64 factorisation, then three direct sweeps
           (``solve_resident(tol, 0)``: no refinement)
   factor  one assembly, then two complex64 factorisations
+  stream  no problem: three copies of 1 GiB between two fresh 1 GiB device
+          buffers with the plain copy kernel k_stream_copy (16-byte elements),
+          for calibrating FETCH_SIZE / WRITE_SIZE against a stream of known
+          size far beyond the L2 and MALL capacities
 
 Run under ``rocprofv3`` with ``AIFMM_GRAPH=0`` (launches issued one by one):
 the profiler crashes in the host runtime during HIP stream capture on this
@@ -27,7 +31,7 @@ def gaussian_bumps(seed, n):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("name", choices=("sweep", "factor"))
+    ap.add_argument("name", choices=("sweep", "factor", "stream"))
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--kappa", type=float, default=40.0)
     ap.add_argument("--tol-pow", type=int, default=7)
@@ -37,6 +41,10 @@ def main(argv=None):
 
     import aifmm_ls_amd as A
 
+    if args.name == "stream":
+        ms = A.selftest_stream(1 << 30, 3)
+        print(f"stream: 1 GiB copy {ms:.3f} ms, {2 * (1 << 30) / ms / 1e6:.1f} GB/s read+write", flush=True)
+        return 0
     g = A.LSProblem(args.levels, args.kappa, args.tol_pow, qchoice=6, bumps=gaussian_bumps(args.seed, args.bumps))
     g.assemble()
     if args.name == "factor":

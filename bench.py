@@ -30,6 +30,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+FP32_MATRIX_PEAK_TFS = 157.3  # MI355X dense FP32 matrix
+FP64_PEAK_TFS = 78.6  # MI355X dense FP64 (vector and matrix)
 _T0 = time.perf_counter()
 
 
@@ -254,31 +256,136 @@ def many_rhs(A, seed, nrhs, refine_tol):
             "rel_err_vs_fp64_max": max(errs), "rel_err_columns": cols}
 
 
-def cpu_baseline(levels, kappa, tol_pow, bumps, theta, n_full):
-    """The oracle (FP64 CPU restatement of the reference, sequential box order)
-    on a smaller tree of the same workload: per-RHS work of the reference =
-    eliminate_phase_efficient (the RHS is fused into it) + back_substitution."""
+def host_cpu():
+    """The host CPU: model name, logical CPUs, physical cores (lscpu), and the
+    OpenMP thread count the oracle runs with."""
+    import subprocess
+    info = {"model": None, "logical_cpus": os.cpu_count(), "physical_cores": None,
+            "omp_threads": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {l.split(":", 1)[0].strip(): l.split(":", 1)[1].strip() for l in out.splitlines() if ":" in l}
+        info["model"] = kv.get("Model name")
+        cps, sockets = kv.get("Core(s) per socket"), kv.get("Socket(s)")
+        if cps and sockets:
+            info["physical_cores"] = int(cps) * int(sockets)
+    except Exception:  # lscpu missing: leave the fields empty
+        pass
+    return info
+
+
+def oracle_run(levels, kappa, tol_pow, qchoice, bumps, theta, schedule):
+    """The oracle (FP64 CPU restatement of the reference) on one workload:
+    (solution, stats) with stats = tables, assembly, factor (elimination, the
+    RHS sweep fused in as in the reference) and back-substitution seconds."""
     from oracle import oracle as O
-    p = O.Problem(levels, kappa, tol_pow, qchoice=6, bumps=bumps)
-    b = p.rhs(theta)
-    _, st = p.solve(b, O.SCHED_EFFICIENT)
+    p = O.Problem(levels, kappa, tol_pow, qchoice=qchoice, bumps=bumps)
+    x, st = p.solve(p.rhs(theta), schedule)
+    return x, st
+
+
+def cpu_baseline(st, levels, n, threads):
+    """The reference's per-RHS work on the host: eliminate_phase_efficient
+    (its RHS sweep is fused into the elimination, so every right-hand side
+    repeats it) plus back_substitution, timed in the oracle at the bench's own
+    configuration."""
     t_rhs = st[2] + st[3]
-    scale = p.N / n_full  # the method is O(N)
     return {
-        "value": round(float(scale / t_rhs), 6),
+        "value": round(1.0 / t_rhs, 6),
         "unit": "solves/s",
-        "cores": 1,
+        "cores": threads,
         "kind": "port",
-        "sample": (f"oracle at {levels} levels (N={p.N}), same kappa/tol/contrast: elimination + "
-                   f"back-substitution {t_rhs:.3f} s per RHS (assembly {st[1]:.2f} s not counted), "
-                   f"scaled by N ratio {scale:.4f} to N={n_full}"),
+        "sample": (f"the full workload ({levels} levels, N={n}): oracle eliminate_phase_efficient + "
+                   f"back_substitution {t_rhs:.2f} s per RHS (elimination {st[2]:.2f} s, back-substitution "
+                   f"{st[3]:.2f} s; sequential over boxes, as the reference); assembly {st[1]:.2f} s on "
+                   f"{threads} OpenMP threads, not counted"),
+        "assemble_s": round(float(st[1]), 3),
+        "threads_in_elimination": 1,
     }
+
+
+def dense_product(A, levels, kappa, tol_pow, bumps, theta, solutions):
+    """The generator's dense rhs = A x (aifmm_ls_dense_matvec) on the bench's
+    problem: kernel times from HIP events, (point, far leaf) pairs per second
+    and the flop estimate of the far kernel against the FP64 peak; and, for
+    each named solution, its relative residual |A x - b| / |b| against the
+    discretised operator itself (every entry as getMatrixEntry), which no
+    compression or elimination schedule enters."""
+    g = A.LSProblem(levels, kappa, tol_pow, qchoice=6, bumps=bumps)
+    x = g.rhs(0.3)
+    g.dense_matvec(x)
+    A.profile_enable(True)
+    A.profile_read(reset=True)
+    t0 = time.perf_counter()
+    g.dense_matvec(x)
+    wall = time.perf_counter() - t0
+    prof = {k["name"]: k for k in A.profile_read(reset=True)}
+    A.profile_enable(False)
+    nleaf = g.N // 36
+    far = prof.get("k_dmv_far")
+    kernel_ms = sum(k["ms"] for k in prof.values())
+    out = {"n": int(g.N), "wall_ms_host_buffers": round(1e3 * wall, 3), "kernel_ms": round(kernel_ms, 3),
+           "pairs_per_s": round(g.N * (nleaf - 9) / (kernel_ms * 1e-3), 1)}
+    if far:
+        tf = far["flops"] / (far["ms"] * 1e-3) / 1e12
+        out.update({"far_kernel_ms": round(far["ms"], 3), "far_tflops_est": round(tf, 3),
+                    "far_frac_of_fp64_peak": round(tf / FP64_PEAK_TFS, 4)})
+    b = g.rhs(theta)
+    names = list(solutions)
+    if names:
+        Y = g.dense_matvec(np.stack([solutions[k] for k in names], axis=1))
+        out["dense_residuals"] = {k: float(np.linalg.norm(Y[:, c] - b) / np.linalg.norm(b))
+                                  for c, k in enumerate(names)}
+    g.close()
+    return out
+
+
+def reference_run(A, O):
+    """The reference's own run `6 10 6 1 0 8 5` (5 levels, N = 36,864, kappa 10,
+    Qchoice 0, TOL_POW 8): GPU assembly, complex128 factorisation and direct
+    solve (the plain method), complex64 factorisation and refined solve, beside
+    the oracle's phases (sequential elimination, as the reference); the direct
+    solve against the oracle's colour schedule under the same zero-pivot
+    rules, and the refined one against the oracle's
+    eliminate_phase_efficient."""
+    g = A.LSProblem(5, 10.0, 8, 0)
+    b = g.rhs(0.0)
+    t0 = time.perf_counter()
+    g.assemble()
+    t_asm = time.perf_counter() - t0
+    g.set_pivot_eps(1e-12, 1e-12)
+    g.factor(A.PREC_C128)
+    t_f64 = g.stats()["t_factor"]
+    g.set_rhs(b)
+    g.solve_resident(1e-12, 0)
+    st = g.solve_resident(1e-12, 0)
+    x_dir = g.get_solution()[:, 0].copy()
+    g.set_pivot_eps()
+    g.factor(A.PREC_C64)
+    t_f32 = g.stats()["t_factor"]
+    g.solve_resident(1e-10, 30)
+    st2 = g.solve_resident(1e-10, 30)
+    x_ref = g.get_solution()[:, 0].copy()
+    g.close()
+    p = O.Problem(5, 10.0, 8, 0, piv_eps=1e-12, base_piv_eps=1e-12)
+    x_col, _ = p.solve(b, O.SCHED_COLORED)
+    pe = O.Problem(5, 10.0, 8, 0)
+    x_eff, st_o = pe.solve(b, O.SCHED_EFFICIENT)
+    rel = lambda a, r: float(np.linalg.norm(a - r) / np.linalg.norm(r))
+    return {"args": "6 10 6 1 0 8 5", "n": int(b.size),
+            "gpu": {"assemble_s": round(t_asm, 4), "factor_c128_s": round(t_f64, 4),
+                    "direct_solve_ms": round(1e3 * st[0], 4), "factor_c64_s": round(t_f32, 4),
+                    "refined_solve_ms": round(1e3 * st2[0], 4), "refine_steps": int(st2[1])},
+            "oracle": {"assemble_s": round(float(st_o[1]), 3), "factor_s": round(float(st_o[2]), 3),
+                       "solve_s": round(float(st_o[3]), 4)},
+            "fp64_direct_vs_oracle_colour": rel(x_dir, x_col),
+            "refined_vs_oracle_reference_schedule": rel(x_ref, x_eff)}
 
 
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--kappa", type=float, default=40.0)
@@ -289,14 +396,15 @@ def main():
     ap.add_argument("--refine-tol", type=float, default=1e-7,
                     help="relative residual target; rel.err vs FP64 is checked against 1e-6")
     ap.add_argument("--max-refine", type=int, default=30)
-    ap.add_argument("--fp64-steps", type=int, default=5)
-    ap.add_argument("--cpu-levels", type=int, default=5, help="tree depth of the CPU baseline sample (0: skip)")
+    ap.add_argument("--fp64-steps", type=int, default=50)
+    ap.add_argument("--cpu", type=int, default=1,
+                    help="1: time the oracle on the bench's workload (CPU baseline, distance to the reference's "
+                         "own direct solve) and on the reference's run 6 10 6 1 0 8 5; 0: skip")
+    ap.add_argument("--dense-product", type=int, default=1, help="1: time the generator's dense A x; 0: skip")
     ap.add_argument("--batch-rhs", type=int, default=16, help="RHS count of the extra batched-throughput line (0: skip)")
     ap.add_argument("--batch-problems", type=int, default=32, help="problems in the uneven-batch line (0: skip)")
     ap.add_argument("--many-rhs", type=int, default=256, help="RHS count of the 7-level many-RHS line (0: skip)")
-    ap.add_argument("--serve-seconds", type=float, default=0.0,
-                    help="length of the serving trace (0: skip; off by default: this trace has stalled "
-                         "without finishing, cause not yet found)")
+    ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the serving trace (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="serving arrivals per second")
     ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="serving device-memory budget")
     args = ap.parse_args()
@@ -315,6 +423,7 @@ def main():
     t_assemble = time.perf_counter() - t0
     note(f"assembled N={g.N} in {t_assemble:.2f} s")
     b = g.rhs(args.theta)
+    N = int(g.N)
     xn = None
 
     # factor times: the median of three factorisations (one alone is noisy)
@@ -332,7 +441,9 @@ def main():
     x_ref = g.get_solution()[:, 0]
     xn = np.linalg.norm(x_ref)
     direct = g.solve_resident(1e-13, 0)
-    rel_direct = float(np.linalg.norm(g.get_solution()[:, 0] - x_ref) / xn)
+    x_dir64 = g.get_solution()[:, 0].copy()
+    rel_direct = float(np.linalg.norm(x_dir64 - x_ref) / xn)
+    flops64 = g.stats()["factor_flops"]
     fp64 = None
     if args.fp64_steps > 0:
         timed_solves(g, 1, args.refine_tol, args.max_refine)
@@ -435,9 +546,32 @@ def main():
                                 1e-8, 3)
         note("serving trace done")
 
-    cpu = None
-    if args.cpu_levels > 0:
-        cpu = cpu_baseline(args.cpu_levels, args.kappa, args.tol_pow, bumps, args.theta, g.N)
+    cpu = vs_oracle = c1 = x_orc = None
+    host = host_cpu()
+    if args.cpu:
+        from oracle import oracle as O
+        threads = int(host["omp_threads"] or host["logical_cpus"] or 1)
+        x_orc, st_orc = oracle_run(args.levels, args.kappa, args.tol_pow, 6, bumps, args.theta, O.SCHED_EFFICIENT)
+        cpu = cpu_baseline(st_orc, args.levels, N, threads)
+        cpu["host"] = host
+        on = np.linalg.norm(x_orc)
+        vs_oracle = {"refined_mixed": float(np.linalg.norm(x - x_orc) / on),
+                     "fp64_direct_colour_schedule": float(np.linalg.norm(x_dir64 - x_orc) / on),
+                     "fp64_refined": float(np.linalg.norm(x_ref - x_orc) / on),
+                     "note": "distance to the oracle's eliminate_phase_efficient, the reference's own direct solve; "
+                             "it holds the reference method's error (its direct solve leaves a residual against "
+                             "the operator it factorises), which refinement removes"}
+        note("oracle at the bench workload done")
+        c1 = reference_run(A, O)
+        note("reference run line done")
+
+    dense = None
+    if args.dense_product:
+        sols = {"mixed_refined": x, "fp64_direct_colour_schedule": x_dir64}
+        if x_orc is not None:
+            sols["oracle_reference_schedule_direct"] = x_orc
+        dense = dense_product(A, args.levels, args.kappa, args.tol_pow, bumps, args.theta, sols)
+        note("dense product line done")
 
     out = {
         "metric": "solves/s + p99 latency at rel.err<=1e-6 vs FP64 (N=147k, kappa=40); factor time",
@@ -453,7 +587,7 @@ def main():
         "dtype": "c64",
         "data": "synthetic",
         "config": {"workload": "single RHS solve, mixed precision (complex64 factor, FP64 refinement)",
-                   "n": int(g.N), "levels": args.levels, "kappa": args.kappa, "tol": 10.0 ** -args.tol_pow,
+                   "n": N, "levels": args.levels, "kappa": args.kappa, "tol": 10.0 ** -args.tol_pow,
                    "contrast": f"{args.bumps} Gaussian bumps, seed {args.seed}", "rhs": f"plane wave theta={args.theta}",
                    "nrhs": 1, "refine_tol": args.refine_tol},
         "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 4),
@@ -465,6 +599,19 @@ def main():
         "fp64_method": fp64,
         "assemble_s": round(t_assemble, 4),
         "factor_flops": st["factor_flops"],
+        "factor_roofline": {
+            "mixed": {"flops": st["factor_flops"], "s": round(t_factor32, 5),
+                      "achieved_tflops": round(st["factor_flops"] / t_factor32 / 1e12, 4),
+                      "peak_tflops": FP32_MATRIX_PEAK_TFS, "bound": "fp32 matrix",
+                      "frac": round(st["factor_flops"] / t_factor32 / 1e12 / FP32_MATRIX_PEAK_TFS, 6)},
+            "fp64_method": {"flops": flops64, "s": round(t_factor64, 5),
+                            "achieved_tflops": round(flops64 / t_factor64 / 1e12, 4),
+                            "peak_tflops": FP64_PEAK_TFS, "bound": "fp64 matrix",
+                            "frac": round(flops64 / t_factor64 / 1e12 / FP64_PEAK_TFS, 6)},
+            "mixed_speedup_vs_fp64_method": round(t_factor64 / t_factor32, 3)},
+        "rel_err_vs_oracle": vs_oracle,
+        "reference_run_c1": c1,
+        "dense_product": dense,
         "reference_order_boxes": int(st["ill_boxes"]),
         "host_buffers": host_io,
         "batched_rhs": batched,

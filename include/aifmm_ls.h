@@ -182,6 +182,11 @@ int aifmm_ls_selftest_prog(int precision, int shape, int nprog, int m, int nterm
                            const double* A, const double* X, const double* S, const double* Bm, double* Y,
                            double* Out);
 
+/* Stream self-test (no reference counterpart; for calibrating hardware traffic
+ * counters): copies `bytes` between two fresh device buffers `reps` times in
+ * alternating directions with the plain copy kernel k_stream_copy; *ms gets
+ * the mean kernel time. */
+int aifmm_ls_selftest_stream(int64_t bytes, int reps, double* ms);
 /* Kernel timing (no reference counterpart; for benchmarks): when enabled, every
  * batched launch records a HIP event pair on its stream together with its
  * algorithmic bytes and flops.  profile_read fills, for up to max kernels,

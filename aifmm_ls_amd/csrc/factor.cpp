@@ -343,7 +343,9 @@ void Factor<T>::recompress(int j, int c) {
     double* dn = sc.get<double>(cand.size());
     ProfScope ps(pb.stream, KNAME(T, "k_fro"), 0, 0);
     launch_fro<T>((int)nd.size(), dd, dn, pb.stream);
+    const double w0 = now_s();
     d2h(norms.data(), dn, 8 * cand.size(), pb.stream);
+    t_wait_ += now_s() - w0;
     HIP_OK(hipStreamSynchronize(pb.stream));
   }
   std::vector<Fill> fills;
@@ -438,7 +440,9 @@ void Factor<T>::recompress(int j, int c) {
     launch_cpqr<T>((int)qd.size(), dq, pb.stream);
   }
   std::vector<int> rk(2 * nt);
+  const double w1 = now_s();
   d2h(rk.data(), dr, 8 * nt, pb.stream);
+  t_wait_ += now_s() - w1;
   HIP_OK(hipStreamSynchronize(pb.stream));
   std::vector<int> rnew(nb, -1);
   for (int i = 0; i < nt; ++i) {
@@ -638,16 +642,31 @@ void Factor<T>::factor() {
   double t0 = now_s();
   B.assign(L + 1, {});
   colors.assign(L + 1, {});
+  // AIFMM_FACTOR_TIMERS=1: host wall time per phase on stderr (a phase's time
+  // includes waits for the device where it synchronises)
+  static const bool timers = getenv("AIFMM_FACTOR_TIMERS") != nullptr;
+  double tph[8] = {0};
+  auto tick = [&](int k, double& t) {
+    const double n = now_s();
+    tph[k] += n - t;
+    t = n;
+  };
+  double tt = now_s();
   for (int j = 2; j <= L; ++j) init_level(j);
+  tick(0, tt);
   for (int j = L; j >= 2; --j) {
     if (j != L) nonleaf_p2p(j);
+    tick(1, tt);
     for (int c = 0; c < 9; ++c) {
       if (colors[j][c].empty()) continue;
       eliminate_color(j, c);
+      tick(2, tt);
       recompress(j, c);
+      tick(3, tt);
     }
   }
   base_factor();
+  tick(4, tt);
   if constexpr (std::is_same<T, double2>::value) {
     // nonzero-pivot counts of every L2M_f factorisation (one gather, one readback)
     std::vector<const int*> src;
@@ -666,13 +685,23 @@ void Factor<T>::factor() {
         if (nz[i] < who[i][2]) st.zero_piv.insert(st.zero_piv.end(), {who[i][0], who[i][1], nz[i], who[i][2]});
     }
   }
+  tick(4, tt);
   sweep_operators();
+  tick(5, tt);
   dense_top_alloc();
   build_plan(1);
+  tick(6, tt);
   dense_top_fill();
   HIP_OK(hipStreamSynchronize(pb.stream));
+  tick(7, tt);
   sc.release();  // factorisation scratch; later users allocate afresh
   st.t_factor = now_s() - t0;
+  if (timers)
+    fprintf(stderr,
+            "[factor] init %.1f ms, nonleaf P2P %.1f, eliminate %.1f, recompress %.1f, base %.1f, sweep operators "
+            "%.1f, plan %.1f, dense top %.1f; recompress waits %.1f ms\n",
+            1e3 * tph[0], 1e3 * tph[1], 1e3 * tph[2], 1e3 * tph[3], 1e3 * tph[4], 1e3 * tph[5], 1e3 * tph[6],
+            1e3 * tph[7], 1e3 * t_wait_);
   st.bytes = ar.used();
   for (int j = 2; j <= L; ++j)
     for (auto& F : B[j]) st.max_rank = std::max(st.max_rank, F.r);

@@ -202,6 +202,7 @@ class Factor {
   void recompress(int j, int c);
   void base_factor();
   Blk<T> nb_blk(int j, int row, int col);  // coupling block (row eq, col unknown)
+  double t_wait_ = 0;                       // host seconds waiting at recompression readbacks
 };
 
 }  // namespace aifmm

@@ -62,6 +62,9 @@ def lib():
         L.aifmm_ls_stats.argtypes = [P, dp, ctypes.c_int]
         L.aifmm_ls_matvec.argtypes = [P, ctypes.c_int, dp, dp]
         L.aifmm_ls_dense_matvec.argtypes = [P, ctypes.c_int, dp, dp]
+        L.aifmm_ls_set_find_phi.argtypes = [P, ctypes.c_int]
+        L.aifmm_ls_incident_field.argtypes = [P, ctypes.c_double, dp]
+        L.aifmm_ls_total_field.argtypes = [P, ctypes.c_double, dp, dp]
         L.aifmm_ls_dense_matvec_noalloc.argtypes = [P, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
                                                      ctypes.c_void_p]
         ip64 = ctypes.POINTER(ctypes.c_int64)
@@ -294,6 +297,32 @@ class LSProblem:
         _chk(lib().aifmm_ls_dense_matvec(self.h, nrhs, _dp(xx), _dp(y)))
         y = y.view(np.complex128).reshape(nrhs, self.N).T
         return y[:, 0] if x.ndim == 1 else y
+
+    def set_find_phi(self, on):
+        """LS2DTree::findPhi for entries() and dense_matvec(): True (default)
+        the LS operator I + kappa^2 q G, False the volume potential G."""
+        _chk(lib().aifmm_ls_set_find_phi(self.h, int(bool(on))))
+
+    def incident_field(self, theta=0.0):
+        """exp(i kappa d.x) at the points, d = (cos theta, sin theta)."""
+        u = np.zeros(2 * self.N)
+        _chk(lib().aifmm_ls_incident_field(self.h, float(theta), _dp(u)))
+        return u.view(np.complex128)
+
+    def scattered_field(self, phi):
+        """G phi with the dense volume-potential product (findPhi false)."""
+        self.set_find_phi(False)
+        try:
+            return self.dense_matvec(phi)
+        finally:
+            self.set_find_phi(True)
+
+    def total_field(self, phi, theta=0.0):
+        """u_inc + G phi for the plane wave at theta (hybrid.cpp:264-281)."""
+        phi = np.ascontiguousarray(np.asarray(phi, np.complex128))
+        u = np.zeros(2 * self.N)
+        _chk(lib().aifmm_ls_total_field(self.h, float(theta), _dp(phi.view(np.float64)), _dp(u)))
+        return u.view(np.complex128)
 
     def aca(self, rows, cols, tol_pow):
         rows = np.ascontiguousarray(rows, np.int64)

@@ -471,9 +471,46 @@ int aifmm_ls_dense_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double
     double2* sc = a.get<double2>(ns);
     for (int c = 0; c < nrhs; ++c) {
       h2d(dx, x + 2 * (size_t)N * c, (size_t)N * 16, P.stream);
-      launch_dense_matvec(P.ctx, N, dx, dy, sc, nch, P.stream);
+      launch_dense_matvec(P.ctx_phi(), N, dx, dy, sc, nch, P.stream);
       d2h(y + 2 * (size_t)N * c, dy, (size_t)N * 16, P.stream);
     }
+  })
+}
+
+int aifmm_ls_set_find_phi(aifmm_ls_problem* h, int on) {
+  GUARD({
+    if (!h) throw std::runtime_error("null handle");
+    h->pb->find_phi = on ? 1 : 0;
+  })
+}
+
+int aifmm_ls_incident_field(aifmm_ls_problem* h, double theta, double* u) {
+  GUARD({
+    if (!h || !u) throw std::runtime_error("null argument");
+    const Problem& P = *h->pb;
+    const double k = P.P.kappa;
+    for (int64_t i = 0; i < P.N; ++i) {
+      // exp(i kappa d.x) (LS_functions.hpp:113-116; d = (1, 0) at theta = 0)
+      const double ph = theta == 0.0 ? k * P.px[i] : k * (cos(theta) * P.px[i] + sin(theta) * P.py[i]);
+      u[2 * i] = cos(ph);
+      u[2 * i + 1] = sin(ph);
+    }
+  })
+}
+
+int aifmm_ls_total_field(aifmm_ls_problem* h, double theta, const double* phi, double* u) {
+  GUARD({
+    if (!h || !phi || !u) throw std::runtime_error("null argument");
+    Problem& P = *h->pb;
+    const int64_t N = P.N;
+    const int keep = P.find_phi;
+    P.find_phi = 0;
+    std::vector<double> us(2 * (size_t)N);
+    const int rc = aifmm_ls_dense_matvec(h, 1, phi, us.data());
+    P.find_phi = keep;
+    if (rc) throw std::runtime_error(aifmm_ls_last_error());
+    if (aifmm_ls_incident_field(h, theta, u)) throw std::runtime_error(aifmm_ls_last_error());
+    for (size_t t = 0; t < us.size(); ++t) u[t] += us[t];
   })
 }
 

@@ -81,6 +81,9 @@ struct EntryCtx {
   double kappa;
   double rleaf;
   double Ldom;
+  // LS2DTree::findPhi (LS_Assembly.hpp:507-512, 530-535): 1, entries of the LS
+  // operator I + kappa^2 q G; 0, of the volume potential G alone
+  int find_phi = 1;
 };
 
 }  // namespace aifmm

@@ -110,10 +110,12 @@ __device__ double2 ls_entry(const EntryCtx& C, int64_t i, int64_t j) {
       for (int s = 0; s < 8; ++s)
         if (c_nb_off[s][0] == dx && c_nb_off[s][1] == dy) nb = s;
     double2 e = C.T[(size_t)nb * R * R + ii + (size_t)R * jj];
-    e = cscale(e, C.kappa);
-    e = cscale(e, C.kappa);
-    e = cscale(e, qi);
-    if (i == j) e.x = 1.0 + e.x;
+    if (C.find_phi) {
+      e = cscale(e, C.kappa);
+      e = cscale(e, C.kappa);
+      e = cscale(e, qi);
+      if (i == j) e.x = 1.0 + e.x;
+    }
     if (hypot(e.x, e.y) < 1e-16) e = mk(0.0, 0.0);
     return e;
   }
@@ -124,8 +126,10 @@ __device__ double2 ls_entry(const EntryCtx& C, int64_t i, int64_t j) {
   double arg = C.kappa * sqrt(R2);
   double2 s = graf_sum(C.M + (size_t)NTERM * jj, arg, phi);
   s = cmul(s, mk(0.0, 0.25));
-  s = cscale(s, C.kappa * C.kappa * qi);
-  if (i == j) s.x += 1.0;
+  if (C.find_phi) {
+    s = cscale(s, C.kappa * C.kappa * qi);
+    if (i == j) s.x += 1.0;
+  }
   if (hypot(s.x, s.y) < 1e-16) s = mk(0.0, 0.0);
   return s;
 }
@@ -662,7 +666,8 @@ __global__ __launch_bounds__(DMV_T) void k_dmv_sum(EntryCtx C, const double2* __
   double2 far = mk(0.0, 0.0);
   for (int c = 0; c < nchunk; ++c) far = cadd(far, part[(size_t)c * N + i]);
   const double qi = C.q[i];
-  double2 acc = cscale(cmul(far, mk(0.0, 0.25)), C.kappa * C.kappa * qi);
+  double2 acc = cmul(far, mk(0.0, 0.25));
+  if (C.find_phi) acc = cscale(acc, C.kappa * C.kappa * qi);
   int xi, yi;
   box_xy(C.L, li, xi, yi);
   const int side = 1 << C.L;
@@ -674,10 +679,12 @@ __global__ __launch_bounds__(DMV_T) void k_dmv_sum(EntryCtx C, const double2* __
     const double2* xl = x + (size_t)lj * R;
     for (int jj = 0; jj < R; ++jj) {
       double2 e = Tt[(size_t)R * jj];
-      e = cscale(e, C.kappa);
-      e = cscale(e, C.kappa);
-      e = cscale(e, qi);
-      if (nb == 8 && jj == ii) e.x = 1.0 + e.x;
+      if (C.find_phi) {
+        e = cscale(e, C.kappa);
+        e = cscale(e, C.kappa);
+        e = cscale(e, qi);
+        if (nb == 8 && jj == ii) e.x = 1.0 + e.x;
+      }
       if (hypot(e.x, e.y) < 1e-16) e = mk(0.0, 0.0);
       acc = cadd(acc, cmul(e, xl[jj]));
     }

@@ -71,8 +71,8 @@ extern "C" int aifmm_ls_dense_matvec_noalloc(aifmm_ls_problem* h, int nrhs, cons
     // the problem's tables and points were uploaded on its own stream
     if (stream) HIP_OK(hipStreamSynchronize(P.stream));
     for (int c = 0; c < nrhs; ++c)
-      launch_dense_matvec(P.ctx, N, (const double2*)x + (size_t)N * c, (double2*)y + (size_t)N * c, sc, nch, s,
-                          false);
+      launch_dense_matvec(P.ctx_phi(), N, (const double2*)x + (size_t)N * c, (double2*)y + (size_t)N * c, sc, nch,
+                          s, false);
     if (!stream) HIP_OK(hipStreamSynchronize(s));
     return 0;
   } catch (const std::exception& e) {

@@ -360,7 +360,7 @@ void Problem::entries(int64_t n, const int64_t* I, const int64_t* J, double2* ou
   int64_t* dI = upload(scratch, Iv, stream);
   int64_t* dJ = upload(scratch, Jv, stream);
   double2* dout = scratch.get<double2>(n);
-  launch_entries(ctx, n, dI, dJ, dout, stream);
+  launch_entries(ctx_phi(), n, dI, dJ, dout, stream);
   d2h(out, dout, n * 16, stream);
   HIP_OK(hipStreamSynchronize(stream));
   scratch.reset();

@@ -60,7 +60,17 @@ class Problem {
   void build_tables();
   void build_lists();
   void assemble();
+  // LS2DTree::findPhi of entries() and the dense product (LS_Assembly.hpp:195,
+  // 507-512, 530-535): 1, the LS operator I + kappa^2 q G; 0, the volume
+  // potential G alone.  Assembly and factorisation always use the LS operator.
+  int find_phi = 1;
   void entries(int64_t n, const int64_t* I, const int64_t* J, double2* out);
+  // the entry context of entries() and the dense product (find_phi applied)
+  EntryCtx ctx_phi() const {
+    EntryCtx c = ctx;
+    c.find_phi = find_phi;
+    return c;
+  }
   int well_sep(int j, int k, int l) const;
   size_t operator_bytes() const { return op.used(); }
   size_t device_bytes() const;  // device memory held (arena chunks)

@@ -77,6 +77,9 @@ class userkernel {
   std::vector<pts2D> gridPoints;
   Vec rhs;  // incident-field RHS, LS_functions.hpp:113-120 (theta = 0)
   std::vector<pts2D> particles_X, particles_Y;
+  // LS2DTree::findPhi (LS_Assembly.hpp:195): true, getMatrixEntry and friends
+  // return entries of I + kappa^2 q G; false, of the volume potential G
+  bool findPhi = true;
 
   // kernel.hpp:106-128.  treeAdaptivity and the particle lists are accepted
   // for source compatibility (uniform tree only).
@@ -118,6 +121,7 @@ class userkernel {
   dtype getMatrixEntry(const unsigned i, const unsigned j) {
     int64_t I = i, J = j;
     double v[2];
+    aifmm::check(aifmm_ls_set_find_phi(h_, findPhi ? 1 : 0));
     aifmm::check(aifmm_ls_entries(h_, 1, &I, &J, v));
     return dtype(v[0], v[1]);
   }
@@ -151,6 +155,7 @@ class userkernel {
     Vec out(I.size());
     if (I.empty()) return out;
     std::vector<double> v(2 * I.size());
+    aifmm::check(aifmm_ls_set_find_phi(h_, findPhi ? 1 : 0));
     aifmm::check(aifmm_ls_entries(h_, (int64_t)I.size(), I.data(), J.data(), v.data()));
     for (size_t t = 0; t < I.size(); ++t) out(t) = dtype(v[2 * t], v[2 * t + 1]);
     return out;

@@ -114,6 +114,20 @@ int aifmm_ls_set_refinement(aifmm_ls_problem* h, int method);
  * one, as getMatrixEntry; unzeroed far entries below 1e-16 change a row sum
  * by less than 1e-16 |x_j| each.  Host buffers, N x nrhs column-major. */
 int aifmm_ls_dense_matvec(aifmm_ls_problem* h, int nrhs, const double* x, double* y);
+/* LS2DTree::findPhi (LS_Assembly.hpp:195, 507-512, 530-535) for
+ * aifmm_ls_entries and the dense products: 1 (default), entries of the LS
+ * operator I + kappa^2 q G; 0, of the volume potential G alone, which maps the
+ * solution phi to the scattered field (hybrid.cpp:264-281 sets it to false for
+ * that product).  Assembly and factorisation always use the LS operator. */
+int aifmm_ls_set_find_phi(aifmm_ls_problem* h, int on);
+/* userKernel::IncidenceFunction at the points (LS_functions.hpp:113-116):
+ * u_i = exp(i kappa d.x_i), d = (cos theta, sin theta); N interleaved complex. */
+int aifmm_ls_incident_field(aifmm_ls_problem* h, double theta, double* u);
+/* The total field from a solution phi of the plane wave at theta
+ * (hybrid.cpp:264-281): u = u_inc + G phi, with the dense volume-potential
+ * product (findPhi false, every entry as getMatrixEntry); host buffers, N
+ * interleaved complex each. */
+int aifmm_ls_total_field(aifmm_ls_problem* h, double theta, const double* phi, double* u);
 /* The dense product of aifmm_ls_dense_matvec without device allocation.  No
  * counterpart in the reference: for callers that cannot allocate device memory
  * (code under HIP graph capture, tight serving loops, harnesses that forbid

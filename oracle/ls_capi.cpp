@@ -78,6 +78,9 @@ int orc_recompress_trace(void* p, int* out, int cap) {
   return (int)t.size();
 }
 
+// LS2DTree::findPhi of later entries and dense matrices (Disc::find_phi).
+void orc_set_find_phi(void* p, int on) { ((Handle*)p)->D.find_phi = on != 0; }
+
 long long orc_n(void* p) { return ((Handle*)p)->D.N; }
 
 void orc_points(void* p, double* xy, double* q) {

@@ -393,10 +393,12 @@ cd Disc::entry(int64_t i, int64_t j) const {
         if (NB_OFF[s][0] == dx && NB_OFF[s][1] == dy) nb = s;
     }
     cd e = T[nb](ii, jj);
-    e = e * P.kappa;
-    e = e * P.kappa;
-    e = e * q[i];
-    if (i == j) e = 1.0 + e;
+    if (find_phi) {  // LS_Assembly.hpp:507-512
+      e = e * P.kappa;
+      e = e * P.kappa;
+      e = e * q[i];
+      if (i == j) e = 1.0 + e;
+    }
     if (std::abs(e) < 1e-16) e = 0.0;
     return e;
   }
@@ -409,8 +411,10 @@ cd Disc::entry(int64_t i, int64_t j) const {
     s *= 1.0 + perturb * ((double)(h % 2000001ull) / 1000000.0 - 1.0);
   }
   s *= cd(0.0, 0.25);
-  s *= P.kappa * P.kappa * q[i];
-  if (i == j) s += 1.0;
+  if (find_phi) {  // LS_Assembly.hpp:530-535
+    s *= P.kappa * P.kappa * q[i];
+    if (i == j) s += 1.0;
+  }
   if (std::abs(s) < 1e-16) s = 0.0;
   return s;
 }

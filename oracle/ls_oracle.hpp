@@ -113,6 +113,10 @@ struct Disc {
   // the solution under last-bit changes of its input.
   double perturb = 0;
   uint64_t perturb_seed = 0;
+  // LS2DTree::findPhi (LS_Assembly.hpp:195, 237): true, the entries of the LS
+  // operator I + kappa^2 q G; false, those of the volume potential G alone
+  // (the scattered field from the solution, hybrid.cpp:264-281)
+  bool find_phi = true;
   void build(const Params& P);
   cd entry(int64_t i, int64_t j) const;
   cd far_sum(int64_t i, int leaf_j, int idx_j) const;

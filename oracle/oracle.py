@@ -36,6 +36,7 @@ def lib():
         L.orc_create.argtypes = [ip, ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_int]
         L.orc_destroy.argtypes = [P]
         L.orc_set_perturb.argtypes = [P, ctypes.c_double, ctypes.c_ulonglong]
+        L.orc_set_find_phi.argtypes = [P, ctypes.c_int]
         L.orc_set_piv_eps.argtypes = [P, ctypes.c_double, ctypes.c_double]
         L.orc_zero_pivots.restype = ctypes.c_int
         L.orc_zero_pivots.argtypes = [P, ip, ctypes.c_int]
@@ -104,6 +105,20 @@ class Problem:
         out = np.zeros(max(n, 1), np.int32)
         lib().orc_zero_pivots(self.h, out, n)
         return out[:n].reshape(-1, 4).copy()
+
+    def set_find_phi(self, on):
+        """LS2DTree::findPhi for later entries() / dense(): True (default) the LS
+        operator I + kappa^2 q G, False the volume potential G alone."""
+        lib().orc_set_find_phi(self.h, int(bool(on)))
+
+    def incident(self, theta=0.0):
+        """The incident plane wave exp(i kappa d.x) at the points, d = (cos th,
+        sin th) (IncidenceFunction, LS_functions.hpp:113-116, at th = 0)."""
+        xy, _ = self.points()
+        k = self.kappa
+        if theta == 0.0:
+            return np.exp(1j * k * xy[:, 0])
+        return np.exp(1j * k * (np.cos(theta) * xy[:, 0] + np.sin(theta) * xy[:, 1]))
 
     def recompress_trace(self):
         """Recompressions of the last colour-schedule solve, one row each:

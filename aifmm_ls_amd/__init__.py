@@ -38,6 +38,9 @@ def lib():
         P = ctypes.c_void_p
         dp = ctypes.POINTER(ctypes.c_double)
         L.aifmm_ls_create.argtypes = [ctypes.POINTER(_Params), ctypes.POINTER(ctypes.c_void_p)]
+        L.aifmm_ls_create_cached.argtypes = [ctypes.POINTER(_Params), ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]
+        L.aifmm_ls_write_precomputations.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
+        L.aifmm_ls_table.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.aifmm_ls_destroy.argtypes = [P]
         L.aifmm_ls_last_error.restype = ctypes.c_char_p
         L.aifmm_ls_size.restype = ctypes.c_int64
@@ -153,11 +156,14 @@ class LSProblem:
 
     Arguments follow the reference driver's argv (testFMM2D_find_error.cpp:44-52):
     n_cheb, kappa, degree, L, qchoice, tol_pow, n_levels; qchoice 6 takes
-    ``bumps`` = rows of (a, w, cx, cy).
+    ``bumps`` = rows of (a, w, cx, cy).  ``precomputations`` = a directory
+    holding the reference's precomputations/ text cache: the tables are read
+    from it instead of computed (LS_Assembly.hpp:1217-1228, 1422-1459; the
+    reference with its three precomputation lines commented out, LS/Readme.md).
     """
 
     def __init__(self, n_levels, kappa, tol_pow=8, qchoice=0, n_cheb=6, degree=6, L=1.0, bumps=None,
-                 quantize=True):
+                 quantize=True, precomputations=None):
         p = _Params()
         p.n_cheb, p.degree, p.n_levels, p.tol_pow, p.qchoice = n_cheb, degree, n_levels, tol_pow, qchoice
         p.quantize_tables = int(quantize)
@@ -168,7 +174,10 @@ class LSProblem:
             p.bumps = _dp(self._bumps)
             p.n_bumps = self._bumps.shape[0]
         h = ctypes.c_void_p()
-        _chk(lib().aifmm_ls_create(ctypes.byref(p), ctypes.byref(h)))
+        if precomputations is None:
+            _chk(lib().aifmm_ls_create(ctypes.byref(p), ctypes.byref(h)))
+        else:
+            _chk(lib().aifmm_ls_create_cached(ctypes.byref(p), os.fsencode(precomputations), ctypes.byref(h)))
         self.h = h
         self.N = int(lib().aifmm_ls_size(h))
         self.n_levels, self.kappa, self.tol_pow = n_levels, kappa, tol_pow
@@ -187,6 +196,25 @@ class LSProblem:
         q = np.zeros(self.N)
         _chk(lib().aifmm_ls_points(self.h, _dp(xy), _dp(q)))
         return xy.reshape(-1, 2), q
+
+    def write_precomputations(self, directory):
+        """Every table of the reference's precomputation (far M at levels
+        0..10; colleague, fine, separated-fine and coarse neighbour operators
+        at levels 2..n_levels) computed on the device and written as the
+        reference's text cache under directory/precomputations
+        (LS_Assembly.hpp:1166-1184, 1328-1420).  Returns the device seconds."""
+        t = ctypes.c_double()
+        _chk(lib().aifmm_ls_write_precomputations(self.h, os.fsencode(directory), ctypes.byref(t)))
+        return t.value
+
+    def table(self, family, level, n=0):
+        """One precomputation table, unrounded: family -1 the far M at level
+        (121 x R); 0..3 (colleague, fine, separated fine, coarse) the neighbour
+        operator n of a target box at level (R x R)."""
+        rows = 121 if family < 0 else self.R
+        out = np.zeros(2 * rows * self.R)
+        _chk(lib().aifmm_ls_table(self.h, family, level, n, _dp(out)))
+        return out.view(np.complex128).reshape(self.R, rows).T
 
     def tables(self):
         M = np.zeros(2 * 121 * self.R)

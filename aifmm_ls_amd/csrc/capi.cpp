@@ -90,33 +90,63 @@ extern "C" {
 
 const char* aifmm_ls_last_error(void) { return g_err.c_str(); }
 
+static aifmm_ls_problem* create_problem(const aifmm_ls_params* p, const std::string& precomp_dir) {
+  if (!p) throw std::runtime_error("null argument");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw std::runtime_error("no HIP device available");
+  Params P;
+  P.n_cheb = p->n_cheb;
+  P.degree = p->degree;
+  P.n_levels = p->n_levels;
+  P.tol_pow = p->tol_pow;
+  P.qchoice = p->qchoice;
+  P.quantize = p->quantize_tables;
+  P.kappa = p->kappa;
+  P.L = p->L;
+  if (p->bumps && p->n_bumps > 0) P.bumps.assign(p->bumps, p->bumps + 4 * p->n_bumps);
+  if (P.n_cheb < 1 || P.degree < 1 || P.degree * (P.degree + 1) / 2 > P.n_cheb * P.n_cheb)
+    throw std::runtime_error("invalid n_cheb/degree");
+  auto* h = new aifmm_ls_problem;
+  try {
+    h->pb.reset(new Problem(P, precomp_dir));
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  h->io.set_chunk(Arena::chunk_for(h->pb->N, 256, 4 << 20, 64 << 20));
+  h->sc.set_chunk(Arena::chunk_for(h->pb->N, 64, 4 << 20, 16 << 20));
+  return h;
+}
+
 int aifmm_ls_create(const aifmm_ls_params* p, aifmm_ls_problem** out) {
   GUARD({
-    if (!p || !out) throw std::runtime_error("null argument");
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw std::runtime_error("no HIP device available");
-    Params P;
-    P.n_cheb = p->n_cheb;
-    P.degree = p->degree;
-    P.n_levels = p->n_levels;
-    P.tol_pow = p->tol_pow;
-    P.qchoice = p->qchoice;
-    P.quantize = p->quantize_tables;
-    P.kappa = p->kappa;
-    P.L = p->L;
-    if (p->bumps && p->n_bumps > 0) P.bumps.assign(p->bumps, p->bumps + 4 * p->n_bumps);
-    if (P.n_cheb < 1 || P.degree < 1 || P.degree * (P.degree + 1) / 2 > P.n_cheb * P.n_cheb)
-      throw std::runtime_error("invalid n_cheb/degree");
-    auto* h = new aifmm_ls_problem;
-    try {
-      h->pb.reset(new Problem(P));
-    } catch (...) {
-      delete h;
-      throw;
-    }
-    h->io.set_chunk(Arena::chunk_for(h->pb->N, 256, 4 << 20, 64 << 20));
-    h->sc.set_chunk(Arena::chunk_for(h->pb->N, 64, 4 << 20, 16 << 20));
-    *out = h;
+    if (!out) throw std::runtime_error("null argument");
+    *out = create_problem(p, std::string());
+  })
+}
+
+int aifmm_ls_create_cached(const aifmm_ls_params* p, const char* dir, aifmm_ls_problem** out) {
+  GUARD({
+    if (!out || !dir || !*dir) throw std::runtime_error("null argument");
+    *out = create_problem(p, dir);
+  })
+}
+
+int aifmm_ls_write_precomputations(aifmm_ls_problem* h, const char* dir, double* seconds) {
+  GUARD({
+    if (!h || !dir || !*dir) throw std::runtime_error("null argument");
+    double t = h->pb->write_precomputations(dir);
+    if (seconds) *seconds = t;
+  })
+}
+
+int aifmm_ls_table(aifmm_ls_problem* h, int family, int level, int n, double* out) {
+  GUARD({
+    if (!h || !out) throw std::runtime_error("null argument");
+    if (family < -1 || family > 3) throw std::runtime_error("family must be -1 (far) or 0..3 (neighbours)");
+    if (level < 0 || level > aifmm::FAR_LEVELS) throw std::runtime_error("level out of range");
+    auto t = h->pb->quad_tables({aifmm::TableSpec{family, level, n}}, nullptr);
+    std::memcpy(out, t[0].data(), t[0].size() * 16);
   })
 }
 

@@ -7,14 +7,12 @@
 // accepted panels in depth-first order.
 //   far  C(n,l)   = int_{[-1,1]^2} J_n(kappa r |s|) e^{-i n theta_s} T_a(s1) T_b(s2) ds
 //                   (FarFieldInteraction.hpp:62-74)
-//   near V_nb(p,l) = int_{[-1,1]^2} H_0(kappa r |X_p - s|) T_a(s1) T_b(s2) ds,
-//                   X_p = xi_p - N_centers[nb] (NearFieldInteraction1.hpp:98-128)
+//   near V(p,l)   = int_{[-1,1]^2} H_0(kappa r_B |X_p - s|) T_a(s1) T_b(s2) ds,
+//                   X_p = (r_A xi_p - c_B) / r_B (NearFieldInteraction1.hpp:98-128)
 #include "common.h"
 #include "kernels.h"
 
 namespace aifmm {
-
-__constant__ double c_ncent[9][2] = {{-2, -2}, {0, -2}, {2, -2}, {2, 0}, {2, 2}, {0, 2}, {-2, 2}, {-2, 0}, {0, 0}};
 
 // J_n(z) for one order n >= 0 by Miller's backward recurrence, normalised by
 // J_0 + 2 sum J_2k = 1 (power-of-two rescaling only).
@@ -47,10 +45,11 @@ __device__ __forceinline__ double2 integrand(const QuadCtx& Q, const QuadTask& t
 #pragma clang fp contract(off)
   int a = Q.basis_a[t.l], b = Q.basis_b[t.l];
   double ta = chebT(x, a), tb = chebT(y, b);
+  const QuadGeom& G = Q.geom[t.g];
   if (t.kind == 0) {
     double R2 = x * x + y * y;
     double nx = sqrt(R2);
-    double arg = Q.kr * nx;
+    double arg = G.kr * nx;
     double th = atan2(y, x);
     int an = t.n < 0 ? -t.n : t.n;
     double jv = besselJn_single(an, arg);
@@ -62,11 +61,13 @@ __device__ __forceinline__ double2 integrand(const QuadCtx& Q, const QuadTask& t
     v = cscale(v, tb);
     return v;
   } else {
-    double Xx = Q.cheb1d[t.p % Q.n_cheb] - c_ncent[t.n][0];
-    double Xy = Q.cheb1d[t.p / Q.n_cheb] - c_ncent[t.n][1];
+    // the target node r_A xi + 0 (box A at the origin), then
+    // (node - c_B) / r_B, in the reference's order of operations
+    double Xx = (G.ra * Q.cheb1d[t.p % Q.n_cheb] + 0.0 - G.cx) / G.rb;
+    double Xy = (G.ra * Q.cheb1d[t.p / Q.n_cheb] + 0.0 - G.cy) / G.rb;
     double R2 = (Xx - x) * (Xx - x) + (Xy - y) * (Xy - y);
     double R = sqrt(R2);
-    double arg = Q.kr * R;
+    double arg = G.kr * R;
     double2 v = mk(j0(arg), y0(arg));
     v = cscale(v, ta);
     v = cscale(v, tb);

@@ -49,18 +49,29 @@ void launch_dense_matvec(const EntryCtx& C, int64_t N, const double2* x, double2
                          hipStream_t s, bool profile = true);
 
 // ----------------------------------------------------------- quadrature
+// One integral per task.  kind 0: far C(n, l) = int J_n(kr |s|) e^{-i n th_s}
+// T_l(s) ds; kind 1: near V(p, l) = int H_0(kr |X_p - s|) T_l(s) ds with the
+// target X_p = (ra xi_p - c) / rb of a box of radius ra centred at the origin
+// seen from a source box of radius rb centred at c (NearFieldInteraction1.hpp:
+// 98-128).  g indexes the geometry table.
 struct QuadTask {
-  int kind;  // 0: far C(n,l) ; 1: near V(p,l) for colleague slot nb
-  int n;     // Bessel order (far) or neighbour slot (near)
+  int kind;  // 0: far ; 1: near
+  int n;     // Bessel order (far)
   int p;     // target Chebyshev point (near)
   int l;     // basis index
+  int g;     // geometry
+};
+struct QuadGeom {
+  double kr;      // kappa * r (far: the box radius; near: the source radius rb)
+  double ra, rb;  // near: target and source box radii
+  double cx, cy;  // near: source box centre (target box at the origin)
 };
 struct QuadCtx {
   const int* basis_a;
   const int* basis_b;
   const double* cheb1d;
+  const QuadGeom* geom;
   int n_cheb;
-  double kr;  // kappa * r_leaf
   double eps;
   const double* gl_t;  // 10 nodes
   const double* gl_w;  // 10 weights

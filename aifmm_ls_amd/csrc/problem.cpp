@@ -133,10 +133,21 @@ static double round6(double v) {
   return strtod(buf, nullptr);
 }
 
-Problem::Problem(const Params& Pin) : P(Pin) {
+Problem::Problem(const Params& Pin, const std::string& precomp_dir) : P(Pin) {
   // non-blocking: no implicit ordering with the null stream, so problems on
   // different host threads overlap on the device
   HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  try {
+    init(precomp_dir);
+  } catch (...) {
+    (void)hipStreamSynchronize(stream);
+    (void)hipStreamDestroy(stream);
+    stream = nullptr;
+    throw;
+  }
+}
+
+void Problem::init(const std::string& precomp_dir) {
   R = P.n_cheb * P.n_cheb;
   nbasis = P.degree * (P.degree + 1) / 2;
   L = P.n_levels;
@@ -182,7 +193,10 @@ Problem::Problem(const Params& Pin) : P(Pin) {
   d_iota = persist.get<int64_t>(N);
   h2d(d_iota, iota.data(), N * 8, stream);
   build_lists();
-  build_tables();
+  if (precomp_dir.empty())
+    build_tables();
+  else
+    load_tables(precomp_dir);
 }
 
 Problem::~Problem() {
@@ -267,13 +281,37 @@ static std::vector<double> q_pinv(int n_cheb, const std::vector<int>& ba, const 
   return out;
 }
 
-// Far-field M = C Q^+ and colleague operators T_nb = (i r^2/4) V_nb Q^+ at the
-// leaf level (FarFieldInteraction.hpp:115-160; LS_Assembly.hpp:1290-1317).
-// The integrals run on the device; the 6-significant-digit rounding of the
-// reference's text cache (LS_Assembly.hpp:1166-1228, 1383-1459) is applied
-// to the finished tables.
-void Problem::build_tables() {
-  double t0 = now_s();
+// Source-box centres of the four neighbour families, in units of the target
+// box radius r_A, target box at the origin (LS_Assembly.hpp:73-120): colleague
+// (N_centers), fine (FN_centers), separated fine (SFN_centers) and coarse
+// (CN_centers) neighbours.  The source box level is level_A + dlevel
+// (LS_Assembly.hpp:1334-1367).
+static const double FAM_C0[9][2] = {{-2, -2}, {0, -2}, {2, -2}, {2, 0}, {2, 2}, {0, 2}, {-2, 2}, {-2, 0}, {0, 0}};
+static const double FAM_C1[12][2] = {{-1.5, -1.5}, {-0.5, -1.5}, {0.5, -1.5}, {1.5, -1.5}, {1.5, -0.5}, {1.5, 0.5},
+                                     {1.5, 1.5},   {0.5, 1.5},   {-0.5, 1.5}, {-1.5, 1.5}, {-1.5, 0.5}, {-1.5, -0.5}};
+static const double FAM_C2[20][2] = {{-2.5, -2.5}, {-1.5, -2.5}, {-0.5, -2.5}, {0.5, -2.5}, {1.5, -2.5},
+                                     {2.5, -2.5},  {2.5, -1.5},  {2.5, -0.5},  {2.5, 0.5},  {2.5, 1.5},
+                                     {2.5, 2.5},   {1.5, 2.5},   {0.5, 2.5},   {-0.5, 2.5}, {-1.5, 2.5},
+                                     {-2.5, 2.5},  {-2.5, 1.5},  {-2.5, 0.5},  {-2.5, -0.5}, {-2.5, -1.5}};
+static const double FAM_C3[12][2] = {{-3, -3}, {-1, -3}, {1, -3}, {3, -3}, {3, -1}, {3, 1},
+                                     {3, 3},   {1, 3},   {-1, 3}, {-3, 3}, {-3, 1}, {-3, -1}};
+const int FAMILY_SIZE[4] = {9, 12, 20, 12};
+static const int FAMILY_DLEVEL[4] = {0, 1, 1, -1};
+static const double (*const FAMILY_CENTRES[4])[2] = {FAM_C0, FAM_C1, FAM_C2, FAM_C3};
+
+double Problem::box_radius(int j) const {  // boxRadius[j] (LS_Assembly.hpp:886-889)
+  double r = P.L;
+  for (int i = 0; i < j; ++i) r = r / 2.0;
+  return r;
+}
+
+// The tables named by specs, unrounded, as computed by the reference's
+// precomputation (FarFieldInteraction.hpp:115-160; LS_Assembly.hpp:1290-1317):
+//   far  M_j = (r_j^2 C_j) Q^+                       NTERM x R
+//   near T   = (i r_B^2 / 4) V Q^+                   R x R
+// The integrals run on the device, one wave per integral; Q^+ and the
+// products follow on the device.  Column-major results, one per spec.
+std::vector<std::vector<double2>> Problem::quad_tables(const std::vector<TableSpec>& specs, int* capped) {
   std::vector<int> ba, bb;
   for (int d = 0; d < P.degree; ++d)
     for (int a = 0; a < d + 1; ++a) ba.push_back(a), bb.push_back(d - a);
@@ -281,67 +319,108 @@ void Problem::build_tables() {
   for (int k = 0; k < P.n_cheb; ++k) cheb[k] = -cos((k + 0.5) / P.n_cheb * PI);
   gauss_legendre10(glx, glw);
   std::vector<QuadTask> tasks;
-  for (int n = -SERIES; n <= SERIES; ++n)
-    for (int l = 0; l < nbasis; ++l) tasks.push_back(QuadTask{0, n, 0, l});
-  int nfar = (int)tasks.size();
-  for (int nb = 0; nb < 9; ++nb)
-    for (int p = 0; p < R; ++p)
-      for (int l = 0; l < nbasis; ++l) tasks.push_back(QuadTask{1, nb, p, l});
+  std::vector<QuadGeom> geo;
+  std::vector<size_t> first(specs.size());
+  for (size_t s = 0; s < specs.size(); ++s) {
+    const TableSpec& t = specs[s];
+    first[s] = tasks.size();
+    int g = (int)geo.size();
+    if (t.family < 0) {
+      if (t.level < 0) throw std::runtime_error("far table level < 0");
+      geo.push_back(QuadGeom{P.kappa * box_radius(t.level), 0, 0, 0, 0});
+      for (int n = -SERIES; n <= SERIES; ++n)
+        for (int l = 0; l < nbasis; ++l) tasks.push_back(QuadTask{0, n, 0, l, g});
+    } else {
+      if (t.family > 3 || t.n < 0 || t.n >= FAMILY_SIZE[t.family]) throw std::runtime_error("no such neighbour table");
+      int lb = t.level + FAMILY_DLEVEL[t.family];
+      if (t.level < 0 || lb < 0) throw std::runtime_error("neighbour table level out of range");
+      double ra = box_radius(t.level), rb = box_radius(lb);
+      const double* c = FAMILY_CENTRES[t.family][t.n];
+      geo.push_back(QuadGeom{P.kappa * rb, ra, rb, c[0] * ra, c[1] * ra});
+      for (int p = 0; p < R; ++p)
+        for (int l = 0; l < nbasis; ++l) tasks.push_back(QuadTask{1, 0, p, l, g});
+    }
+  }
   QuadCtx Q;
   Q.basis_a = upload(scratch, ba, stream);
   Q.basis_b = upload(scratch, bb, stream);
   Q.cheb1d = upload(scratch, cheb, stream);
   Q.gl_t = upload(scratch, glx, stream);
   Q.gl_w = upload(scratch, glw, stream);
+  Q.geom = upload(scratch, geo, stream);
   Q.n_cheb = P.n_cheb;
-  Q.kr = P.kappa * rleaf;
-  Q.eps = 1e-7;
+  Q.eps = 1e-7;  // FarFieldInteraction.hpp:37, NearFieldInteraction1.hpp:37, 112
   QuadTask* dt = upload(scratch, tasks, stream);
   double2* dint = scratch.get<double2>(tasks.size());
   int* dflag = scratch.get<int>(1);
   HIP_OK(hipMemsetAsync(dflag, 0, 4, stream));
   launch_quadrature(Q, (int)tasks.size(), dt, dint, dflag, stream);
-  // M = (r^2 C) Q^+ ;  T_nb = (i r^2 / 4) V_nb Q^+
   std::vector<double> qp = q_pinv(P.n_cheb, ba, bb);
   std::vector<double2> qpc(qp.size());
   for (size_t i = 0; i < qp.size(); ++i) qpc[i] = make_double2(qp[i], 0.0);
   Blk<double2> QP{upload(scratch, qpc, stream), nbasis, R, nbasis};
-  // far tasks are laid out [n][l]: column-major C^T (nbasis x NTERM)
-  Blk<double2> CT{dint, nbasis, NTERM, nbasis};
-  Blk<double2> Md = new_blk<double2>(scratch, NTERM, R);
-  GemmBatch<double2> g;
-  double r2 = rleaf * rleaf;
-  Blk<double2> Cm = new_blk<double2>(scratch, NTERM, nbasis);
+  // far tasks are laid out [n][l]: column-major C^T (nbasis x NTERM); near
+  // tasks [p][l]: V^T (nbasis x R).  Transpose, then multiply by Q^+.
   CopyBatch<double2> cp;
-  cp.add(CT, Cm, 2);
-  cp.run(scratch, stream);
-  g.add(Cm, 0, QP, 0, Md, make_double2(r2, 0), make_double2(0, 0));
-  Blk<double2> Td = new_blk<double2>(scratch, R, 9 * R);
-  std::vector<Blk<double2>> Vm(9);
-  for (int nb = 0; nb < 9; ++nb) {
-    // near tasks for slot nb are [p][l] -> V^T (nbasis x R) column-major
-    Blk<double2> VT{dint + nfar + (size_t)nb * R * nbasis, nbasis, R, nbasis};
-    Vm[nb] = new_blk<double2>(scratch, R, nbasis);
-    cp.add(VT, Vm[nb], 2);
+  GemmBatch<double2> g;
+  std::vector<Blk<double2>> outb(specs.size());
+  for (size_t s = 0; s < specs.size(); ++s) {
+    bool far = specs[s].family < 0;
+    int rows = far ? NTERM : R;
+    Blk<double2> XT{dint + first[s], nbasis, rows, nbasis};
+    Blk<double2> X = new_blk<double2>(scratch, rows, nbasis);
+    cp.add(XT, X, 2);
+    outb[s] = new_blk<double2>(scratch, rows, R);
+    double r;
+    if (far) {
+      r = box_radius(specs[s].level);
+      g.add(X, 0, QP, 0, outb[s], make_double2(r * r, 0), make_double2(0, 0));
+    } else {
+      r = box_radius(specs[s].level + FAMILY_DLEVEL[specs[s].family]);
+      g.add(X, 0, QP, 0, outb[s], make_double2(0, r * r / 4.0), make_double2(0, 0));
+    }
   }
   cp.run(scratch, stream);
-  for (int nb = 0; nb < 9; ++nb) g.add(Vm[nb], 0, QP, 0, Td.sub(0, nb * R, R, R), make_double2(0, r2 / 4.0), make_double2(0, 0));
   g.run(scratch, stream);
-  M_h.resize((size_t)NTERM * R);
-  T_h.resize((size_t)9 * R * R);
-  d2h(M_h.data(), Md.p, M_h.size() * 16, stream);
-  d2h(T_h.data(), Td.p, T_h.size() * 16, stream);
-  d2h(&quad_capped, dflag, 4, stream);
+  std::vector<std::vector<double2>> out(specs.size());
+  for (size_t s = 0; s < specs.size(); ++s) {
+    out[s].resize(outb[s].size());
+    d2h(out[s].data(), outb[s].p, out[s].size() * 16, stream);
+  }
+  int cap = 0;
+  d2h(&cap, dflag, 4, stream);
   HIP_OK(hipStreamSynchronize(stream));
+  scratch.reset();
+  if (capped) *capped = cap;
+  return out;
+}
+
+// Far-field M and colleague operators T_nb at the leaf level, the tables the
+// uniform tree reads (LS_Assembly.hpp:496, 527).  With quantize, the
+// 6-significant-digit rounding of the reference's text cache
+// (LS_Assembly.hpp:1166-1228, 1383-1459) is applied to the finished tables.
+void Problem::build_tables() {
+  double t0 = now_s();
+  std::vector<TableSpec> specs{{-1, L, 0}};
+  for (int nb = 0; nb < 9; ++nb) specs.push_back({0, L, nb});
+  auto tabs = quad_tables(specs, &quad_capped);
+  M_h = std::move(tabs[0]);
+  T_h.clear();
+  for (int nb = 0; nb < 9; ++nb) T_h.insert(T_h.end(), tabs[1 + nb].begin(), tabs[1 + nb].end());
   if (P.quantize) {
     for (auto& v : M_h) v = make_double2(round6(v.x), round6(v.y));
     for (auto& v : T_h) v = make_double2(round6(v.x), round6(v.y));
   }
+  install_tables();
+  t_tables = now_s() - t0;
+}
+
+// M_h and T_h to the device and into the entry context.
+void Problem::install_tables() {
   dM = persist.get<double2>(M_h.size());
   dT = persist.get<double2>(T_h.size());
   h2d(dM, M_h.data(), M_h.size() * 16, stream);
   h2d(dT, T_h.data(), T_h.size() * 16, stream);
-  scratch.reset();
   ctx.px = dpx;
   ctx.py = dpy;
   ctx.q = dq;
@@ -352,7 +431,6 @@ void Problem::build_tables() {
   ctx.kappa = P.kappa;
   ctx.rleaf = rleaf;
   ctx.Ldom = P.L;
-  t_tables = now_s() - t0;
 }
 
 void Problem::entries(int64_t n, const int64_t* I, const int64_t* J, double2* out) {

@@ -3,6 +3,7 @@
 // FP64 hierarchical operator (NCA skeletons, L2P/P2M, M2L, leaf P2P).
 #pragma once
 #include <map>
+#include <string>
 #include <vector>
 
 #include "batch.h"
@@ -14,6 +15,15 @@ struct Params {
   double kappa = 10.0, L = 1.0;
   std::vector<double> bumps;  // a, w, cx, cy per bump (qchoice 6)
 };
+
+// One precomputation table: family -1 the far M at level (NTERM x R);
+// 0..3 the colleague / fine / separated-fine / coarse neighbour operator n of
+// a target box at level (R x R) (LS_Assembly.hpp:1328-1367).
+struct TableSpec {
+  int family, level, n;
+};
+extern const int FAMILY_SIZE[4];
+constexpr int FAR_LEVELS = 10;  // M is precomputed for levels 0..10 (FarFieldInteraction.hpp:38)
 
 struct Lists {
   int nb[8], inner[16], outer[24];
@@ -32,7 +42,10 @@ void gauss_legendre10(std::vector<double>& x, std::vector<double>& w);
 
 class Problem {
  public:
-  explicit Problem(const Params& P);
+  // precomp_dir empty: tables by device quadrature; else the tables are read
+  // from the precomputation cache under precomp_dir (the reference's
+  // precomputations/ text files; LS_Assembly.hpp:1217-1228, 1422-1459)
+  explicit Problem(const Params& P, const std::string& precomp_dir = std::string());
   ~Problem();
   Params P;
   int R = 36, nbasis = 21, L = 5, nleaf = 0;
@@ -58,6 +71,15 @@ class Problem {
   hipStream_t stream = nullptr;
 
   void build_tables();
+  void load_tables(const std::string& dir);
+  void install_tables();
+  double box_radius(int j) const;
+  std::vector<std::vector<double2>> quad_tables(const std::vector<TableSpec>& specs, int* capped);
+  // every table of the reference's precomputation (M at levels 0..10, the 53
+  // neighbour operators at levels 2..L) in the text layout under dir;
+  // returns the seconds spent on quadrature
+  double write_precomputations(const std::string& dir);
+  double t_precompute = 0;
   void build_lists();
   void assemble();
   // LS2DTree::findPhi of entries() and the dense product (LS_Assembly.hpp:195,
@@ -76,6 +98,7 @@ class Problem {
   size_t device_bytes() const;  // device memory held (arena chunks)
 
  private:
+  void init(const std::string& precomp_dir);
   void nca_level(int j);
 };
 

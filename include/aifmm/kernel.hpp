@@ -12,6 +12,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdint>
+#include <cstdlib>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -68,6 +69,32 @@ inline std::vector<double>& contrast_bumps() {
 }
 inline void set_contrast_bumps(const std::vector<double>& rows_a_w_cx_cy) { contrast_bumps() = rows_a_w_cx_cy; }
 
+// Where the tables come from (LS_Assembly.hpp:262-275).  PRECOMP_COMPUTE: GPU
+// quadrature, no files.  PRECOMP_WRITE: the same, and every table written as
+// precomputations/ under precomp_dir(), as the reference's ctor does.
+// PRECOMP_READ: read from precomputations/ under precomp_dir(), as the
+// reference does with its three write lines commented out (LS/Readme.md:1-5).
+// The default comes from the environment variable AIFMM_PRECOMPUTATIONS
+// ("write" or "read"; anything else: compute) and the directory from
+// AIFMM_PRECOMPUTATIONS_DIR (default: the working directory).
+enum { PRECOMP_COMPUTE = 0, PRECOMP_WRITE = 1, PRECOMP_READ = 2 };
+inline int& precomp_mode() {
+  static int m = [] {
+    const char* e = std::getenv("AIFMM_PRECOMPUTATIONS");
+    if (!e) return (int)PRECOMP_COMPUTE;
+    std::string v(e);
+    return v == "write" ? (int)PRECOMP_WRITE : v == "read" ? (int)PRECOMP_READ : (int)PRECOMP_COMPUTE;
+  }();
+  return m;
+}
+inline std::string& precomp_dir() {
+  static std::string d = [] {
+    const char* e = std::getenv("AIFMM_PRECOMPUTATIONS_DIR");
+    return std::string(e && *e ? e : ".");
+  }();
+  return d;
+}
+
 }  // namespace aifmm
 
 class userkernel {
@@ -99,7 +126,13 @@ class userkernel {
     const auto& bumps = aifmm::contrast_bumps();
     p.bumps = bumps.empty() ? nullptr : bumps.data();
     p.n_bumps = (int)(bumps.size() / 4);
-    aifmm::check(aifmm_ls_create(&p, &h_));
+    if (aifmm::precomp_mode() == aifmm::PRECOMP_READ) {
+      aifmm::check(aifmm_ls_create_cached(&p, aifmm::precomp_dir().c_str(), &h_));
+    } else {
+      aifmm::check(aifmm_ls_create(&p, &h_));
+      if (aifmm::precomp_mode() == aifmm::PRECOMP_WRITE)
+        aifmm::check(aifmm_ls_write_precomputations(h_, aifmm::precomp_dir().c_str(), nullptr));
+    }
     N = (int)aifmm_ls_size(h_);
     nLevelsResolve = nLevelsUniform;
     std::vector<double> xy(2 * (size_t)N), q(N);

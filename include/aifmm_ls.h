@@ -33,6 +33,24 @@ typedef struct {
 /* userkernel ctor + LS2DTree ctor (kernel.hpp:106-128, LS_Assembly.hpp:234-277):
  * points, contrast, far/near tables by device quadrature. */
 int aifmm_ls_create(const aifmm_ls_params* p, aifmm_ls_problem** out);
+/* The same with the tables read from the precomputation text cache under dir
+ * (dir/precomputations/{M,Neighbor}/...; readMFromFile + readNeighborInteractions,
+ * LS_Assembly.hpp:1217-1228, 1422-1459): the reference's ctor with its three
+ * precomputation lines commented out (LS_Assembly.hpp:269-272, LS/Readme.md:1-5).
+ * Every file a tree of n_levels levels reads must exist and have its shape; a
+ * cache whose aifmm_key file names another kappa, L, n_cheb or degree is refused. */
+int aifmm_ls_create_cached(const aifmm_ls_params* p, const char* dir, aifmm_ls_problem** out);
+/* evaluatePrecomputations + writeMToFile + getNeighborInteractions
+ * (LS_Assembly.hpp:269-272, 1166-1184, 1320-1420): far M at levels 0..10 and the
+ * 53 neighbour operators (9 colleague, 12 fine, 20 separated fine, 12 coarse)
+ * at levels 2..n_levels by device quadrature, written as the reference's text
+ * cache under dir; *seconds (may be NULL) = time of the quadrature batch. */
+int aifmm_ls_write_precomputations(aifmm_ls_problem* h, const char* dir, double* seconds);
+/* One precomputation table, unrounded (FarFieldInteraction::evaluateM for
+ * family -1, LS2DTree::getOperator for families 0..3 = colleague, fine,
+ * separated fine, coarse; FarFieldInteraction.hpp:153-160, LS_Assembly.hpp:1290-1317):
+ * family -1: M at level (121 x R); else operator n of a target box at level (R x R). */
+int aifmm_ls_table(aifmm_ls_problem* h, int family, int level, int n, double* out);
 void aifmm_ls_destroy(aifmm_ls_problem* h);
 const char* aifmm_ls_last_error(void);
 

@@ -3,6 +3,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <stdexcept>
 
 #include "ls_oracle.hpp"
 
@@ -100,6 +101,20 @@ void orc_tables(void* p, double* M, double* T, double* Qpinv) {
     for (int n = 0; n < 9; ++n)
       std::memcpy(T + (size_t)n * h->D.rank * h->D.rank * 2, h->D.T[n].a.data(), h->D.T[n].a.size() * sizeof(cd));
   if (Qpinv) std::memcpy(Qpinv, h->D.Qpinv.a.data(), h->D.Qpinv.a.size() * sizeof(cd));
+}
+
+// One precomputation table, unrounded (Disc::table): family -1 the far M at
+// `level` (121 x rank), 0..3 a neighbour operator (rank x rank); column-major
+// interleaved complex.  Returns the number of rows, or -1 for no such table.
+int orc_table(void* p, int family, int level, int n, double* out) {
+  Handle* h = (Handle*)p;
+  try {
+    Mat t = h->D.table(family, level, n);
+    std::memcpy(out, t.a.data(), t.a.size() * sizeof(cd));
+    return t.r;
+  } catch (const std::exception&) {
+    return -1;
+  }
 }
 
 void orc_entries(void* p, long long n, const long long* I, const long long* J, double* out) {

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <functional>
+#include <stdexcept>
 
 #include "ls_oracle.hpp"
 
@@ -233,8 +234,83 @@ static double round6(double v) {
 }
 static cd round6(cd v) { return cd(round6(v.real()), round6(v.imag())); }
 
-// N_centers (LS_Assembly.hpp:94-102): source colleague offsets, in units of r.
+// Source-box centres of the neighbour families in units of the target box
+// radius, target box at the origin (LS_Assembly.hpp:73-120): N_centers
+// (colleague), FN_centers (fine), SFN_centers (separated fine), CN_centers
+// (coarse).  Source level = target level + FAM_DLEVEL (LS_Assembly.hpp:1334-1367).
 static const double NC[9][2] = {{-2, -2}, {0, -2}, {2, -2}, {2, 0}, {2, 2}, {0, 2}, {-2, 2}, {-2, 0}, {0, 0}};
+static const double FNC[12][2] = {{-1.5, -1.5}, {-0.5, -1.5}, {0.5, -1.5}, {1.5, -1.5}, {1.5, -0.5}, {1.5, 0.5},
+                                  {1.5, 1.5},   {0.5, 1.5},   {-0.5, 1.5}, {-1.5, 1.5}, {-1.5, 0.5}, {-1.5, -0.5}};
+static const double SFNC[20][2] = {{-2.5, -2.5}, {-1.5, -2.5}, {-0.5, -2.5}, {0.5, -2.5}, {1.5, -2.5},
+                                   {2.5, -2.5},  {2.5, -1.5},  {2.5, -0.5},  {2.5, 0.5},  {2.5, 1.5},
+                                   {2.5, 2.5},   {1.5, 2.5},   {0.5, 2.5},   {-0.5, 2.5}, {-1.5, 2.5},
+                                   {-2.5, 2.5},  {-2.5, 1.5},  {-2.5, 0.5},  {-2.5, -0.5}, {-2.5, -1.5}};
+static const double CNC[12][2] = {{-3, -3}, {-1, -3}, {1, -3}, {3, -3}, {3, -1}, {3, 1},
+                                  {3, 3},   {1, 3},   {-1, 3}, {-3, 3}, {-3, 1}, {-3, -1}};
+static const int FAM_SIZE[4] = {9, 12, 20, 12};
+static const int FAM_DLEVEL[4] = {0, 1, 1, -1};
+static const double (*const FAM_C[4])[2] = {NC, FNC, SFNC, CNC};
+
+// boxRadius[j] = L / 2^j (FarFieldInteraction.hpp:42-45, LS_Assembly.hpp:886-889)
+double Disc::box_radius(int j) const {
+  double r = P.L;
+  for (int i = 0; i < j; ++i) r = r / 2.0;
+  return r;
+}
+
+// One precomputation table, unrounded.
+//   family -1: far M_j = C_j Q^+, C_j(n, l) = r_j^2 int J_n(kappa r_j |s|)
+//              e^{-i n th_s} T_l(s) ds  (FarFieldInteraction.hpp:60-74, 115-135, 153-160)
+//   family 0..3, neighbour n: T = (V i r_B^2 / 4) Q^+, V(p, l) = int H_0(kappa
+//              r_B |X_p - s|) T_l(s) ds with X_p = (r_A xi_p + 0 - c_B) / r_B and
+//              c_B = centre * r_A (NearFieldInteraction1.hpp:98-128,
+//              LS_Assembly.hpp:1290-1317, 1328-1367)
+Mat Disc::table(int family, int level, int n) const {
+  Quad quad;
+  const double eps = 1e-7;  // FarFieldInteraction.hpp:37, NearFieldInteraction1.hpp:37,112
+  if (family < 0) {
+    const double rj = box_radius(level);
+    const double kr = P.kappa * rj;
+    Mat C(2 * series + 1, nb_basis);
+#pragma omp parallel for schedule(dynamic) collapse(2)
+    for (int m = -series; m <= series; ++m)
+      for (int l = 0; l < nb_basis; ++l) {
+        int a = basis_a[l], b = basis_b[l];
+        auto f = [&](double x, double y) -> cd {
+          double R2 = x * x + y * y;
+          double normX = sqrt(R2);
+          double arg = kr * normX;
+          double th = atan2(y, x);
+          return besselJn(m, arg) * std::exp(cd(0.0, -(double)m * th)) * cheb_T(x, a) * cheb_T(y, b);
+        };
+        cd I = quad.integrate(f, eps);
+        C(m + series, l) = (rj * rj) * I;
+      }
+    return mul(C, Qpinv);
+  }
+  if (family > 3 || n < 0 || n >= FAM_SIZE[family]) throw std::runtime_error("no such neighbour table");
+  const double ra = box_radius(level), rb = box_radius(level + FAM_DLEVEL[family]);
+  const double cbx = FAM_C[family][n][0] * ra, cby = FAM_C[family][n][1] * ra;
+  const double kr = P.kappa * rb;
+  Mat V(rank, nb_basis);
+#pragma omp parallel for schedule(dynamic) collapse(2)
+  for (int p = 0; p < rank; ++p)
+    for (int l = 0; l < nb_basis; ++l) {
+      double Xx = (ra * cheb1d[p % P.n_cheb] + 0.0 - cbx) / rb;
+      double Xy = (ra * cheb1d[p / P.n_cheb] + 0.0 - cby) / rb;
+      int a = basis_a[l], b = basis_b[l];
+      auto f = [&](double x, double y) -> cd {
+        double R2 = (Xx - x) * (Xx - x) + (Xy - y) * (Xy - y);
+        double R = sqrt(R2);
+        double arg = kr * R;
+        return (cd(::j0(arg), 0.0) + cd(0.0, 1.0) * ::y0(arg)) * cheb_T(x, a) * cheb_T(y, b);
+      };
+      V(p, l) = quad.integrate(f, eps);
+    }
+  Mat Vt(rank, nb_basis);
+  for (size_t i = 0; i < V.a.size(); ++i) Vt.a[i] = V.a[i] * cd(0.0, 1.0) * rb * rb / 4.0;
+  return mul(Vt, Qpinv);
+}
 
 void Disc::build(const Params& Pin) {
   P = Pin;
@@ -293,48 +369,10 @@ void Disc::build(const Params& Pin) {
     for (int i = 0; i < nb_basis; ++i)
       for (int col = 0; col < rank; ++col) Qpinv(f.perm[i], col) = Rinv_Q1H(i, col);
   }
-  Quad quad;
-  const double eps = 1e-7;  // FarFieldInteraction.hpp:37, NearFieldInteraction1.hpp:37,112
-  const double kr = P.kappa * r_leaf;
-  // Far-field C at level L (FarFieldInteraction.hpp:62-74, 115-135)
-  Mat C(2 * series + 1, nb_basis);
-#pragma omp parallel for schedule(dynamic) collapse(2)
-  for (int n = -series; n <= series; ++n)
-    for (int l = 0; l < nb_basis; ++l) {
-      int a = basis_a[l], b = basis_b[l];
-      auto f = [&](double x, double y) -> cd {
-        double R2 = x * x + y * y;
-        double normX = sqrt(R2);
-        double arg = kr * normX;
-        double th = atan2(y, x);
-        return besselJn(n, arg) * std::exp(cd(0.0, -(double)n * th)) * cheb_T(x, a) * cheb_T(y, b);
-      };
-      cd I = quad.integrate(f, eps);
-      C(n + series, l) = (r_leaf * r_leaf) * I;
-    }
-  M = mul(C, Qpinv);
-  // Colleague near-field operators at level L (NearFieldInteraction1.hpp:98-128,
-  // LS_Assembly.hpp:1290-1317): T = (V * i * r^2/4) * Q^+.
-  for (int nb = 0; nb < 9; ++nb) {
-    Mat V(rank, nb_basis);
-#pragma omp parallel for schedule(dynamic) collapse(2)
-    for (int p = 0; p < rank; ++p)
-      for (int l = 0; l < nb_basis; ++l) {
-        double Xx = (r_leaf * cheb1d[p % P.n_cheb] - NC[nb][0] * r_leaf) / r_leaf;
-        double Xy = (r_leaf * cheb1d[p / P.n_cheb] - NC[nb][1] * r_leaf) / r_leaf;
-        int a = basis_a[l], b = basis_b[l];
-        auto f = [&](double x, double y) -> cd {
-          double R2 = (Xx - x) * (Xx - x) + (Xy - y) * (Xy - y);
-          double R = sqrt(R2);
-          double arg = kr * R;
-          return (cd(::j0(arg), 0.0) + cd(0.0, 1.0) * ::y0(arg)) * cheb_T(x, a) * cheb_T(y, b);
-        };
-        V(p, l) = quad.integrate(f, eps);
-      }
-    Mat Vt(rank, nb_basis);
-    for (size_t i = 0; i < V.a.size(); ++i) Vt.a[i] = V.a[i] * cd(0.0, 1.0) * r_leaf * r_leaf / 4.0;
-    T[nb] = mul(Vt, Qpinv);
-  }
+  // the tables the uniform tree reads: M and the colleague operators at the
+  // leaf level (LS_Assembly.hpp:496, 527)
+  M = table(-1, P.n_levels, 0);
+  for (int nb = 0; nb < 9; ++nb) T[nb] = table(0, P.n_levels, nb);
   if (P.quantize) {
     for (auto& v : M.a) v = round6(v);
     for (int nb = 0; nb < 9; ++nb)

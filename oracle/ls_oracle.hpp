@@ -118,6 +118,11 @@ struct Disc {
   // (the scattered field from the solution, hybrid.cpp:264-281)
   bool find_phi = true;
   void build(const Params& P);
+  double box_radius(int j) const;
+  // a precomputation table, unrounded: family -1 the far M at `level`
+  // (121 x rank); 0..3 the colleague / fine / separated-fine / coarse
+  // neighbour operator n of a target box at `level` (rank x rank)
+  Mat table(int family, int level, int n) const;
   cd entry(int64_t i, int64_t j) const;
   cd far_sum(int64_t i, int leaf_j, int idx_j) const;
 };

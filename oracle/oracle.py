@@ -46,6 +46,8 @@ def lib():
         L.orc_n.argtypes = [P]
         L.orc_points.argtypes = [P, dp, dp]
         L.orc_tables.argtypes = [P, dp, dp, dp]
+        L.orc_table.restype = ctypes.c_int
+        L.orc_table.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.orc_entries.argtypes = [P, ctypes.c_longlong, lp, lp, dp]
         L.orc_bessel.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, dp, dp]
         L.orc_gauss_legendre.argtypes = [ctypes.c_int, dp, dp]
@@ -148,6 +150,16 @@ class Problem:
         T = T.view(np.complex128).reshape(9, self.rank, self.rank).transpose(0, 2, 1)
         Qp = Qp.view(np.complex128).reshape(self.rank, self.nbasis).T
         return M, T, Qp
+
+    def table(self, family, level, n=0):
+        """One precomputation table, unrounded: family -1 the far M at level
+        (121 x rank); 0..3 (colleague, fine, separated fine, coarse) the
+        neighbour operator n of a target box at level (rank x rank)."""
+        rows = 121 if family < 0 else self.rank
+        out = np.zeros(rows * self.rank * 2)
+        if lib().orc_table(self.h, family, level, n, out) != rows:
+            raise ValueError(f"no table family={family} level={level} n={n}")
+        return out.view(np.complex128).reshape(self.rank, rows).T
 
     def entries(self, I, J):
         I = np.ascontiguousarray(I, np.int64)

@@ -129,3 +129,31 @@ def test_generator_then_find_error(tmp_path):
     ref = np.linalg.norm(xe - x) / np.linalg.norm(x)
     print(f"  forward error {fwd:.3e}, reference schedule {ref:.3e}", flush=True)
     assert fwd < 1.5 * ref + 1e-9
+
+
+@pytest.mark.gpu
+def test_driver_writes_then_reads_precomputations(tmp_path):
+    """The reference's ctor writes precomputations/ under the working
+    directory and its Readme reuses them by commenting the writes out
+    (LS_Assembly.hpp:262-275, LS/Readme.md:1-5); the drivers do the same under
+    AIFMM_PRECOMPUTATIONS=write / read.  Both runs print the same error."""
+    args = ["6", "10", "6", "1", "0", "8", "3"]
+    print("\n[driver] generator, then find_error writing and reading precomputations/", flush=True)
+    r = subprocess.run([GEN, *args], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = []
+    for mode in ("write", "read"):
+        env = dict(os.environ, AIFMM_PRECOMPUTATIONS=mode)
+        r = subprocess.run([DRIVER, *args], capture_output=True, text=True, cwd=tmp_path, timeout=120, env=env)
+        assert r.returncode == 0, (mode, r.stderr)
+        out.append(r.stdout.strip().splitlines()[-1])
+        md = tmp_path / "precomputations" / "M" / "M_6_10_6"
+        nd = tmp_path / "precomputations" / "Neighbor" / "Neighbor_6_10_6"
+        assert len(os.listdir(md)) == 12 and len(os.listdir(nd)) == 2 * 53 + 1
+    print(f"  {out[0]} | {out[1]}", flush=True)
+    assert out[0] == out[1]
+    # a deeper tree than the files hold: the read fails as the reference's does
+    env = dict(os.environ, AIFMM_PRECOMPUTATIONS="read")
+    r = subprocess.run([DRIVER, "6", "10", "6", "1", "0", "8", "4"], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=120, env=env)
+    assert r.returncode != 0 and "could not open" in (r.stderr + r.stdout)

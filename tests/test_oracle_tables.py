@@ -170,3 +170,124 @@ def test_quantised_tables_are_six_digit(oracle):
         for z in arr[::97]:
             for v in (z.real, z.imag):
                 assert float("%.6g" % v) == v
+
+
+# ---------------------------------------------------------------- neighbour families
+# Source boxes of each family around a target box of radius r at the origin:
+# all boxes of the family's size on one square ring, counter-clockwise from the
+# lower-left corner (colleague: same size, ring at 2r; fine: half size,
+# touching, ring at 1.5r; separated fine: half size, one fine box away, ring at
+# 2.5r; coarse: double size, touching, ring at 3r), the self box last for
+# colleagues.  Derived here from that rule, not from the oracle's tables.
+FAMILIES = {0: (2.0, 2.0, 1.0), 1: (1.5, 1.0, 0.5), 2: (2.5, 1.0, 0.5), 3: (3.0, 2.0, 2.0)}
+
+
+def ring(family):
+    d, step, _ = FAMILIES[family]
+    vals = np.arange(-d, d + 1e-9, step)
+    pts = [(x, y) for x in vals for y in vals if max(abs(x), abs(y)) == d]
+    key = [(np.arctan2(y, x) + 0.75 * np.pi) % (2 * np.pi) for x, y in pts]
+    pts = [pts[i] for i in np.argsort(key, kind="stable")]
+    if family == 0:
+        pts.append((0.0, 0.0))
+    return pts
+
+
+def test_family_sizes():
+    assert [len(ring(f)) for f in range(4)] == [9, 12, 20, 12]
+    assert ring(0)[:3] == [(-2.0, -2.0), (0.0, -2.0), (2.0, -2.0)]
+
+
+def graded_1d(c, n=16, k=12):
+    """Composite Gauss rule on [-1, 1] with panels halving towards both ends
+    and towards c (where the integrand's singularity, just off the interval
+    or off the square, comes closest)."""
+    pts = {-1.0, 1.0, -0.5, 0.5, 0.0}
+    for i in range(1, k + 1):
+        for e in (-1 + 2.0 ** -i, 1 - 2.0 ** -i, c - 2.0 ** -i, c + 2.0 ** -i):
+            if -1 < e < 1:
+                pts.add(e)
+    if -1 < c < 1:
+        pts.add(c)
+    edges = sorted(pts)
+    t, w = np.polynomial.legendre.leggauss(n)
+    xs, ws = [], []
+    for a, b in zip(edges[:-1], edges[1:]):
+        xs.append(0.5 * (b - a) * t + 0.5 * (a + b))
+        ws.append(0.5 * (b - a) * w)
+    return np.concatenate(xs), np.concatenate(ws)
+
+
+def graded_rule(X0, Y0):
+    xs, wx = graded_1d(X0)
+    ys, wy = graded_1d(Y0)
+    X, Y = np.meshgrid(xs, ys, indexing="ij")
+    return X.ravel(), Y.ravel(), np.outer(wx, wy).ravel()
+
+
+@pytest.fixture(scope="module")
+def fam_prob(oracle):
+    return oracle.Problem(n_levels=3, kappa=10.0, tol_pow=8, qchoice=0, quantize=False)
+
+
+@pytest.mark.parametrize("family", [1, 2, 3])
+def test_neighbour_family_against_direct(fam_prob, family):
+    """T = (i r_B^2 / 4) V Q^+ with V(p, l) = int H0(kappa r_B |X_p - s|) T_l(s) ds,
+    X_p = (r_A xi_p - c r_A) / r_B, by a graded tensor rule (the targets lie
+    outside the source box: the integrand is smooth)."""
+    level, ra = 3, 1.0 / 8
+    rb = ra * FAMILIES[family][2]
+    xi = cheb_nodes()
+    P = qpinv()
+    centres = ring(family)
+    for n in sorted({0, 1, len(centres) // 2, len(centres) - 1}):
+        T = fam_prob.table(family, level, n)
+        cx, cy = centres[n]
+        for p in (0, 14, 35):
+            X0 = (ra * xi[p % N_CHEB] - cx * ra) / rb
+            Y0 = (ra * xi[p // N_CHEB] - cy * ra) / rb
+            Xg, Yg, W = graded_rule(X0, Y0)
+            B = cheb_basis_vals(Xg, Yg)
+            H = sp.hankel1(0, 10.0 * rb * np.hypot(Xg - X0, Yg - Y0))
+            V = np.array([np.sum(W * H * B[l]) for l in range(len(basis()))])
+            ref = (1j * rb * rb / 4) * V @ P
+            err = np.max(np.abs(T[p] - ref))
+            assert err < 5e-9 + 2e-6 * np.max(np.abs(ref)), (family, n, p, err)
+
+
+def mirror_index(n_cheb):
+    """Node permutation of x -> -x on the tensor Chebyshev grid (index jy*n+kx)."""
+    return np.array([(i // n_cheb) * n_cheb + (n_cheb - 1 - i % n_cheb) for i in range(n_cheb * n_cheb)])
+
+
+@pytest.mark.parametrize("family", [0, 1, 2, 3])
+def test_neighbour_family_mirror_symmetry(fam_prob, family):
+    """Reflecting x -> -x maps target and source nodes by the same permutation
+    and the source centre (cx, cy) to (-cx, cy): T_c[p, q] = T_c'[p', q']."""
+    centres = ring(family)
+    perm = mirror_index(N_CHEB)
+    for n, (cx, cy) in enumerate(centres):
+        m = centres.index((-cx, cy) if cx != 0 else (0.0 * cx, cy))
+        if m < n:
+            continue
+        Tn = fam_prob.table(family, 3, n)
+        Tm = fam_prob.table(family, 3, m)
+        err = np.max(np.abs(Tn - Tm[np.ix_(perm, perm)]))
+        assert err < 1e-9 * np.max(np.abs(Tn)), (family, n, m, err)
+
+
+def test_far_tables_by_level(fam_prob):
+    """M at level j is the level-L computation with r = L / 2^j: the level-3
+    table equals the leaf table of this 3-level problem, and level 0 matches a
+    tensor rule with kr = kappa (r = 1)."""
+    M, _, _ = fam_prob.tables()
+    np.testing.assert_array_equal(fam_prob.table(-1, 3), M)
+    M0 = fam_prob.table(-1, 0)
+    X, Y, W = tensor_rule(24, 8)
+    rho, th = np.hypot(X, Y), np.arctan2(Y, X)
+    B = cheb_basis_vals(X, Y)
+    P = qpinv()
+    for n in (-60, -5, 0, 3, 40):
+        Cn = np.array([np.sum(W * sp.jv(n, 10.0 * rho) * np.exp(-1j * n * th) * B[l]) for l in range(len(basis()))])
+        err = np.max(np.abs(M0[n + 60] - Cn @ P))
+        assert err < 2e-9 + 1e-6 * np.max(np.abs(Cn @ P)), (n, err)

@@ -47,6 +47,13 @@ Blk<T> new_blk(Arena& a, int r, int c) {
 // synchronises stream s only.  Neither waits for other streams.
 void h2d(void* dst, const void* src, size_t bytes, hipStream_t s);
 void d2h(void* dst, const void* src, size_t bytes, hipStream_t s);
+// Streams of the library.  The staging rings and the profiler keep events
+// recorded on them, and a HIP event refers to the stream it was last recorded
+// on, so streams are never destroyed: stream_destroy synchronises the stream,
+// folds pending profiler events and returns it to a pool that stream_create
+// draws from.
+hipStream_t stream_create(unsigned flags);
+void stream_destroy(hipStream_t s);
 
 // Upload a host vector into a transient device buffer in the given scratch
 // arena (stream-ordered; the host vector may be released on return).

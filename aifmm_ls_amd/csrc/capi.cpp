@@ -605,8 +605,7 @@ template <class T>
 static void selftest_t(int op, int m, int n, int k, const double* A, const double* B, double* C, int* iout,
                        double thr) {
   Arena a{size_t(64) << 20}, sc{size_t(16) << 20};
-  hipStream_t s;
-  HIP_OK(hipStreamCreate(&s));
+  hipStream_t s = stream_create(hipStreamDefault);
   auto up = [&](const double* h, int r, int c) {
     Blk<double2> d = new_blk<double2>(a, r, c);
     HIP_OK(hipMemcpy(d.p, h, (size_t)r * c * 16, hipMemcpyHostToDevice));
@@ -677,7 +676,7 @@ static void selftest_t(int op, int m, int n, int k, const double* A, const doubl
     down(dB, C);
   }
   HIP_OK(hipStreamSynchronize(s));
-  (void)hipStreamDestroy(s);
+  stream_destroy(s);
 }
 
 // One ACA (NCA pivot search) on explicit point lists, as getNodes_* run it.
@@ -755,8 +754,7 @@ template <class T>
 static void selftest_prog_t(int shape, int nprog, int m, int nterms, int cols, int nrhs, const double* A,
                             const double* X, const double* S, const double* Bm, double* Y, double* Out) {
   Arena a{size_t(64) << 20}, sc{size_t(16) << 20};
-  hipStream_t s;
-  HIP_OK(hipStreamCreate(&s));
+  hipStream_t s = stream_create(hipStreamDefault);
   try {
     auto up = [&](const double* h, size_t n) {
       double2* d = a.get<double2>(n);
@@ -766,20 +764,23 @@ static void selftest_prog_t(int shape, int nprog, int m, int nterms, int cols, i
       launch_cvt<T>(1, upload(sc, cv, s), s);
       return t;
     };
-    auto down = [&](const T* t, size_t n, double* h) {
+    // vectors stay complex128 (the programs' vector type)
+    auto upv = [&](const double* h, size_t n) {
       double2* d = a.get<double2>(n);
-      if (std::is_same<T, double2>::value) HIP_OK(hipMemcpyAsync(d, t, n * 16, hipMemcpyDeviceToDevice, s));
-      else launch_f2d((const float2*)t, d, (int64_t)n, s);
+      HIP_OK(hipMemcpy(d, h, n * 16, hipMemcpyHostToDevice));
+      return d;
+    };
+    auto down = [&](const double2* d, size_t n, double* h) {
       HIP_OK(hipStreamSynchronize(s));
       HIP_OK(hipMemcpy(h, d, n * 16, hipMemcpyDeviceToHost));
     };
     const size_t mc = (size_t)m * cols, cr = (size_t)cols * nrhs, mr = (size_t)m * nrhs;
     T* dA = up(A, (size_t)nprog * nterms * mc);
-    T* dX = up(X, (size_t)nprog * nterms * cr);
-    T* dS = up(S, (size_t)nprog * mr);
+    double2* dX = upv(X, (size_t)nprog * nterms * cr);
+    double2* dS = upv(S, (size_t)nprog * mr);
     T* dB = up(Bm, (size_t)nprog * m * m);
-    T* dY = a.get<T>((size_t)nprog * mr);
-    T* dO = a.get<T>((size_t)nprog * mr);
+    double2* dY = a.get<double2>((size_t)nprog * mr);
+    double2* dO = a.get<double2>((size_t)nprog * mr);
     Program<T> P(nrhs);  // tiled as a program built for nrhs columns
     P.stage();
     for (int p = 0; p < nprog; ++p) {
@@ -803,11 +804,10 @@ static void selftest_prog_t(int shape, int nprog, int m, int nterms, int cols, i
     down(dY, (size_t)nprog * mr, Y);
     down(dO, (size_t)nprog * mr, Out);
   } catch (...) {
-    (void)hipStreamSynchronize(s);
-    (void)hipStreamDestroy(s);
+    stream_destroy(s);
     throw;
   }
-  HIP_OK(hipStreamDestroy(s));
+  stream_destroy(s);
 }
 
 extern "C" int aifmm_ls_selftest_prog(int precision, int shape, int nprog, int m, int nterms, int cols, int nrhs,
@@ -839,8 +839,7 @@ extern "C" int aifmm_ls_selftest_stream(int64_t bytes, int reps, double* ms) {
     Arena a{size_t(2 * n) * 16 + 4096};
     double2* src = a.get<double2>(n);
     double2* dst = a.get<double2>(n);
-    hipStream_t s;
-    HIP_OK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipStream_t s = stream_create(hipStreamNonBlocking);
     hipEvent_t e0, e1;
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
@@ -858,7 +857,7 @@ extern "C" int aifmm_ls_selftest_stream(int64_t bytes, int reps, double* ms) {
     if (ms) *ms = tot / reps;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(s);
+    stream_destroy(s);
   })
 }
 

@@ -687,6 +687,7 @@ void Factor<T>::factor() {
   }
   tick(4, tt);
   sweep_operators();
+  compose_sweep();
   tick(5, tt);
   dense_top_alloc();
   build_plan(1);
@@ -826,18 +827,133 @@ void Factor<T>::sweep_operators() {
   sc.reset();  // scratch reuse is ordered on the stream
 }
 
+// Couplings of the sweeps composed with the sweep operator of the box they
+// read from, formed once per factorisation:
+//   P2P[k][nn] <- P2P[k][nn] Z_nn             nn regular
+//   PU[k][nn]   = P2P[k][nn] U_nn             nn regular, colour nn > colour k
+//   P2L[nn][k] <- P2L[nn][k] Z_k              k regular, colour k > colour nn
+// (ill boxes keep their couplings: their outputs are formed in the
+// reference's order and read as they are).  Every P2P[k][nn] is read by one
+// sweep only (the forward when nn comes earlier, the back-substitution when
+// later), so the product replaces it.  The products are formed in FP64 from
+// the stored factors and rounded once to the factor's precision.
+namespace {
+template <class T>
+struct Composer {
+  // one product C = A B into dst (dst may be A)
+  struct Job {
+    Blk<T> a, b, dst;
+  };
+  std::vector<Job> jobs;
+  size_t bytes = 0;  // scratch the pending jobs take
+  void add(const Blk<T>& a, const Blk<T>& b, const Blk<T>& dst) {
+    jobs.push_back(Job{a, b, dst});
+    bytes += sizeof(double2) * ((size_t)a.size() + b.size() + dst.size());
+  }
+  void run(Arena& sc, hipStream_t s) {
+    if (jobs.empty()) return;
+    GemmBatch<double2> g;
+    if constexpr (std::is_same<T, double2>::value) {
+      CopyBatch<double2> cp;
+      for (auto& j : jobs) {
+        Blk<double2> t = new_blk<double2>(sc, j.dst.r, j.dst.c);
+        g.add(j.a, 0, j.b, 0, t, ONE, ZERO);
+        cp.add(t, j.dst);
+      }
+      g.run(sc, s);
+      cp.run(sc, s);
+    } else {
+      std::vector<WidenDesc> wd;
+      std::vector<CvtDesc<float2>> nd;
+      std::map<const float2*, Blk<double2>> wide;  // each source widened once
+      auto widen = [&](const Blk<float2>& x) {
+        if (x.ld != x.r) throw std::runtime_error("sweep composition needs contiguous blocks");
+        auto it = wide.find(x.p);
+        if (it != wide.end()) return it->second;
+        Blk<double2> w = new_blk<double2>(sc, x.r, x.c);
+        wd.push_back(WidenDesc{x.p, w.p, (int64_t)x.size()});
+        wide[x.p] = w;
+        return w;
+      };
+      for (auto& j : jobs) {
+        Blk<double2> a = widen(j.a), b = widen(j.b);
+        Blk<double2> t = new_blk<double2>(sc, j.dst.r, j.dst.c);
+        g.add(a, 0, b, 0, t, ONE, ZERO);
+        if (j.dst.ld != j.dst.r) throw std::runtime_error("sweep composition needs contiguous blocks");
+        nd.push_back(CvtDesc<float2>{t.p, j.dst.p, (int64_t)j.dst.size()});
+      }
+      launch_widen((int)wd.size(), upload(sc, wd, s), s);
+      g.run(sc, s);  // every product reads the widened couplings before any is replaced
+      launch_cvt<float2>((int)nd.size(), upload(sc, nd, s), s);
+    }
+    jobs.clear();
+    bytes = 0;
+    sc.reset();  // scratch reuse is ordered on the stream
+  }
+};
+}  // namespace
+
+template <class T>
+void Factor<T>::compose_sweep() {
+  const size_t chunk = size_t(256) << 20;
+  Composer<T> C;
+  for (int j = 2; j <= L; ++j) {
+    const auto& LS = pb.lists[j];
+    for (int k = 0; k < (1 << (2 * j)); ++k) {
+      FBox& F = B[j][k];
+      for (int n = 0; n < 8; ++n) {
+        const int nn = LS[k].nb[n];
+        if (nn < 0) continue;
+        FBox& Fn = B[j][nn];
+        if (Fn.ill || !Fn.m) continue;
+        auto it = F.P2P.find(nn);
+        if (F.m && it != F.P2P.end() && it->second.size()) {
+          Blk<T>& P = it->second;
+          if (Fn.color > F.color && Fn.r) {
+            Blk<T> pu = new_blk<T>(ar, P.r, Fn.r);
+            C.add(P, Fn.U, pu);
+            st.flops += 8.0 * P.r * (double)P.c * Fn.r;
+            F.PU[nn] = pu;
+          }
+          C.add(P, Fn.Z, P);
+          st.flops += 8.0 * P.r * (double)P.c * P.c;
+        }
+        // P2L held by box k towards a later-coloured regular neighbour nn
+        auto il = F.P2L.find(nn);
+        if (F.r && Fn.color > F.color && il != F.P2L.end() && il->second.size()) {
+          C.add(il->second, Fn.Z, il->second);
+          st.flops += 8.0 * il->second.r * (double)il->second.c * il->second.c;
+        }
+      }
+      if (C.bytes > chunk) C.run(sc, pb.stream);
+    }
+  }
+  C.run(sc, pb.stream);
+}
+
 // The forward sweep (the RHS updates the reference fuses into eliminate_*),
 // the base solve and the back-substitution (back_substitution_phase,
-// FMM2DTree_find_error.hpp:2487-2532) as one box program:
-//  forward, level j = L..2, colour c = 0..8, box k of colour c:
-//     y = b_k - sum_{nn earlier colour} P2P[k][nn] b_nn ;  bs_k = y
-//     b_k = Z_k y ;  mr_k = H_k y            (mr_k = rows of the parent's b)
-//  then every box nn:  mr_nn -= sum_{k later colour} P2L[nn][k] b_k
+// FMM2DTree_find_error.hpp:2487-2532) as one box program.  With y_k the
+// phase-1 vector of box k and the composed couplings above:
+//  forward, level j = L..2:
+//     colour c = 0..8, box k of colour c:
+//        y_k = b_k - sum_{nn earlier colour} P2P[k][nn] y_nn   (= P2P Z_nn y_nn)
+//     (the eliminated b_nn = Z_nn y_nn is never formed), then every box nn:
+//        mr_nn = H_nn y_nn - sum_{k later colour} P2L[nn][k] y_k
+//     (mr_nn = rows of the parent's b)
 //  base: xb = (level-2 system)^{-1} yb       (yb = the level-2 mr stacked)
-//  backward, level j = 2..L, colour c = 8..0, box k:
-//     x0 = bs_k - sum_{nn later colour} P2P[k][nn] x_nn
-//               - sum_{nn earlier colour} M2P[k][nn] y_nn
-//     x_k = Z_k x0 - U_k y_k                  (y_k = rows of the parent's x)
+//  backward, level j = 2..L:
+//     colour c = 8..0, box k of colour c:
+//        y_k = bs_k - sum_{nn later colour} (P2P[k][nn] y_nn - PU[k][nn] yp_nn)
+//                   - sum_{nn earlier colour} M2P[k][nn] yp_nn
+//     (yp_nn = rows of the parent's x, known before the level starts), then
+//     every box:  x_k = Z_k y_k - U_k yp_k
+// A colour's launch reads only phase-1 vectors of earlier launches, so each
+// colour is one launch even when its boxes are cut into row tiles, and each
+// level's outputs are one more.  An ill box (reference order) forms its
+// outputs right after its colour, and its neighbours read those:
+//     forward   mr_k = H y_k ;  b_k = A^{-1} y_k - W_r mr_k
+//     backward  v_k = yp_k - P2M A^{-1} y_k ;  x_k = A^{-1} y_k - U v_k
 // A pulled update equals the reference's pushed one: each neighbour update
 // is applied before the value it changes is next read.
 template <class T>
@@ -851,14 +967,14 @@ static void need_shape(const Blk<T>& b, int rows, int cols, const char* what) {
 // boxes stacked in order), so the sweep at levels jd..2 can run as one dense
 // product.
 template <class T>
-void Factor<T>::alloc_vectors(Arena& a, int cap, int jhi, const Blk<T>* b2, const Blk<T>* x2) {
+void Factor<T>::alloc_vectors(Arena& a, int cap, int jhi, const Blk<double2>* b2, const Blk<double2>* x2) {
   if (jhi == L) {
-    Bt = new_blk<T>(a, (int)pb.N, cap);
-    Xt = new_blk<T>(a, (int)pb.N, cap);
+    Bt = new_blk<double2>(a, (int)pb.N, cap);
+    Xt = new_blk<double2>(a, (int)pb.N, cap);
   }
   const int nbase = boff.empty() ? 0 : boff[16];
-  yb = new_blk<T>(a, nbase, cap);
-  xb = new_blk<T>(a, nbase, cap);
+  yb = new_blk<double2>(a, nbase, cap);
+  xb = new_blk<double2>(a, nbase, cap);
   bk.assign(L + 1, {});
   bs.assign(L + 1, {});
   mr.assign(L + 1, {});
@@ -881,10 +997,10 @@ void Factor<T>::alloc_vectors(Arena& a, int cap, int jhi, const Blk<T>* b2, cons
         bk[j][k] = b2->sub(off2[k], 0, F.m, cap);
         xk[j][k] = x2->sub(off2[k], 0, F.m, cap);
       } else {
-        bk[j][k] = new_blk<T>(a, F.m, cap);
-        xk[j][k] = new_blk<T>(a, F.m, cap);
+        bk[j][k] = new_blk<double2>(a, F.m, cap);
+        xk[j][k] = new_blk<double2>(a, F.m, cap);
       }
-      bs[j][k] = new_blk<T>(a, F.m, cap);
+      bs[j][k] = new_blk<double2>(a, F.m, cap);
     }
   }
   for (int j = jhi; j >= 2; --j) {
@@ -909,14 +1025,13 @@ void Factor<T>::alloc_vectors(Arena& a, int cap, int jhi, const Blk<T>* b2, cons
     yt[j].resize(1 << (2 * j));
     vk[j].resize(1 << (2 * j));
     for (int k = 0; k < (1 << (2 * j)); ++k) {
-      if (B[j][k].m > Program<T>::tile_m(cap)) yt[j][k] = new_blk<T>(a, B[j][k].m, cap);
-      if (B[j][k].ill) vk[j][k] = new_blk<T>(a, B[j][k].r, cap);
+      if (B[j][k].m) yt[j][k] = new_blk<double2>(a, B[j][k].m, cap);
+      if (B[j][k].ill) vk[j][k] = new_blk<double2>(a, B[j][k].r, cap);
     }
   }
 }
 
-// forward sweep of levels jhi..jlo (each level's colours, then its deferred
-// P2L updates)
+// forward sweep of levels jhi..jlo
 template <class T>
 void Factor<T>::emit_forward(Program<T>& P, int jlo, int jhi) {
   for (int j = jhi; j >= jlo; --j) {
@@ -925,46 +1040,43 @@ void Factor<T>::emit_forward(Program<T>& P, int jlo, int jhi) {
       P.stage();
       for (int k : colors[j][c]) {
         FBox& F = B[j][k];
-        if (!F.m && !F.r) continue;
+        if (!F.m) continue;
         BoxOp<T> op;
         op.m = F.m;
         op.src = bk[j][k].p;
         op.ldsrc = bk[j][k].ld;
         op.save = bs[j][k].p;
         op.ldsave = bs[j][k].ld;
-        for (int n = 0; n < 8 && F.m; ++n) {
+        for (int n = 0; n < 8; ++n) {
           int nn = LS[k].nb[n];
-          if (nn < 0 || B[j][nn].color >= c) continue;
+          if (nn < 0 || B[j][nn].color >= c || !B[j][nn].m) continue;
           auto it = F.P2P.find(nn);
           if (it == F.P2P.end() || !it->second.size()) continue;
           need_shape(it->second, F.m, B[j][nn].m, "P2P (forward)");
-          op.term(it->second.p, it->second.ld, it->second.c, bk[j][nn].p, bk[j][nn].ld, MONE);
-        }
-        const Blk<T>& Zf = F.ill ? F.Ai : F.Z;
-        if (F.m) out_term<T>(op.out(bk[j][k].p, bk[j][k].ld, F.m), Zf.p, Zf.ld, F.m, nullptr, 0, ONE);
-        if (F.r) {
-          auto& o = op.out(mr[j][k].p, mr[j][k].ld, F.r);
-          if (F.m) out_term<T>(o, F.H.p, F.H.ld, F.m, nullptr, 0, ONE);
+          const Blk<double2>& v = B[j][nn].ill ? bk[j][nn] : bs[j][nn];
+          op.term(it->second.p, it->second.ld, it->second.c, v.p, v.ld, MONE);
         }
         P.add(op);
       }
-      // ill boxes of this colour: b_k = A^{-1} y - W_r mr_k
+      // ill boxes of this colour: mr_k = H y ; b_k = A^{-1} y - W_r mr_k
       bool staged = false;
       for (int k : colors[j][c]) {
         FBox& F = B[j][k];
-        if (!F.ill) continue;
+        if (!F.ill || !F.m) continue;
         if (!staged) P.stage(), staged = true;
         BoxOp<T> op;
-        op.m = F.m;
-        op.src = bk[j][k].p;
-        op.ldsrc = bk[j][k].ld;
-        op.save = bk[j][k].p;
-        op.ldsave = bk[j][k].ld;
-        op.term(F.W.p, F.W.ld, F.r, mr[j][k].p, mr[j][k].ld, MONE);
+        op.m = F.r;
+        op.save = mr[j][k].p;
+        op.ldsave = mr[j][k].ld;
+        op.whole = true;
+        op.term(F.H.p, F.H.ld, F.m, bs[j][k].p, bs[j][k].ld, ONE);
+        auto& o = op.out(bk[j][k].p, bk[j][k].ld, F.m);
+        out_term<T>(o, F.Ai.p, F.Ai.ld, F.m, bs[j][k].p, bs[j][k].ld, ONE);
+        out_term<T>(o, F.W.p, F.W.ld, F.r, nullptr, 0, MONE);
         P.add(op);
       }
     }
-    // deferred P2L updates of earlier-coloured boxes
+    // every box's contribution to its parent's b
     P.stage();
     int nb = 1 << (2 * j);
     for (int nn = 0; nn < nb; ++nn) {
@@ -972,19 +1084,24 @@ void Factor<T>::emit_forward(Program<T>& P, int jlo, int jhi) {
       if (!Fn.r) continue;
       BoxOp<T> op;
       op.m = Fn.r;
-      op.src = mr[j][nn].p;
-      op.ldsrc = mr[j][nn].ld;
       op.save = mr[j][nn].p;
       op.ldsave = mr[j][nn].ld;
+      if (Fn.ill) {
+        op.src = mr[j][nn].p;
+        op.ldsrc = mr[j][nn].ld;
+      } else if (Fn.m) {
+        op.term(Fn.H.p, Fn.H.ld, Fn.m, bs[j][nn].p, bs[j][nn].ld, ONE);
+      }
       for (int n = 0; n < 8; ++n) {
         int k = LS[nn].nb[n];
         if (k < 0 || B[j][k].color <= Fn.color || !B[j][k].m) continue;
         auto it = Fn.P2L.find(k);
         if (it == Fn.P2L.end() || !it->second.size()) continue;
         need_shape(it->second, Fn.r, B[j][k].m, "P2L");
-        op.term(it->second.p, it->second.ld, it->second.c, bk[j][k].p, bk[j][k].ld, MONE);
+        const Blk<double2>& v = B[j][k].ill ? bk[j][k] : bs[j][k];
+        op.term(it->second.p, it->second.ld, it->second.c, v.p, v.ld, MONE);
       }
-      if (!op.terms.empty()) P.add(op);
+      P.add(op);
     }
   }
 }
@@ -1015,46 +1132,68 @@ void Factor<T>::emit_backward(Program<T>& P, int jlo, int jhi) {
         op.m = F.m;
         op.src = bs[j][k].p;
         op.ldsrc = bs[j][k].ld;
-        op.ybuf = yt[j][k].p;
-        op.ldybuf = yt[j][k].ld;
+        op.save = yt[j][k].p;
+        op.ldsave = yt[j][k].ld;
         for (int n = 0; n < 8; ++n) {
           int nn = LS[k].nb[n];
           if (nn < 0) continue;
-          bool after = B[j][nn].color > c;
-          auto& rm = after ? F.P2P : F.M2P;
-          auto it = rm.find(nn);
-          if (it == rm.end() || !it->second.size()) continue;
-          const Blk<T>& v = after ? xk[j][nn] : yk[j][nn];
-          need_shape(it->second, F.m, v.r, after ? "P2P (backward)" : "M2P");
-          op.term(it->second.p, it->second.ld, it->second.c, v.p, v.ld, MONE);
-        }
-        auto& o = op.out(xk[j][k].p, xk[j][k].ld, F.m);
-        if (F.ill) {
-          out_term<T>(o, F.Ai.p, F.Ai.ld, F.m, nullptr, 0, ONE);
-          auto& ov = op.out(vk[j][k].p, vk[j][k].ld, F.r);
-          out_term<T>(ov, eye.p, eye.ld, F.r, yk[j][k].p, yk[j][k].ld, ONE);
-          out_term<T>(ov, F.PA.p, F.PA.ld, F.m, nullptr, 0, ONE);
-        } else {
-          out_term<T>(o, F.Z.p, F.Z.ld, F.m, nullptr, 0, ONE);
-          if (F.r) out_term<T>(o, F.U.p, F.U.ld, F.r, yk[j][k].p, yk[j][k].ld, MONE);
+          const FBox& Fn = B[j][nn];
+          if (Fn.color > c) {
+            if (!Fn.m) continue;
+            auto it = F.P2P.find(nn);
+            if (it == F.P2P.end() || !it->second.size()) continue;
+            need_shape(it->second, F.m, Fn.m, "P2P (backward)");
+            if (Fn.ill) {
+              op.term(it->second.p, it->second.ld, it->second.c, xk[j][nn].p, xk[j][nn].ld, MONE);
+            } else {
+              op.term(it->second.p, it->second.ld, it->second.c, yt[j][nn].p, yt[j][nn].ld, MONE);
+              auto iu = F.PU.find(nn);
+              if (iu != F.PU.end() && iu->second.size()) {
+                need_shape(iu->second, F.m, yk[j][nn].r, "P2P U (backward)");
+                op.term(iu->second.p, iu->second.ld, iu->second.c, yk[j][nn].p, yk[j][nn].ld, ONE);
+              }
+            }
+          } else {
+            auto it = F.M2P.find(nn);
+            if (it == F.M2P.end() || !it->second.size()) continue;
+            need_shape(it->second, F.m, yk[j][nn].r, "M2P");
+            op.term(it->second.p, it->second.ld, it->second.c, yk[j][nn].p, yk[j][nn].ld, MONE);
+          }
         }
         P.add(op);
       }
-      // ill boxes of this colour: x_k = A^{-1} x0 - U (y_k - P2M A^{-1} x0)
+      // ill boxes of this colour: v_k = yp_k - P2M A^{-1} y ; x_k = A^{-1} y - U v_k
       bool staged = false;
       for (int k : colors[j][c]) {
         FBox& F = B[j][k];
-        if (!F.ill) continue;
+        if (!F.ill || !F.m) continue;
         if (!staged) P.stage(), staged = true;
         BoxOp<T> op;
-        op.m = F.m;
-        op.src = xk[j][k].p;
-        op.ldsrc = xk[j][k].ld;
-        op.save = xk[j][k].p;
-        op.ldsave = xk[j][k].ld;
-        op.term(F.U.p, F.U.ld, F.r, vk[j][k].p, vk[j][k].ld, MONE);
+        op.m = F.r;
+        op.src = yk[j][k].p;
+        op.ldsrc = yk[j][k].ld;
+        op.save = vk[j][k].p;
+        op.ldsave = vk[j][k].ld;
+        op.whole = true;
+        op.term(F.PA.p, F.PA.ld, F.m, yt[j][k].p, yt[j][k].ld, ONE);
+        auto& o = op.out(xk[j][k].p, xk[j][k].ld, F.m);
+        out_term<T>(o, F.Ai.p, F.Ai.ld, F.m, yt[j][k].p, yt[j][k].ld, ONE);
+        out_term<T>(o, F.U.p, F.U.ld, F.r, nullptr, 0, MONE);
         P.add(op);
       }
+    }
+    // x of every regular box
+    P.stage();
+    for (int k = 0; k < (1 << (2 * j)); ++k) {
+      FBox& F = B[j][k];
+      if (!F.m || F.ill) continue;
+      BoxOp<T> op;
+      op.m = F.m;
+      op.save = xk[j][k].p;
+      op.ldsave = xk[j][k].ld;
+      op.term(F.Z.p, F.Z.ld, F.m, yt[j][k].p, yt[j][k].ld, ONE);
+      if (F.r) op.term(F.U.p, F.U.ld, F.r, yk[j][k].p, yk[j][k].ld, MONE);
+      P.add(op);
     }
   }
 }
@@ -1100,7 +1239,7 @@ template <class T>
 void Factor<T>::dense_top_fill() {
   if (!n2) return;
   const int n = n2;
-  Blk<T> b2 = new_blk<T>(sc, n, n), x2 = new_blk<T>(sc, n, n);
+  Blk<double2> b2 = new_blk<double2>(sc, n, n), x2 = new_blk<double2>(sc, n, n);
   alloc_vectors(sc, n, jd, &b2, &x2);  // the sweep program already holds its own
   Program<T> P(n);
   emit_forward(P, 2, jd);
@@ -1111,12 +1250,17 @@ void Factor<T>::dense_top_fill() {
   // on MI355X (N = 147,456: level 3, 2,485 columns, 7.3 ms against 16.1 ms at
   // the default shape; level 2, 790 columns, 0.8 against 1.5 ms)
   P.force_shape(n, 256 + 2 * PROG_SHAPE_CH);
-  CopyBatch<T> cp;
+  CopyBatch<double2> cp;
   cp.add(b2, b2, 3);
   cp.run(sc, pb.stream);
   P.run(n, pb.stream);
-  cp.add(x2, d2);
-  cp.run(sc, pb.stream);
+  if constexpr (std::is_same<T, double2>::value) {
+    cp.add(x2, d2);
+    cp.run(sc, pb.stream);
+  } else {
+    std::vector<CvtDesc<T>> cv{CvtDesc<T>{x2.p, d2.p, (int64_t)n * n}};
+    launch_cvt<T>(1, upload(sc, cv, pb.stream), pb.stream);
+  }
   HIP_OK(hipStreamSynchronize(pb.stream));  // before P and its launches go
 }
 
@@ -1131,7 +1275,7 @@ void Factor<T>::build_plan(int cap) {
   // the worst column then needed a fourth refinement step (718 -> 621 RHS
   // solves/s at N = 147,456), more than the saved launches return there.
   if (n2 && cap == 1) {
-    Blk<T> b2 = new_blk<T>(pv, n2, cap), x2 = new_blk<T>(pv, n2, cap);
+    Blk<double2> b2 = new_blk<double2>(pv, n2, cap), x2 = new_blk<double2>(pv, n2, cap);
     alloc_vectors(pv, cap, L, &b2, &x2);
     emit_forward(P, jd + 1, L);
     // levels jd..2 and the base solve: x2 = D b2, in row tiles
@@ -1159,19 +1303,15 @@ Program<T>& Factor<T>::sweep_program(int nrhs) {
   return *plan;
 }
 
-// b and Bt/Xt share the leading dimension N
+// b and Bt/Xt share the leading dimension N (complex128 in either precision)
 template <class T>
 void Factor<T>::sweep_load(const double2* b, int nrhs, hipStream_t s) {
-  const int64_t N = pb.N;
-  if (std::is_same<T, double2>::value) HIP_OK(hipMemcpyAsync(Bt.p, b, N * nrhs * sizeof(T), hipMemcpyDeviceToDevice, s));
-  else launch_d2f(b, (float2*)Bt.p, N * nrhs, s);
+  HIP_OK(hipMemcpyAsync(Bt.p, b, pb.N * nrhs * sizeof(double2), hipMemcpyDeviceToDevice, s));
 }
 
 template <class T>
 void Factor<T>::sweep_store(double2* x, int nrhs, hipStream_t s) {
-  const int64_t N = pb.N;
-  if (std::is_same<T, double2>::value) HIP_OK(hipMemcpyAsync(x, Xt.p, N * nrhs * sizeof(T), hipMemcpyDeviceToDevice, s));
-  else launch_f2d((const float2*)Xt.p, x, N * nrhs, s);
+  HIP_OK(hipMemcpyAsync(x, Xt.p, pb.N * nrhs * sizeof(double2), hipMemcpyDeviceToDevice, s));
 }
 
 template <class T>

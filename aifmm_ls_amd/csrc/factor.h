@@ -151,6 +151,11 @@ class Factor {
     bool ill = false;
     Blk<T> Ai;  // A^{-1}                (m x m)
     Blk<T> PA;  // -P2M A^{-1}           (r x m)
+    // Composed sweep couplings (compose_sweep): for a regular neighbour nn,
+    // P2P[nn] holds P2P[k][nn] Z_nn, and for a regular later-coloured k,
+    // P2L[k] holds P2L[nn][k] Z_k; PU[nn] = P2P[k][nn] U_nn (original P2P)
+    // for a regular later-coloured nn.
+    std::map<int, Blk<T>> PU;
   };
   static constexpr double kIllCond = 1e4;
   // Zero-pivot epsilons of the complex128 factorisations (Eigen's
@@ -176,10 +181,12 @@ class Factor {
   Arena pv{size_t(256) << 20};
   std::unique_ptr<Program<T>> plan;
   int plan_cap = 0;
-  Blk<T> Bt, Xt, yb, xb;
-  std::vector<std::vector<Blk<T>>> bk, bs, mr, xk, yk;
-  std::vector<std::vector<Blk<T>>> yt;  // phase-1 vectors of tall boxes (back-substitution)
-  std::vector<std::vector<Blk<T>>> vk;  // y_k - P2M A^{-1} x0 of the ill boxes
+  // (vectors are complex128 in either precision: the box programs form every
+  // product and sum in FP64)
+  Blk<double2> Bt, Xt, yb, xb;
+  std::vector<std::vector<Blk<double2>>> bk, bs, mr, xk, yk;
+  std::vector<std::vector<Blk<double2>>> yt;  // phase-1 vectors of the back-substitution
+  std::vector<std::vector<Blk<double2>>> vk;  // y_k - P2M A^{-1} x0 of the ill boxes
   Blk<T> eye;                           // identity, rank x rank of the largest box
   // the sweep at levels jd..2 and the base solve as one dense matrix (n2 x
   // n2; n2 = 0: the staged form), level-jd box offsets in its stacked vectors
@@ -188,9 +195,10 @@ class Factor {
   int n2 = 0, jd = 2;
   std::vector<int> off2;
   void sweep_operators();
+  void compose_sweep();
   void dense_top_alloc();
   void dense_top_fill();
-  void alloc_vectors(Arena& a, int cap, int jhi, const Blk<T>* b2, const Blk<T>* x2);
+  void alloc_vectors(Arena& a, int cap, int jhi, const Blk<double2>* b2, const Blk<double2>* x2);
   void emit_forward(Program<T>& P, int jlo, int jhi);
   void emit_base(Program<T>& P);
   void emit_backward(Program<T>& P, int jlo, int jhi);

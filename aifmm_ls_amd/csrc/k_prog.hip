@@ -1,5 +1,7 @@
 // Box-program kernel (descriptors in prog.h): one workgroup (4, 8 or 16
 // wave64) per program and group of CH right-hand-side columns (blockIdx.y).
+// Matrices are complex128 or complex64 (T); vectors, products and sums are
+// complex128 (V).
 // Each output row block of up to 256 rows is spread over the
 // workgroup as rp rows x ns column slices (rp = rows rounded up to 16, 32 or
 // a multiple of 64; ns = threads / rp), so a wave reads a contiguous column
@@ -13,6 +15,10 @@
 
 namespace aifmm {
 
+using V = double2;
+__device__ __forceinline__ V wv(float2 a) { return mk((double)a.x, (double)a.y); }
+__device__ __forceinline__ V wv(double2 a) { return a; }
+
 // dynamic LDS of a program (the phase-1 vector and the reduction buffer); the
 // staged term descriptors take the rest of the 64 KB a workgroup may address
 constexpr int PROG_LDS = 63 * 1024;
@@ -21,14 +27,14 @@ constexpr int PROG_LDS = 63 * 1024;
 // column slice s (columns s, s + ns, ...), four independent loads in flight.
 template <class T, int CH, int NTH>
 __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __restrict__ terms, int t0, int t1,
-                                              int c0, int nc, const T* ylds, int ldy, T* red, T (&acc)[CH],
+                                              int c0, int nc, const V* ylds, int ldy, V* red, V (&acc)[CH],
                                               int& row) {
   const int tid = threadIdx.x;
   const int rp = rr <= 16 ? 16 : rr <= 32 ? 32 : ((rr + 63) >> 6) << 6;
   const int ns = NTH / rp;
   const int i = tid % rp, s = tid / rp;
 #pragma unroll
-  for (int c = 0; c < CH; ++c) acc[c] = czero<T>();
+  for (int c = 0; c < CH; ++c) acc[c] = czero<V>();
   const int nt = t1 - t0;
   // One column per pass.  Complex64 (the sweeps) walks the terms' columns as
   // one sequence (path 2).  Complex128 (the FP64 operator products: many
@@ -47,22 +53,22 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
       const PTerm<T> tm = terms[t0 + s % nt];
       const int q = s / nt;
       const T* A = tm.A + rb + i;
-      const T* x = tm.x ? tm.x + (size_t)tm.ldx * c0 : ylds;
+      const V* x = tm.x ? tm.x + (size_t)tm.ldx * c0 : ylds;
       const size_t la = tm.lda;
       // two accumulators: alternate products do not wait on each other
-      T pa = czero<T>(), pb = czero<T>();
+      V pa = czero<V>(), pb = czero<V>();
       int j = q;
       for (; j + 3 * spt < tm.cols; j += 4 * spt) {
         const T a0 = A[la * j], a1 = A[la * (j + spt)], a2 = A[la * (j + 2 * spt)], a3 = A[la * (j + 3 * spt)];
-        const T x0 = x[j], x1 = x[j + spt], x2 = x[j + 2 * spt], x3 = x[j + 3 * spt];
-        pa = cfma(a0, x0, pa);
-        pb = cfma(a1, x1, pb);
-        pa = cfma(a2, x2, pa);
-        pb = cfma(a3, x3, pb);
+        const V x0 = x[j], x1 = x[j + spt], x2 = x[j + 2 * spt], x3 = x[j + 3 * spt];
+        pa = cfma(wv(a0), x0, pa);
+        pb = cfma(wv(a1), x1, pb);
+        pa = cfma(wv(a2), x2, pa);
+        pb = cfma(wv(a3), x3, pb);
       }
-      for (; j < tm.cols; j += spt) pa = cfma(A[la * j], x[j], pa);
+      for (; j < tm.cols; j += spt) pa = cfma(wv(A[la * j]), x[j], pa);
       pa = cadd(pa, pb);
-      acc[0] = cmul(from2<T>(tm.alpha), pa);
+      acc[0] = cmul(tm.alpha, pa);
     }
   } else if (sizeof(T) == 8 && path == 2) {
     // the terms' columns form one sequence (term t0's columns, then t0 + 1's,
@@ -75,9 +81,9 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
       // the current term's fields stay in registers; they are read again
       // (all at once) when the walk enters the next term
       const T* Ab;
-      const T* xb;
+      const V* xb;
       size_t la;
-      T alp;
+      V alp;
       int ct;
       int t = t0, j = s;
       auto enter = [&](int tt) {
@@ -85,7 +91,7 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
         Ab = tm.A + rb + i;
         la = tm.lda;
         xb = tm.x ? tm.x + (size_t)tm.ldx * c0 : ylds;
-        alp = from2<T>(tm.alpha);
+        alp = tm.alpha;
         ct = tm.cols;
       };
       auto land = [&]() {
@@ -98,10 +104,12 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
       enter(t0);
       land();
       while (t < t1) {
-        T av[U], xv[U], al[U];
+        T av[U];
+        V xv[U], al[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-          av[u] = xv[u] = al[u] = czero<T>();
+          av[u] = czero<T>();
+          xv[u] = al[u] = czero<V>();
           if (t < t1) {
             av[u] = Ab[la * j];
             xv[u] = xb[j];
@@ -111,14 +119,14 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
           }
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) acc[0] = cfma(al[u], cmul(av[u], xv[u]), acc[0]);
+        for (int u = 0; u < U; ++u) acc[0] = cfma(al[u], cmul(wv(av[u]), xv[u]), acc[0]);
       }
     }
   } else if (s < ns && i < rr) {
     for (int t = t0; t < t1; ++t) {
       const PTerm<T> tm = terms[t];
       const T* A = tm.A + rb + i;
-      const T* x;
+      const V* x;
       size_t ldx;
       if (tm.x) {
         x = tm.x + (size_t)tm.ldx * c0;
@@ -128,29 +136,29 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
         ldx = ldy;
       }
       const size_t la = tm.lda;
-      T pa[CH];
+      V pa[CH];
 #pragma unroll
-      for (int c = 0; c < CH; ++c) pa[c] = czero<T>();
+      for (int c = 0; c < CH; ++c) pa[c] = czero<V>();
       int j = s;
       for (; j + 3 * ns < tm.cols; j += 4 * ns) {
         const T a0 = A[la * j], a1 = A[la * (j + ns)], a2 = A[la * (j + 2 * ns)], a3 = A[la * (j + 3 * ns)];
 #pragma unroll
         for (int c = 0; c < CH; ++c)
           if (c < nc) {
-            const T* xc = x + ldx * c;
-            pa[c] = cfma(a0, xc[j], pa[c]);
-            pa[c] = cfma(a1, xc[j + ns], pa[c]);
-            pa[c] = cfma(a2, xc[j + 2 * ns], pa[c]);
-            pa[c] = cfma(a3, xc[j + 3 * ns], pa[c]);
+            const V* xc = x + ldx * c;
+            pa[c] = cfma(wv(a0), xc[j], pa[c]);
+            pa[c] = cfma(wv(a1), xc[j + ns], pa[c]);
+            pa[c] = cfma(wv(a2), xc[j + 2 * ns], pa[c]);
+            pa[c] = cfma(wv(a3), xc[j + 3 * ns], pa[c]);
           }
       }
       for (; j < tm.cols; j += ns) {
-        const T a = A[la * j];
+        const V a = wv(A[la * j]);
 #pragma unroll
         for (int c = 0; c < CH; ++c)
           if (c < nc) pa[c] = cfma(a, x[j + ldx * c], pa[c]);
       }
-      const T al = from2<T>(tm.alpha);
+      const V al = tm.alpha;
 #pragma unroll
       for (int c = 0; c < CH; ++c) acc[c] = cfma(al, pa[c], acc[c]);
     }
@@ -174,7 +182,7 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
     if (w == 0) {
       const int g = lane / rp, ng = 64 / rp;
 #pragma unroll
-      for (int c = 0; c < CH; ++c) acc[c] = czero<T>();
+      for (int c = 0; c < CH; ++c) acc[c] = czero<V>();
       for (int q = g; q < NW; q += ng)
 #pragma unroll
         for (int c = 0; c < CH; ++c) acc[c] = cadd(acc[c], red[q * rp + i + NTH * c]);
@@ -204,7 +212,7 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
 // vector, red: NTH x CH reduction buffer, both in LDS).
 template <class T, int CH, int NTH>
 __device__ __forceinline__ void run_program(const PProg<T>* __restrict__ P, const PTerm<T>* __restrict__ terms,
-                                            int nrhs, int ldy, T* ylds, T* red) {
+                                            int nrhs, int ldy, V* ylds, V* red) {
   const int m = P->m;
   // column groups of CH right-hand sides are independent (each has its own
   // phase-1 vector and writes its own columns): workgroup (program, g) runs
@@ -213,25 +221,25 @@ __device__ __forceinline__ void run_program(const PProg<T>* __restrict__ P, cons
     const int nc = min(CH, nrhs - c0);
     for (int rb = 0; rb < m; rb += 256) {
       const int rr = min(256, m - rb);
-      T acc[CH];
+      V acc[CH];
       int row;
       // the source rows are loaded ahead of the term sums (same row mapping
       // as row_block_sum: row rb + tid % rp, written by slice 0)
-      T sv[CH];
+      V sv[CH];
       {
         const int rp = rr <= 16 ? 16 : rr <= 32 ? 32 : ((rr + 63) >> 6) << 6;
         const int i = threadIdx.x % rp;
 #pragma unroll
         for (int c = 0; c < CH; ++c)
           sv[c] = (P->src && threadIdx.x < rp && i < rr && c < nc) ? P->src[rb + i + (size_t)P->ldsrc * (c0 + c)]
-                                                                  : czero<T>();
+                                                                  : czero<V>();
       }
       if (row_block_sum<T, CH, NTH>(rb, rr, terms, P->t0, P->t1, c0, nc, ylds, ldy, red, acc, row)) {
-        const T sa = from2<T>(P->src_alpha);
+        const V sa = P->src_alpha;
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
           if (c >= nc) break;
-          T v = acc[c];
+          V v = acc[c];
           if (P->src) v = cfma(sa, sv[c], v);
           ylds[row + (size_t)ldy * c] = v;
           if (P->save) P->save[row + (size_t)P->ldsave * (c0 + c)] = v;
@@ -243,10 +251,10 @@ __device__ __forceinline__ void run_program(const PProg<T>* __restrict__ P, cons
       const int rows = P->rows[o];
       for (int rb = 0; rb < rows; rb += 256) {
         const int rr = min(256, rows - rb);
-        T acc[CH];
+        V acc[CH];
         int row;
         if (row_block_sum<T, CH, NTH>(rb, rr, terms, P->u0[o], P->u1[o], c0, nc, ylds, ldy, red, acc, row)) {
-          T* out = P->out[o];
+          V* out = P->out[o];
           const size_t ldo = P->ldo[o];
 #pragma unroll
           for (int c = 0; c < CH; ++c) {
@@ -269,8 +277,8 @@ __global__ __launch_bounds__(NTH) void k_prog(const PSlot<T>* __restrict__ slots
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ PTerm<T> tl[PSLOT_TERMS];
   static_assert(sizeof(PSlot<T>) % 16 == 0 && offsetof(PSlot<T>, t) % 16 == 0, "slot layout");
-  T* ylds = (T*)smem;
-  T* red = ylds + (size_t)ldy * CH;
+  V* ylds = (V*)smem;
+  V* red = ylds + (size_t)ldy * CH;
   const PSlot<T>* S = slots + blockIdx.x;
   const int tw = slot_words - (int)(offsetof(PSlot<T>, t) / 16);
   const int4* src = (const int4*)S->t;
@@ -290,7 +298,7 @@ static int pick_ch(int nrhs, int ldy, int eb, int nth) {
   return 0;
 }
 
-int prog_max_m(int elem_bytes) { return PROG_LDS / elem_bytes - 1024; }
+int prog_max_m() { return PROG_LDS / (int)sizeof(V) - 1024; }
 
 // columns per pass asked for (ch_req), checked against the LDS: 1, 2, 4, or
 // 8 at up to 512 threads (8 columns of accumulators need the registers
@@ -306,9 +314,9 @@ template <class T, int NTH>
 static void launch_nth(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, int sw,
                        hipStream_t s, int ch_req) {
   if (sw < 1 || sw > (int)(sizeof(PSlot<T>) / 16)) throw std::runtime_error("bad program slot size");
-  const int ch = fit_ch(nrhs, ldy, sizeof(T), NTH, ch_req);
+  const int ch = fit_ch(nrhs, ldy, sizeof(V), NTH, ch_req);
   if (!ch) throw std::runtime_error("box program vector of " + std::to_string(ldy) + " rows exceeds LDS");
-  const size_t lds = (size_t)(ldy + NTH) * ch * sizeof(T);
+  const size_t lds = (size_t)(ldy + NTH) * ch * sizeof(V);
   // one workgroup per program and group of ch right-hand-side columns
   const int groups = (nrhs + ch - 1) / ch;
   if (groups > 65535) throw std::runtime_error("too many right-hand-side column groups");
@@ -337,7 +345,7 @@ int prog_default_nth(int nprog) { return nprog > 1024 ? 256 : 1024; }
 template <class T>
 bool prog_shape_fits(int nrhs, int ldy, int shape) {
   const int nth = shape % PROG_SHAPE_CH, ch = shape / PROG_SHAPE_CH;
-  return (nth == 256 || nth == 512 || nth == 1024) && fit_ch(nrhs, ldy < 1 ? 1 : ldy, sizeof(T), nth, ch) > 0;
+  return (nth == 256 || nth == 512 || nth == 1024) && fit_ch(nrhs, ldy < 1 ? 1 : ldy, sizeof(V), nth, ch) > 0;
 }
 
 template <class T>

@@ -136,12 +136,11 @@ static double round6(double v) {
 Problem::Problem(const Params& Pin, const std::string& precomp_dir) : P(Pin) {
   // non-blocking: no implicit ordering with the null stream, so problems on
   // different host threads overlap on the device
-  HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+  stream = stream_create(hipStreamNonBlocking);
   try {
     init(precomp_dir);
   } catch (...) {
-    (void)hipStreamSynchronize(stream);
-    (void)hipStreamDestroy(stream);
+    stream_destroy(stream);
     stream = nullptr;
     throw;
   }
@@ -201,8 +200,7 @@ void Problem::init(const std::string& precomp_dir) {
 
 Problem::~Problem() {
   if (!stream) return;
-  (void)hipStreamSynchronize(stream);  // queued work still reads this problem's memory
-  (void)hipStreamDestroy(stream);
+  stream_destroy(stream);  // synchronises first: queued work still reads this problem's memory
 }
 
 void Problem::build_lists() {

@@ -76,6 +76,12 @@ std::vector<LaunchRecord> prof_launches(bool reset) {
   return out;
 }
 
+// profiler events still pending (called before a stream is destroyed)
+void prof_fold_pending() {
+  std::lock_guard<std::mutex> g(mu);
+  fold();
+}
+
 ProfScope::ProfScope(hipStream_t s, const char* name, double bytes, double flops)
     : s_(s), name_(name), bytes_(bytes), flops_(flops) {
   if (!enabled || !name) return;  // a null name records nothing
@@ -104,10 +110,14 @@ ProfScope::~ProfScope() {
 // until the process ends.  Threads that end hand their ring back with its
 // pending events; the next owner waits for them before it writes.
 namespace {
+struct Mark {
+  hipStream_t s;
+  hipEvent_t e;
+};
 struct Ring {
   char* base = nullptr;
   size_t cap = 0, off = 0;
-  std::vector<std::pair<hipStream_t, hipEvent_t>> marks;  // last copy per stream since the last wrap
+  std::vector<Mark> marks;  // last copy per stream since the last wrap
   std::vector<hipEvent_t> spare_ev;
   std::vector<char*> retired;
 };
@@ -139,8 +149,8 @@ Ring& ring() {
 
 void wait_marks(Ring& R) {
   for (auto& m : R.marks) {
-    HIP_OK(hipEventSynchronize(m.second));
-    R.spare_ev.push_back(m.second);
+    HIP_OK(hipEventSynchronize(m.e));
+    R.spare_ev.push_back(m.e);
   }
   R.marks.clear();
 }
@@ -150,7 +160,7 @@ char* take(Ring& R, size_t bytes) {
   const size_t need = (bytes + 255) & ~size_t(255);
   if (!R.base || need > R.cap) {
     if (R.base) R.retired.push_back(R.base);
-    for (auto& m : R.marks) R.spare_ev.push_back(m.second);  // their copies read the old buffer
+    for (auto& m : R.marks) R.spare_ev.push_back(m.e);  // their copies read the old buffer
     R.marks.clear();
     R.cap = std::max<size_t>(size_t(64) << 20, need);
     HIP_OK(hipHostMalloc((void**)&R.base, R.cap, hipHostMallocDefault));
@@ -167,8 +177,8 @@ char* take(Ring& R, size_t bytes) {
 
 void mark(Ring& R, hipStream_t s) {
   for (auto& m : R.marks)
-    if (m.first == s) {
-      HIP_OK(hipEventRecord(m.second, s));
+    if (m.s == s) {
+      HIP_OK(hipEventRecord(m.e, s));
       return;
     }
   hipEvent_t e;
@@ -179,11 +189,44 @@ void mark(Ring& R, hipStream_t s) {
     HIP_OK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   HIP_OK(hipEventRecord(e, s));
-  R.marks.emplace_back(s, e);
+  R.marks.push_back(Mark{s, e});
 }
 
 constexpr size_t kChunk = size_t(16) << 20;
 }  // namespace
+
+// Streams go back to a pool instead of being destroyed, so that no event a
+// ring or the profiler still holds refers to a destroyed stream.
+namespace {
+std::mutex pool_mu;
+std::map<unsigned, std::vector<hipStream_t>> stream_pool;
+std::map<hipStream_t, unsigned> stream_flags;
+}  // namespace
+
+hipStream_t stream_create(unsigned flags) {
+  {
+    std::lock_guard<std::mutex> g(pool_mu);
+    auto& v = stream_pool[flags];
+    if (!v.empty()) {
+      hipStream_t s = v.back();
+      v.pop_back();
+      return s;
+    }
+  }
+  hipStream_t s = nullptr;
+  HIP_OK(hipStreamCreateWithFlags(&s, flags));
+  std::lock_guard<std::mutex> g(pool_mu);
+  stream_flags[s] = flags;
+  return s;
+}
+
+void stream_destroy(hipStream_t s) {
+  if (!s) return;
+  (void)hipStreamSynchronize(s);
+  prof_fold_pending();
+  std::lock_guard<std::mutex> g(pool_mu);
+  stream_pool[stream_flags[s]].push_back(s);
+}
 
 void h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
   Ring& R = ring();

@@ -26,7 +26,7 @@ void Program<T>::add(const BoxOp<T>& op) {
     maxrows = std::max(maxrows, o.rows);
     for (const auto& t : o.terms) uses_y |= t.x == nullptr;
   }
-  if (op.m <= tile_m_ && maxrows <= tile_m_) {
+  if ((op.m <= tile_m_ && maxrows <= tile_m_) || (op.whole && op.m <= 1024)) {
     PProg<T> p{};
     p.m = op.m;
     p.src = op.src;
@@ -51,7 +51,7 @@ void Program<T>::add(const BoxOp<T>& op) {
   }
   // tiled: phase 1 in row tiles into y (save or ybuf), then the outputs in
   // row tiles reading y from memory
-  T* y = op.save ? op.save : op.ybuf;
+  double2* y = op.save ? op.save : op.ybuf;
   const int ldy = op.save ? op.ldsave : op.ldybuf;
   if (op.m > 0 && !y) throw std::runtime_error("tiled box program without a vector for phase 1");
   if (uses_y && op.m == 0) throw std::runtime_error("box program term reads an empty phase-1 vector");
@@ -120,7 +120,7 @@ void Program<T>::merge_from(const std::vector<const Program<T>*>& ps) {
 template <class T>
 void Program<T>::finalize(Arena& a, hipStream_t s) {
   close();
-  const int lim = prog_max_m(sizeof(T));
+  const int lim = prog_max_m();
   if (!terms_.empty()) {
     dterms_ = a.get<PTerm<T>>(terms_.size());
     h2d(dterms_, terms_.data(), terms_.size() * sizeof(PTerm<T>), s);
@@ -162,19 +162,20 @@ void Program<T>::finalize(Arena& a, hipStream_t s) {
 template <class T>
 static double stage_bytes(const std::vector<PProg<T>>& ps, const std::vector<PTerm<T>>& ts, int nrhs) {
   double b = 0;
+  const double vb = sizeof(double2);  // vectors are complex128
   auto term_bytes = [&](int t0, int t1, int rows) {
     for (int t = t0; t < t1; ++t) {
       b += (double)rows * ts[t].cols * sizeof(T);
-      if (ts[t].x) b += (double)ts[t].cols * nrhs * sizeof(T);
+      if (ts[t].x) b += (double)ts[t].cols * nrhs * vb;
     }
   };
   for (auto& p : ps) {
     term_bytes(p.t0, p.t1, p.m);
-    if (p.src) b += (double)p.m * nrhs * sizeof(T);
-    if (p.save) b += (double)p.m * nrhs * sizeof(T);
+    if (p.src) b += (double)p.m * nrhs * vb;
+    if (p.save) b += (double)p.m * nrhs * vb;
     for (int o = 0; o < p.nout; ++o) {
       term_bytes(p.u0[o], p.u1[o], p.rows[o]);
-      b += (double)p.rows[o] * nrhs * sizeof(T);
+      b += (double)p.rows[o] * nrhs * vb;
     }
   }
   return b;

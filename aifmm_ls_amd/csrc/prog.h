@@ -6,7 +6,10 @@
 // in LDS (optionally storing it to `save`); phase 2 forms up to two outputs
 //     out_o = sum_u alpha_u A_u x_u                        (rows_o rows)
 // where a term with x == nullptr multiplies the phase-1 vector.  All vectors
-// are column-major with nrhs columns.  Every descriptor of one stage writes
+// are column-major with nrhs columns.  The matrices A are of the program's
+// element type T (complex128 or complex64); the vectors are complex128 and
+// every product is formed and summed in FP64, so a complex64 program applies
+// its complex64 matrices without rounding its vectors to complex64.  Every descriptor of one stage writes
 // its own output blocks, and reads only blocks no descriptor of the same stage
 // writes, so the workgroups of a stage are independent.
 //
@@ -29,8 +32,8 @@ namespace aifmm {
 
 template <class T>
 struct PTerm {
-  const T* A;  // rows x cols, leading dimension lda
-  const T* x;  // cols x nrhs, leading dimension ldx; nullptr: the phase-1 vector
+  const T* A;        // rows x cols, leading dimension lda
+  const double2* x;  // cols x nrhs, leading dimension ldx; nullptr: the phase-1 vector
   int lda, ldx, cols;
   double2 alpha;
 };
@@ -38,14 +41,14 @@ struct PTerm {
 template <class T>
 struct PProg {
   int m;  // phase-1 rows
-  const T* src;
+  const double2* src;
   int ldsrc;
   double2 src_alpha;
   int t0, t1;  // phase-1 terms
-  T* save;
+  double2* save;
   int ldsave;
   int nout;
-  T* out[2];
+  double2* out[2];
   int ldo[2], rows[2], u0[2], u1[2];
   int ext;  // in a PSlot: 1 if the term indices refer to the stage-wide array
 };
@@ -78,8 +81,8 @@ void prog_set_tuning(int mode);
 // whether a stage of phase-1 length ldy fits the LDS in this shape
 template <class T>
 bool prog_shape_fits(int nrhs, int ldy, int shape);
-// largest phase-1 length a stage may use
-int prog_max_m(int elem_bytes);
+// largest phase-1 length a stage may use (complex128 vectors in LDS)
+int prog_max_m();
 // while held (counted), programs launch directly instead of through graphs
 void prog_graphs_hold(bool on);
 // Threads inside the library (one ApiScope per C ABI call, reentrant per
@@ -98,31 +101,33 @@ struct ApiScope {
 template <class T>
 struct BoxOp {
   struct Out {
-    T* out;
+    double2* out;
     int ldo, rows;
     std::vector<PTerm<T>> terms;
   };
   int m = 0;
-  const T* src = nullptr;
+  const double2* src = nullptr;
   int ldsrc = 0;
   double2 src_alpha = {1, 0};
   std::vector<PTerm<T>> terms;  // phase 1
-  T* save = nullptr;            // phase-1 result (optional)
+  double2* save = nullptr;      // phase-1 result (optional)
   int ldsave = 0;
-  T* ybuf = nullptr;            // where a tiled box keeps y when save is null
+  double2* ybuf = nullptr;      // where a tiled box keeps y when save is null
   int ldybuf = 0;
   std::vector<Out> outs;        // phase 2 (at most 2)
-  void term(const T* A, int lda, int cols, const T* x, int ldx, double2 alpha) {
+  bool whole = false;           // one program even when tall (a few boxes whose outputs read all of y)
+  void term(const T* A, int lda, int cols, const double2* x, int ldx, double2 alpha) {
     if (cols > 0) terms.push_back(PTerm<T>{A, x, lda, ldx, cols, alpha});
   }
-  Out& out(T* o, int ldo, int rows) {
+  Out& out(double2* o, int ldo, int rows) {
     outs.push_back(Out{o, ldo, rows, {}});
     return outs.back();
   }
 };
 
 template <class T>
-inline void out_term(typename BoxOp<T>::Out& o, const T* A, int lda, int cols, const T* x, int ldx, double2 alpha) {
+inline void out_term(typename BoxOp<T>::Out& o, const T* A, int lda, int cols, const double2* x, int ldx,
+                     double2 alpha) {
   if (cols > 0) o.terms.push_back(PTerm<T>{A, x, lda, ldx, cols, alpha});
 }
 

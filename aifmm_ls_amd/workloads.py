@@ -8,6 +8,8 @@ profiler sees that part on its own:
   sweep   a complex64 factorisation, then three direct sweeps
           (``solve_resident(tol, 0)``: no refinement)
   factor  one assembly, then two complex64 factorisations
+  solve   a complex64 factorisation, the residual after 1..5 refinement
+          steps, then three refined solves (tolerance 1e-7)
   stream  no problem: three copies of 1 GiB between two fresh 1 GiB device
           buffers with the plain copy kernel k_stream_copy (16-byte elements),
           for calibrating FETCH_SIZE / WRITE_SIZE against a stream of known
@@ -31,7 +33,7 @@ def gaussian_bumps(seed, n):
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
-    ap.add_argument("name", choices=("sweep", "factor", "stream"))
+    ap.add_argument("name", choices=("sweep", "factor", "solve", "stream"))
     ap.add_argument("--levels", type=int, default=6)
     ap.add_argument("--kappa", type=float, default=40.0)
     ap.add_argument("--tol-pow", type=int, default=7)
@@ -51,6 +53,15 @@ def main(argv=None):
         for i in range(2):
             g.factor(A.PREC_C64)
             print(f"factor {i}: {g.stats()['t_factor']:.3f} s", flush=True)
+    elif args.name == "solve":
+        g.factor(A.PREC_C64)
+        g.set_rhs(g.rhs(0.3))
+        for k in range(0, 6):
+            st = g.solve_resident(1e-30, k)
+            print(f"after {k} steps: relres {st[2]:.3e}", flush=True)
+        for i in range(3):
+            st = g.solve_resident(1e-7, 30)
+            print(f"solve {i}: {1e3 * st[0]:.3f} ms, {st[1]:.0f} steps, relres {st[2]:.3e}", flush=True)
     else:
         g.factor(A.PREC_C64)
         g.set_rhs(g.rhs(0.3))

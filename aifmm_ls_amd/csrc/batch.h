@@ -164,7 +164,16 @@ struct LUBatch {
     if (prof_on())
       for (auto& u : d) by += sizeof(T) * 2.0 * u.n * u.n, fl += 8.0 * u.n * u.n * u.n / 3;
     ProfScope ps(s, KNAME(T, "k_lu"), by, fl);
-    launch_lu<T>((int)d.size(), dd, s);
+    // blocks that fit the LDS by the LDS kernel, the rest by the global one
+    const int lim = lu_lds_max(sizeof(T));
+    std::vector<LUDesc<T>> big;
+    int nsmall = 0;
+    for (auto& u : d) {
+      if (u.n > lim) big.push_back(u);
+      else nsmall = std::max(nsmall, u.n);
+    }
+    if (nsmall) launch_lu_lds<T>((int)d.size(), dd, nsmall, s);
+    if (!big.empty()) launch_lu<T>((int)big.size(), upload(scratch, big, s), s);
     d.clear();
   }
 };
@@ -186,7 +195,20 @@ struct SolveBatch {
     if (prof_on())
       for (auto& u : d) by += sizeof(T) * ((double)u.n * u.n + 2.0 * u.n * u.nrhs), fl += 8.0 * u.n * u.n * u.nrhs;
     ProfScope ps(s, KNAME(T, "k_lu_solve"), by, fl);
-    launch_lu_solve<T>((int)d.size(), maxc, dd, s);
+    const int lim = lu_lds_max(sizeof(T));
+    std::vector<SolveDesc<T>> big;
+    int nsmall = 0, csmall = 0, cbig = 0;
+    for (auto& u : d) {
+      if (u.n > lim) {
+        big.push_back(u);
+        cbig = std::max(cbig, (u.nrhs + 63) / 64);
+      } else {
+        nsmall = std::max(nsmall, u.n);
+        csmall = std::max(csmall, u.nrhs);
+      }
+    }
+    if (nsmall) launch_lu_solve_lds<T>((int)d.size(), csmall, dd, nsmall, s);
+    if (!big.empty()) launch_lu_solve<T>((int)big.size(), cbig, upload(scratch, big, s), s);
     d.clear();
     maxc = 0;
   }

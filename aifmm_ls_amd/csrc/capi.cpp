@@ -612,7 +612,7 @@ static void selftest_t(int op, int m, int n, int k, const double* A, const doubl
     Blk<T> t = new_blk<T>(a, r, c);
     std::vector<CvtDesc<T>> cv{CvtDesc<T>{d.p, t.p, (int64_t)r * c}};
     auto* dd = upload(sc, cv, s);
-    launch_cvt<T>(1, dd, s);
+    launch_cvt<T>(1, dd, s, max_len(cv));
     HIP_OK(hipStreamSynchronize(s));
     return t;
   };
@@ -761,7 +761,7 @@ static void selftest_prog_t(int shape, int nprog, int m, int nterms, int cols, i
       HIP_OK(hipMemcpy(d, h, n * 16, hipMemcpyHostToDevice));
       T* t = a.get<T>(n);
       std::vector<CvtDesc<T>> cv{CvtDesc<T>{d, t, (int64_t)n}};
-      launch_cvt<T>(1, upload(sc, cv, s), s);
+      launch_cvt<T>(1, upload(sc, cv, s), s, max_len(cv));
       return t;
     };
     // vectors stay complex128 (the programs' vector type)

@@ -79,7 +79,7 @@ void Factor<T>::init_level(int j) {
       for (auto& e : pb.P2P[k]) F.P2P[e.first] = conv(e.second);
   }
   auto* d = upload(sc, cv, pb.stream);
-  launch_cvt<T>((int)cv.size(), d, pb.stream);
+  launch_cvt<T>((int)cv.size(), d, pb.stream, max_len(cv));
   sc.reset();  // scratch reuse is ordered on the stream
 }
 
@@ -882,9 +882,9 @@ struct Composer {
         if (j.dst.ld != j.dst.r) throw std::runtime_error("sweep composition needs contiguous blocks");
         nd.push_back(CvtDesc<float2>{t.p, j.dst.p, (int64_t)j.dst.size()});
       }
-      launch_widen((int)wd.size(), upload(sc, wd, s), s);
+      launch_widen((int)wd.size(), upload(sc, wd, s), s, max_len(wd));
       g.run(sc, s);  // every product reads the widened couplings before any is replaced
-      launch_cvt<float2>((int)nd.size(), upload(sc, nd, s), s);
+      launch_cvt<float2>((int)nd.size(), upload(sc, nd, s), s, max_len(nd));
     }
     jobs.clear();
     bytes = 0;
@@ -1031,6 +1031,13 @@ void Factor<T>::alloc_vectors(Arena& a, int cap, int jhi, const Blk<double2>* b2
   }
 }
 
+static int env_int(const char* name, int dflt) {
+  const char* e = getenv(name);
+  return e && *e ? atoi(e) : dflt;
+}
+static int tile_colour() { static const int v = env_int("AIFMM_TILE_COLOUR", 0); return v; }
+static int tile_out() { static const int v = env_int("AIFMM_TILE_OUT", 64); return v; }
+
 // forward sweep of levels jhi..jlo
 template <class T>
 void Factor<T>::emit_forward(Program<T>& P, int jlo, int jhi) {
@@ -1047,6 +1054,7 @@ void Factor<T>::emit_forward(Program<T>& P, int jlo, int jhi) {
         op.ldsrc = bk[j][k].ld;
         op.save = bs[j][k].p;
         op.ldsave = bs[j][k].ld;
+        op.tile = tile_colour();
         for (int n = 0; n < 8; ++n) {
           int nn = LS[k].nb[n];
           if (nn < 0 || B[j][nn].color >= c || !B[j][nn].m) continue;
@@ -1086,6 +1094,7 @@ void Factor<T>::emit_forward(Program<T>& P, int jlo, int jhi) {
       op.m = Fn.r;
       op.save = mr[j][nn].p;
       op.ldsave = mr[j][nn].ld;
+      op.tile = tile_out();
       if (Fn.ill) {
         op.src = mr[j][nn].p;
         op.ldsrc = mr[j][nn].ld;
@@ -1134,6 +1143,7 @@ void Factor<T>::emit_backward(Program<T>& P, int jlo, int jhi) {
         op.ldsrc = bs[j][k].ld;
         op.save = yt[j][k].p;
         op.ldsave = yt[j][k].ld;
+        op.tile = tile_colour();
         for (int n = 0; n < 8; ++n) {
           int nn = LS[k].nb[n];
           if (nn < 0) continue;
@@ -1191,6 +1201,7 @@ void Factor<T>::emit_backward(Program<T>& P, int jlo, int jhi) {
       op.m = F.m;
       op.save = xk[j][k].p;
       op.ldsave = xk[j][k].ld;
+      op.tile = tile_out();
       op.term(F.Z.p, F.Z.ld, F.m, yt[j][k].p, yt[j][k].ld, ONE);
       if (F.r) op.term(F.U.p, F.U.ld, F.r, yk[j][k].p, yk[j][k].ld, MONE);
       P.add(op);
@@ -1259,7 +1270,7 @@ void Factor<T>::dense_top_fill() {
     cp.run(sc, pb.stream);
   } else {
     std::vector<CvtDesc<T>> cv{CvtDesc<T>{x2.p, d2.p, (int64_t)n * n}};
-    launch_cvt<T>(1, upload(sc, cv, pb.stream), pb.stream);
+    launch_cvt<T>(1, upload(sc, cv, pb.stream), pb.stream, max_len(cv));
   }
   HIP_OK(hipStreamSynchronize(pb.stream));  // before P and its launches go
 }
@@ -1270,11 +1281,11 @@ void Factor<T>::build_plan(int cap) {
   pv.reset();
   plan_cap = cap;
   Program<T>& P = *plan;
-  // The dense top serves one-column sweeps only: applying the explicit
-  // matrix rounds differently from the staged factors, and with 16 columns
-  // the worst column then needed a fourth refinement step (718 -> 621 RHS
-  // solves/s at N = 147,456), more than the saved launches return there.
-  if (n2 && cap == 1) {
+  // The dense top serves sweeps of up to 64 columns (AIFMM_DENSE_COLS): at
+  // N = 147,456, 16 columns took 22.6 ms per refined solve with it against
+  // 23.9 ms staged, in the same 3 refinement steps; 256 columns gain nothing.
+  static const int dense_cols = env_int("AIFMM_DENSE_COLS", 64);
+  if (n2 && cap <= dense_cols) {
     Blk<double2> b2 = new_blk<double2>(pv, n2, cap), x2 = new_blk<double2>(pv, n2, cap);
     alloc_vectors(pv, cap, L, &b2, &x2);
     emit_forward(P, jd + 1, L);

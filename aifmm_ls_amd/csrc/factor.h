@@ -93,15 +93,15 @@ struct DenseOps<float2> {
   // the FP64 array holding the factor (U on and above the diagonal)
   static Blk<double2> fp64_factor(const DFact<float2>& f) { return f.a64; }
   void run_factor(Arena& sc, hipStream_t s) {
-    if (!wf.empty()) launch_widen((int)wf.size(), upload(sc, wf, s), s);
+    if (!wf.empty()) launch_widen((int)wf.size(), upload(sc, wf, s), s, max_len(wf));
     wf.clear();
     lf.run(sc, s);
   }
   void run_solve(Arena& sc, hipStream_t s) {
-    if (!ws.empty()) launch_widen((int)ws.size(), upload(sc, ws, s), s);
+    if (!ws.empty()) launch_widen((int)ws.size(), upload(sc, ws, s), s, max_len(ws));
     ws.clear();
     ls.run(sc, s);
-    if (!nb.empty()) launch_cvt<float2>((int)nb.size(), upload(sc, nb, s), s);
+    if (!nb.empty()) launch_cvt<float2>((int)nb.size(), upload(sc, nb, s), s, max_len(nb));
     nb.clear();
   }
 };

@@ -4,6 +4,7 @@
 // formation and strided block copies.  One workgroup (4 wave64) per matrix
 // or per output tile; every batch is described by a device array of
 // descriptors built on the host.
+#include <stdexcept>
 #include <type_traits>
 
 #include "common.h"
@@ -234,6 +235,111 @@ __global__ __launch_bounds__(NT) void k_lu(const LUDesc<T>* __restrict__ ds) {
         A[i + (size_t)ld * j] = cfms(A[i + (size_t)ld * k], A[k + (size_t)ld * j], A[i + (size_t)ld * j]);
       }
     __syncthreads();
+  }
+}
+
+// The same LU for a block that fits the LDS (n x n elements, n <= 64): the
+// block is loaded once, factored in LDS with the same operations in the same
+// order (bitwise the global-memory kernel's result) and written back.
+template <class T>
+__global__ __launch_bounds__(NT) void k_lu_lds(const LUDesc<T>* __restrict__ ds, int nmax) {
+  using R = typename CplxOf<T>::real;
+  extern __shared__ __align__(16) unsigned char lu_lds[];
+  T* S = (T*)lu_lds;
+  __shared__ R shv[4];
+  __shared__ int shi[4];
+  const LUDesc<T> d = ds[blockIdx.x];
+  const int n = d.n, ld = d.lda;
+  if (n > nmax) return;  // the launcher gives such blocks to k_lu
+  for (int e = threadIdx.x; e < n * n; e += NT) S[e] = d.A[e % n + (size_t)ld * (e / n)];
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    R bv = -1;
+    int bi = -1;
+    for (int i = k + threadIdx.x; i < n; i += NT) {
+      R v = cabsT(S[i + n * k]);
+      if (bi < 0 || v > bv) bv = v, bi = i;
+    }
+    R pv;
+    int p;
+    block_argmax_first(bv, bi, shv, shi, pv, p);
+    if (p < 0) p = k;
+    if (p != k)
+      for (int j = threadIdx.x; j < n; j += NT) {
+        T t = S[k + n * j];
+        S[k + n * j] = S[p + n * j];
+        S[p + n * j] = t;
+      }
+    if (threadIdx.x == 0) d.piv[k] = p;
+    __syncthreads();
+    T dk = S[k + n * k];
+    bool nz = dk.x != 0 || dk.y != 0;
+    if (nz)
+      for (int i = k + 1 + threadIdx.x; i < n; i += NT) S[i + n * k] = cdiv(S[i + n * k], dk);
+    __syncthreads();
+    int m = n - k - 1;
+    if (nz && m > 0)
+      for (int e = threadIdx.x; e < m * m; e += NT) {
+        int i = k + 1 + e % m, j = k + 1 + e / m;
+        S[i + n * j] = cfms(S[i + n * k], S[k + n * j], S[i + n * j]);
+      }
+    __syncthreads();
+  }
+  for (int e = threadIdx.x; e < n * n; e += NT) d.A[e % n + (size_t)ld * (e / n)] = S[e];
+}
+
+template <class T>
+__device__ __forceinline__ T shfl_c(T v, int lane) {
+  v.x = __shfl(v.x, lane);
+  v.y = __shfl(v.y, lane);
+  return v;
+}
+
+// LU solves of a block with n <= 64 rows: the factors and the row
+// permutation in LDS, one right-hand-side column per wave at a time with row
+// i in lane i, so the substitutions need no workgroup barriers (the same
+// operations per element in the same order as k_lu_solve, bitwise).
+// grid = (matrix, group of LUS_COLS columns)
+constexpr int LUS_COLS = 16;
+template <class T>
+__global__ __launch_bounds__(NT) void k_lu_solve_wave(const SolveDesc<T>* __restrict__ ds, int nmax) {
+  extern __shared__ __align__(16) unsigned char lus_lds[];
+  T* S = (T*)lus_lds;
+  __shared__ int perm[64];
+  const SolveDesc<T> d = ds[blockIdx.x];
+  const int n = d.n, la = d.lda, lb = d.ldb;
+  const int c0 = blockIdx.y * LUS_COLS;
+  if (n > nmax || c0 >= d.nrhs) return;  // the launcher gives larger blocks to k_lu_solve
+  const int nc = d.nrhs - c0 < LUS_COLS ? d.nrhs - c0 : LUS_COLS;
+  for (int e = threadIdx.x; e < n * n; e += NT) S[e] = d.A[e % n + (size_t)la * (e / n)];
+  if (threadIdx.x == 0) {
+    // the row interchanges k <-> piv[k], k = 0..n-1, applied in order
+    for (int i = 0; i < n; ++i) perm[i] = i;
+    for (int k = 0; k < n; ++k) {
+      const int p = d.piv[k];
+      const int t = perm[k];
+      perm[k] = perm[p];
+      perm[p] = t;
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int c = w; c < nc; c += NT / 64) {
+    T* B = d.B + (size_t)lb * (c0 + c);
+    T b = czero<T>();
+    if (lane < n) b = B[perm[lane]];
+    for (int k = 0; k < n - 1; ++k) {
+      const T bk = shfl_c(b, k);
+      if (lane > k && lane < n) b = cfms(S[lane + n * k], bk, b);
+    }
+    for (int k = n - 1; k >= 0; --k) {
+      const T dk = S[k + n * k];
+      const bool nzp = dk.x != 0 || dk.y != 0;
+      if (lane == k) b = nzp ? cdiv(b, dk) : czero<T>();
+      const T bk = shfl_c(b, k);
+      if (lane < k) b = cfms(S[lane + n * k], bk, b);
+    }
+    if (lane < n) B[lane] = b;
   }
 }
 
@@ -560,10 +666,12 @@ __global__ __launch_bounds__(NT) void k_fro(const CopyDesc<T>* __restrict__ ds, 
   if (threadIdx.x == 0) out[blockIdx.x] = sqrt((double)s);
 }
 
+// grid = (descriptor, chunk): the chunks of a descriptor stride its elements
 template <class T>
 __global__ __launch_bounds__(NT) void k_cvt(const CvtDesc<T>* __restrict__ ds) {
   const CvtDesc<T> d = ds[blockIdx.x];
-  for (int64_t e = threadIdx.x; e < d.n; e += NT) d.dst[e] = from2<T>(d.src[e]);
+  for (int64_t e = threadIdx.x + (int64_t)blockIdx.y * NT; e < d.n; e += (int64_t)NT * gridDim.y)
+    d.dst[e] = from2<T>(d.src[e]);
 }
 
 __global__ void k_d2f(const double2* __restrict__ s, float2* __restrict__ d, int64_t n) {
@@ -598,6 +706,26 @@ void launch_lu_solve(int nb, int max_chunks, const SolveDesc<T>* d, hipStream_t 
   hipLaunchKernelGGL(k_lu_solve<T>, dim3(nb, max_chunks), dim3(NT), 0, s, d);
   HIP_OK(hipGetLastError());
 }
+int lu_lds_max(int elem_bytes) {
+  int n = 64;
+  while ((size_t)n * n * elem_bytes > 60 * 1024) --n;
+  return n;
+}
+template <class T>
+void launch_lu_lds(int nb, const LUDesc<T>* d, int nmax, hipStream_t s) {
+  if (!nb) return;
+  if (nmax > lu_lds_max(sizeof(T))) throw std::runtime_error("LU block too large for the LDS kernel");
+  hipLaunchKernelGGL(k_lu_lds<T>, dim3(nb), dim3(NT), (size_t)nmax * nmax * sizeof(T), s, d, nmax);
+  HIP_OK(hipGetLastError());
+}
+template <class T>
+void launch_lu_solve_lds(int nb, int max_cols, const SolveDesc<T>* d, int nmax, hipStream_t s) {
+  if (!nb || !max_cols) return;
+  if (nmax > lu_lds_max(sizeof(T))) throw std::runtime_error("LU block too large for the LDS kernel");
+  hipLaunchKernelGGL(k_lu_solve_wave<T>, dim3(nb, (max_cols + LUS_COLS - 1) / LUS_COLS), dim3(NT),
+                     (size_t)nmax * nmax * sizeof(T), s, d, nmax);
+  HIP_OK(hipGetLastError());
+}
 template <class T>
 void launch_qr_solve(int nb, int max_chunks, const QSolveDesc<T>* d, hipStream_t s) {
   if (!nb || !max_chunks) return;
@@ -630,19 +758,25 @@ void launch_fro(int nb, const CopyDesc<T>* d, double* out, hipStream_t s) {
   hipLaunchKernelGGL(k_fro<T>, dim3(nb), dim3(NT), 0, s, d, out);
   HIP_OK(hipGetLastError());
 }
+// chunks per descriptor for the longest one: about 4 elements per thread
+static unsigned conv_chunks(int64_t max_n) {
+  const int64_t c = (max_n + 4 * NT - 1) / (4 * NT);
+  return (unsigned)(c < 1 ? 1 : c > 1024 ? 1024 : c);
+}
 template <class T>
-void launch_cvt(int nb, const CvtDesc<T>* d, hipStream_t s) {
+void launch_cvt(int nb, const CvtDesc<T>* d, hipStream_t s, int64_t max_n) {
   if (!nb) return;
-  hipLaunchKernelGGL(k_cvt<T>, dim3(nb), dim3(NT), 0, s, d);
+  hipLaunchKernelGGL(k_cvt<T>, dim3(nb, conv_chunks(max_n)), dim3(NT), 0, s, d);
   HIP_OK(hipGetLastError());
 }
 __global__ __launch_bounds__(NT) void k_widen(const WidenDesc* __restrict__ ds) {
   const WidenDesc d = ds[blockIdx.x];
-  for (int64_t i = threadIdx.x; i < d.n; i += NT) d.dst[i] = mk((double)d.src[i].x, (double)d.src[i].y);
+  for (int64_t i = threadIdx.x + (int64_t)blockIdx.y * NT; i < d.n; i += (int64_t)NT * gridDim.y)
+    d.dst[i] = mk((double)d.src[i].x, (double)d.src[i].y);
 }
-void launch_widen(int nb, const WidenDesc* d, hipStream_t s) {
+void launch_widen(int nb, const WidenDesc* d, hipStream_t s, int64_t max_n) {
   if (!nb) return;
-  hipLaunchKernelGGL(k_widen, dim3(nb), dim3(NT), 0, s, d);
+  hipLaunchKernelGGL(k_widen, dim3(nb, conv_chunks(max_n)), dim3(NT), 0, s, d);
   HIP_OK(hipGetLastError());
 }
 void launch_d2f(const double2* src, float2* dst, int64_t n, hipStream_t s) {
@@ -660,12 +794,14 @@ void launch_f2d(const float2* src, double2* dst, int64_t n, hipStream_t s) {
   template void launch_gemm<T>(int, const TileTask*, const GemmDesc<T>*, hipStream_t);  \
   template void launch_lu<T>(int, const LUDesc<T>*, hipStream_t);                       \
   template void launch_lu_solve<T>(int, int, const SolveDesc<T>*, hipStream_t);         \
+  template void launch_lu_lds<T>(int, const LUDesc<T>*, int, hipStream_t);               \
+  template void launch_lu_solve_lds<T>(int, int, const SolveDesc<T>*, int, hipStream_t);  \
   template void launch_cpqr<T>(int, const QRDesc<T>*, hipStream_t);                     \
   template void launch_qr_solve<T>(int, int, const QSolveDesc<T>*, hipStream_t);         \
   template void launch_qform<T>(int, const QFormDesc<T>*, hipStream_t);                 \
   template void launch_copy<T>(int, const CopyDesc<T>*, hipStream_t);                   \
   template void launch_fro<T>(int, const CopyDesc<T>*, double*, hipStream_t);            \
-  template void launch_cvt<T>(int, const CvtDesc<T>*, hipStream_t);
+  template void launch_cvt<T>(int, const CvtDesc<T>*, hipStream_t, int64_t);
 INST(double2)
 INST(float2)
 

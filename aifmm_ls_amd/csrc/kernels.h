@@ -1,5 +1,6 @@
 // Descriptor structs and launchers of the gfx950 kernels.
 #pragma once
+#include <vector>
 #include "common.h"
 
 namespace aifmm {
@@ -170,15 +171,23 @@ struct CvtDesc {
   T* dst;
   int64_t n;
 };
+// max_n: the longest descriptor's n (sets the chunks per descriptor)
 template <class T>
-void launch_cvt(int nb, const CvtDesc<T>* d, hipStream_t s);
+void launch_cvt(int nb, const CvtDesc<T>* d, hipStream_t s, int64_t max_n);
 // dst[i] = (double2) src[i], i < n (complex64 blocks -> FP64 work copies)
 struct WidenDesc {
   const float2* src;
   double2* dst;
   int64_t n;
 };
-void launch_widen(int nb, const WidenDesc* d, hipStream_t s);
+void launch_widen(int nb, const WidenDesc* d, hipStream_t s, int64_t max_n);
+// the longest n of a batch of conversion descriptors
+template <class D>
+inline int64_t max_len(const std::vector<D>& v) {
+  int64_t m = 0;
+  for (const auto& d : v) m = d.n > m ? d.n : m;
+  return m;
+}
 
 template <class T>
 void launch_gemm(int ntiles, const TileTask* tiles, const GemmDesc<T>* d, hipStream_t s);
@@ -186,6 +195,14 @@ template <class T>
 void launch_lu(int nb, const LUDesc<T>* d, hipStream_t s);
 template <class T>
 void launch_lu_solve(int nb, int max_chunks, const SolveDesc<T>* d, hipStream_t s);
+// the same for blocks of at most nmax <= lu_lds_max(sizeof(T)) rows (larger
+// descriptors are skipped): factors in LDS; the solve keeps one column per
+// wave in registers (bitwise the results of the two kernels above)
+int lu_lds_max(int elem_bytes);
+template <class T>
+void launch_lu_lds(int nb, const LUDesc<T>* d, int nmax, hipStream_t s);
+template <class T>
+void launch_lu_solve_lds(int nb, int max_cols, const SolveDesc<T>* d, int nmax, hipStream_t s);
 template <class T>
 void launch_cpqr(int nb, const QRDesc<T>* d, hipStream_t s);
 template <class T>

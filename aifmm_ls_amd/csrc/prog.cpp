@@ -26,7 +26,9 @@ void Program<T>::add(const BoxOp<T>& op) {
     maxrows = std::max(maxrows, o.rows);
     for (const auto& t : o.terms) uses_y |= t.x == nullptr;
   }
-  if ((op.m <= tile_m_ && maxrows <= tile_m_) || (op.whole && op.m <= 1024)) {
+  if (op.tile > 0 && !op.outs.empty()) throw std::runtime_error("row tiles of a chosen height need a box without outputs");
+  const int tm = op.tile > 0 ? op.tile : tile_m_;
+  if ((op.m <= tm && maxrows <= tm) || (op.whole && op.m <= 1024)) {
     PProg<T> p{};
     p.m = op.m;
     p.src = op.src;
@@ -55,9 +57,10 @@ void Program<T>::add(const BoxOp<T>& op) {
   const int ldy = op.save ? op.ldsave : op.ldybuf;
   if (op.m > 0 && !y) throw std::runtime_error("tiled box program without a vector for phase 1");
   if (uses_y && op.m == 0) throw std::runtime_error("box program term reads an empty phase-1 vector");
-  for (int r0 = 0; r0 < op.m; r0 += TILE) {
+  const int th = op.tile > 0 ? op.tile : TILE;
+  for (int r0 = 0; r0 < op.m; r0 += th) {
     PProg<T> p{};
-    p.m = std::min(TILE, op.m - r0);
+    p.m = std::min(th, op.m - r0);
     p.src = op.src ? op.src + r0 : nullptr;
     p.ldsrc = op.ldsrc;
     p.src_alpha = op.src_alpha;

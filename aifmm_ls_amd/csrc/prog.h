@@ -116,6 +116,7 @@ struct BoxOp {
   int ldybuf = 0;
   std::vector<Out> outs;        // phase 2 (at most 2)
   bool whole = false;           // one program even when tall (a few boxes whose outputs read all of y)
+  int tile = 0;                 // rows per program of a box without outputs (0: the program's default)
   void term(const T* A, int lda, int cols, const double2* x, int ldx, double2 alpha) {
     if (cols > 0) terms.push_back(PTerm<T>{A, x, lda, ldx, cols, alpha});
   }

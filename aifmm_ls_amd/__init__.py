@@ -55,6 +55,7 @@ def lib():
         L.aifmm_ls_skeleton.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int)]
         L.aifmm_ls_factor.argtypes = [P, ctypes.c_int]
+        L.aifmm_ls_factor_batch.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, dp]
         L.aifmm_ls_set_pivot_eps.argtypes = [P, ctypes.c_double, ctypes.c_double]
         L.aifmm_ls_factor_trace.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_int)]
@@ -459,6 +460,21 @@ def _params(n_levels, kappa, tol_pow, qchoice, n_cheb, degree, L, bumps, quantiz
         p.bumps = _dp(keep)
         p.n_bumps = keep.shape[0]
     return p, keep
+
+
+def factor_batch(problems, precision=PREC_C64):
+    """Factorise a batch of LSProblems together (aifmm_ls_factor_batch): the
+    eliminations of all of them as one merged level-synchronous launch
+    sequence.  Returns {"wall_s", "factor_s", "rounds", "launches",
+    "launch_s", "gather_s"}: wall seconds (assembly included), seconds of
+    the factorisations, launch rounds, kernel launches, seconds spent
+    launching, and seconds between rounds (the members' host work)."""
+    n = len(problems)
+    hs = (ctypes.c_void_p * n)(*[p.h.value for p in problems])
+    info = np.zeros(6)
+    _chk(lib().aifmm_ls_factor_batch(hs, n, int(precision), _dp(info)))
+    return {"wall_s": float(info[0]), "factor_s": float(info[1]), "rounds": int(info[2]),
+            "launches": int(info[3]), "launch_s": float(info[4]), "gather_s": float(info[5])}
 
 
 def solve_batch(problems, rhs, tol=1e-9, max_refine=30):

@@ -1,11 +1,13 @@
 // Host-side helpers that collect descriptors for the batched kernels,
 // upload them and launch.
 #pragma once
+#include <algorithm>
 #include <type_traits>
 #include <vector>
 
 #include "arena.h"
 #include "kernels.h"
+#include "merge.h"
 #include "prof.h"
 
 namespace aifmm {
@@ -66,6 +68,55 @@ D* upload(Arena& a, const std::vector<D>& v, hipStream_t s) {
   return d;
 }
 
+// The descriptors of n batches of one type as one vector (the batches are
+// left empty): a merged launch of several host threads' batches.
+template <class B, class D>
+std::vector<D> take_all(void* const* bs, int n, std::vector<D> B::*mem) {
+  std::vector<D> all;
+  if (n == 1) {
+    all.swap(((B*)bs[0])->*mem);
+    return all;
+  }
+  size_t tot = 0;
+  for (int i = 0; i < n; ++i) tot += (((B*)bs[i])->*mem).size();
+  all.reserve(tot);
+  for (int i = 0; i < n; ++i) {
+    std::vector<D>& v = ((B*)bs[i])->*mem;
+    all.insert(all.end(), v.begin(), v.end());
+    v.clear();
+  }
+  return all;
+}
+
+// Launch context of a merged run: one stream for every member, and scratch
+// for the merged descriptor uploads (reset after each round: the next
+// round's uploads are ordered after this round's launches on the stream).
+struct MergeCtx {
+  hipStream_t s = nullptr;
+  Arena sc{size_t(16) << 20};
+  static void round_end(void* c) { ((MergeCtx*)c)->sc.reset(); }
+};
+template <class Batch>
+int run_merged(void* const* bs, int n, void* ctx) {
+  MergeCtx* c = (MergeCtx*)ctx;
+  return Batch::run_many(bs, n, c->sc, c->s);
+}
+// Run a batch through the calling thread's group, or launch it alone.
+template <class Batch>
+inline void merge_or_run(Batch* b, Arena& scratch, hipStream_t s) {
+  if (MergeGroup* g = merge_group()) {
+    g->submit(&run_merged<Batch>, b);
+    return;
+  }
+  void* one = b;
+  Batch::run_many(&one, 1, scratch, s);
+}
+
+// Every batch type: add() collects descriptors, run() launches them (merged
+// with the other members' batches of the same type inside a MergeGroup; an
+// empty batch then still takes part in the round, which keeps problems of
+// equal structure in step), run_many() is the launch over several batches.
+
 // Every descriptor of one launch must write a distinct output block: tiles of
 // different descriptors run concurrently.
 template <class T>
@@ -98,7 +149,12 @@ struct GemmBatch {
     d.push_back(g);
   }
   void run(Arena& scratch, hipStream_t s) {
-    if (d.empty()) return;
+    if (d.empty() && !merge_group()) return;
+    merge_or_run(this, scratch, s);
+  }
+  static int run_many(void* const* bs, int n, Arena& scratch, hipStream_t s) {
+    std::vector<GemmDesc<T>> d = take_all(bs, n, &GemmBatch::d);
+    if (d.empty()) return 0;
     std::vector<TileTask> tiles;
     for (int i = 0; i < (int)d.size(); ++i) {
       int tm = (d[i].m + 31) / 32, tn = (d[i].n + 31) / 32;
@@ -116,7 +172,7 @@ struct GemmBatch {
       }
     ProfScope ps(s, KNAME(T, "k_gemm"), by, fl);
     launch_gemm<T>((int)tiles.size(), dt, dd, s);
-    d.clear();
+    return 1;
   }
 };
 
@@ -139,14 +195,19 @@ struct CopyBatch {
     d.push_back(c);
   }
   void run(Arena& scratch, hipStream_t s) {
-    if (d.empty()) return;
+    if (d.empty() && !merge_group()) return;
+    merge_or_run(this, scratch, s);
+  }
+  static int run_many(void* const* bs, int n, Arena& scratch, hipStream_t s) {
+    std::vector<CopyDesc<T>> d = take_all(bs, n, &CopyBatch::d);
+    if (d.empty()) return 0;
     auto* dd = upload(scratch, d, s);
     double by = 0;
     if (prof_on())
       for (auto& c : d) by += sizeof(T) * (double)c.nr * c.nc * ((c.trans >= 3 ? 1 : 2) + c.accumulate);
     ProfScope ps(s, KNAME(T, "k_copy"), by, 0);
     launch_copy<T>((int)d.size(), dd, s);
-    d.clear();
+    return 1;
   }
 };
 
@@ -158,7 +219,12 @@ struct LUBatch {
     d.push_back(LUDesc<T>{A.p, piv, A.r, A.ld});
   }
   void run(Arena& scratch, hipStream_t s) {
-    if (d.empty()) return;
+    if (d.empty() && !merge_group()) return;
+    merge_or_run(this, scratch, s);
+  }
+  static int run_many(void* const* bs, int n, Arena& scratch, hipStream_t s) {
+    std::vector<LUDesc<T>> d = take_all(bs, n, &LUBatch::d);
+    if (d.empty()) return 0;
     auto* dd = upload(scratch, d, s);
     double by = 0, fl = 0;
     if (prof_on())
@@ -174,7 +240,7 @@ struct LUBatch {
     }
     if (nsmall) launch_lu_lds<T>((int)d.size(), dd, nsmall, s);
     if (!big.empty()) launch_lu<T>((int)big.size(), upload(scratch, big, s), s);
-    d.clear();
+    return (nsmall ? 1 : 0) + (big.empty() ? 0 : 1);
   }
 };
 
@@ -189,7 +255,13 @@ struct SolveBatch {
     if (ch > maxc) maxc = ch;
   }
   void run(Arena& scratch, hipStream_t s) {
-    if (d.empty()) return;
+    maxc = 0;
+    if (d.empty() && !merge_group()) return;
+    merge_or_run(this, scratch, s);
+  }
+  static int run_many(void* const* bs, int n, Arena& scratch, hipStream_t s) {
+    std::vector<SolveDesc<T>> d = take_all(bs, n, &SolveBatch::d);
+    if (d.empty()) return 0;
     auto* dd = upload(scratch, d, s);
     double by = 0, fl = 0;
     if (prof_on())
@@ -209,8 +281,7 @@ struct SolveBatch {
     }
     if (nsmall) launch_lu_solve_lds<T>((int)d.size(), csmall, dd, nsmall, s);
     if (!big.empty()) launch_lu_solve<T>((int)big.size(), cbig, upload(scratch, big, s), s);
-    d.clear();
-    maxc = 0;
+    return (nsmall ? 1 : 0) + (big.empty() ? 0 : 1);
   }
 };
 
@@ -259,14 +330,19 @@ struct QRBatch {
     d.push_back(q);
   }
   void run(Arena& scratch, hipStream_t s) {
-    if (d.empty()) return;
+    if (d.empty() && !merge_group()) return;
+    merge_or_run(this, scratch, s);
+  }
+  static int run_many(void* const* bs, int n, Arena& scratch, hipStream_t s) {
+    std::vector<QRDesc<T>> d = take_all(bs, n, &QRBatch::d);
+    if (d.empty()) return 0;
     auto* dd = upload(scratch, d, s);
     double fl = 0;
     if (prof_on())
       for (auto& q : d) fl += 8.0 * ((double)q.m * q.n * q.n - (double)q.n * q.n * q.n / 3);
     ProfScope ps(s, KNAME(T, "k_cpqr"), 0, fl);
     launch_cpqr<T>((int)d.size(), dd, s);
-    d.clear();
+    return 1;
   }
 };
 
@@ -293,15 +369,105 @@ struct QRSolveBatch {
     if (ch > maxc) maxc = ch;
   }
   void run(Arena& scratch, hipStream_t s) {
-    if (d.empty()) return;
+    maxc = 0;
+    if (d.empty() && !merge_group()) return;
+    merge_or_run(this, scratch, s);
+  }
+  static int run_many(void* const* bs, int n, Arena& scratch, hipStream_t s) {
+    std::vector<QSolveDesc<T>> d = take_all(bs, n, &QRSolveBatch::d);
+    if (d.empty()) return 0;
+    int maxc = 0;
+    for (auto& q : d) maxc = std::max(maxc, (q.nrhs + 15) / 16);
     auto* dd = upload(scratch, d, s);
     double by = 0, fl = 0;
     if (prof_on())
       for (auto& q : d) by += sizeof(T) * ((double)q.n * q.n + 2.0 * q.n * q.nrhs), fl += 16.0 * q.n * q.n * q.nrhs;
     ProfScope ps(s, KNAME(T, "k_qr_solve"), by, fl);
     launch_qr_solve<T>((int)d.size(), maxc, dd, s);
-    d.clear();
-    maxc = 0;
+    return 1;
+  }
+};
+
+// Householder vectors of a column-pivoted QR expanded into the first k
+// columns of Q.
+template <class T>
+struct QFormBatch {
+  std::vector<QFormDesc<T>> d;
+  void run(Arena& scratch, hipStream_t s) {
+    if (d.empty() && !merge_group()) return;
+    merge_or_run(this, scratch, s);
+  }
+  static int run_many(void* const* bs, int n, Arena& scratch, hipStream_t s) {
+    std::vector<QFormDesc<T>> d = take_all(bs, n, &QFormBatch::d);
+    if (d.empty()) return 0;
+    auto* dd = upload(scratch, d, s);
+    ProfScope ps(s, KNAME(T, "k_qform"), 0, 0);
+    launch_qform<T>((int)d.size(), dd, s);
+    return 1;
+  }
+};
+
+// Frobenius norms: block (src, nr x nc, lds) -> *out.
+template <class T>
+struct FroBatch {
+  std::vector<CopyDesc<T>> d;
+  void add(const T* src, int nr, int nc, int lds, double* out) {
+    CopyDesc<T> c{};
+    c.src = src;
+    c.dst = (T*)out;  // the kernel writes the norm through dst
+    c.nr = nr;
+    c.nc = nc;
+    c.lds = lds;
+    d.push_back(c);
+  }
+  void run(Arena& scratch, hipStream_t s) {
+    if (d.empty() && !merge_group()) return;
+    merge_or_run(this, scratch, s);
+  }
+  static int run_many(void* const* bs, int n, Arena& scratch, hipStream_t s) {
+    std::vector<CopyDesc<T>> d = take_all(bs, n, &FroBatch::d);
+    if (d.empty()) return 0;
+    auto* dd = upload(scratch, d, s);
+    ProfScope ps(s, KNAME(T, "k_fro"), 0, 0);
+    launch_fro<T>((int)d.size(), dd, nullptr, s);
+    return 1;
+  }
+};
+
+// Element-wise FP64 -> working precision conversions.
+template <class T>
+struct CvtBatch {
+  std::vector<CvtDesc<T>> d;
+  void add(const double2* src, T* dst, int64_t n) {
+    if (n > 0) d.push_back(CvtDesc<T>{src, dst, n});
+  }
+  void run(Arena& scratch, hipStream_t s) {
+    if (d.empty() && !merge_group()) return;
+    merge_or_run(this, scratch, s);
+  }
+  static int run_many(void* const* bs, int n, Arena& scratch, hipStream_t s) {
+    std::vector<CvtDesc<T>> d = take_all(bs, n, &CvtBatch::d);
+    if (d.empty()) return 0;
+    launch_cvt<T>((int)d.size(), upload(scratch, d, s), s, max_len(d));
+    return 1;
+  }
+};
+
+// Element-wise complex64 -> FP64 conversions.
+struct WidenBatch {
+  std::vector<WidenDesc> d;
+  void add(const float2* src, double2* dst, int64_t n) {
+    if (n > 0) d.push_back(WidenDesc{src, dst, n});
+  }
+  void run(Arena& scratch, hipStream_t s) {
+    if (d.empty() && !merge_group()) return;
+    merge_or_run(this, scratch, s);
+  }
+  static int run_many(void* const* bs, int n, Arena& scratch, hipStream_t s) {
+    std::vector<WidenDesc> d = take_all(bs, n, &WidenBatch::d);
+    if (d.empty()) return 0;
+    launch_widen((int)d.size(), upload(scratch, d, s), s, max_len(d));
+    return 1;
   }
 };
 

@@ -4,13 +4,16 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <exception>
 #include <string>
+#include <thread>
 #include <vector>
 #include <algorithm>
 #include <type_traits>
 
 #include "../../include/aifmm_ls.h"
 #include "factor.h"
+#include "merge.h"
 #include "prof.h"
 #include "krylov.h"
 #include "h2.h"
@@ -251,6 +254,101 @@ int aifmm_ls_factor(aifmm_ls_problem* h, int precision) {
       h->f32 = std::move(f);
     }
     h->precision = precision;
+  })
+}
+
+// One host thread per problem, all on one stream, their batched launches
+// merged round by round (MergeGroup): the eliminations of every problem's
+// level step, colour by colour, go out as one launch sequence.  Problems of
+// different depth start together at their leaf levels.  Each factorisation
+// equals the one aifmm_ls_factor makes (every descriptor is computed alone).
+int aifmm_ls_factor_batch(aifmm_ls_problem* const* hs, int n, int precision, double* info) {
+  GUARD({
+    if (precision != 0 && precision != 1) throw std::runtime_error("precision must be 0 (c128) or 1 (c64)");
+    if (n < 1 || !hs) throw std::runtime_error("empty batch");
+    for (int i = 0; i < n; ++i) {
+      if (!hs[i]) throw std::runtime_error("null handle in batch");
+      for (int k = 0; k < i; ++k)
+        if (hs[k] == hs[i]) throw std::runtime_error("a problem appears twice in the batch");
+    }
+    const double t0 = now_s();
+    std::vector<std::exception_ptr> err(n);
+    auto each = [&](const std::function<void(int)>& fn) {
+      std::vector<std::thread> th;
+      th.reserve(n);
+      for (int i = 0; i < n; ++i) th.emplace_back([&, i] {
+        try {
+          fn(i);
+        } catch (...) {
+          err[i] = std::current_exception();
+        }
+      });
+      for (auto& t : th) t.join();
+    };
+    // assembly: each problem on its own stream, concurrently
+    each([&](int i) {
+      ApiScope api;
+      hs[i]->pb->assemble();
+      HIP_OK(hipStreamSynchronize(hs[i]->pb->stream));
+    });
+    for (int i = 0; i < n; ++i)
+      if (err[i]) std::rethrow_exception(err[i]);
+    const double t1 = now_s();
+    for (int i = 0; i < n; ++i) {
+      hs[i]->precision = -1;
+      hs[i]->f64.reset();
+      hs[i]->f32.reset();
+    }
+    MergeCtx mc;
+    mc.s = stream_create(hipStreamNonBlocking);
+    const hipStream_t gs = mc.s;
+    MergeGroup grp(n, &mc, &MergeCtx::round_end);
+    each([&](int i) {
+      MergeMember member(&grp);  // first: any exit leaves the group
+      ApiScope api;              // counted inside the library: no graph capture meanwhile
+      aifmm_ls_problem* h = hs[i];
+      Problem& pb = *h->pb;
+      const hipStream_t own = pb.stream;
+      pb.stream = gs;
+      struct Back {
+        Problem& pb;
+        hipStream_t s;
+        ~Back() { pb.stream = s; }
+      } back{pb, own};
+      if (precision == 0) {
+        std::unique_ptr<Factor<double2>> f(new Factor<double2>(pb));
+        f->l2mf_eps = h->l2mf_eps;
+        f->base_eps = h->base_eps;
+        f->factor();
+        HIP_OK(hipStreamSynchronize(gs));
+        h->f64 = std::move(f);
+      } else {
+        std::unique_ptr<Factor<float2>> f(new Factor<float2>(pb));
+        f->l2mf_eps = h->l2mf_eps;
+        f->base_eps = h->base_eps;
+        f->factor();
+        HIP_OK(hipStreamSynchronize(gs));
+        h->f32 = std::move(f);
+      }
+      h->precision = precision;
+    });
+    stream_destroy(gs);
+    if (info) {
+      info[0] = now_s() - t0;  // wall seconds: assembly + factorisation
+      info[1] = now_s() - t1;  // wall seconds of the merged factorisations
+      info[2] = (double)grp.rounds();
+      info[3] = (double)grp.launches();
+      info[4] = grp.launch_s();
+      info[5] = grp.gather_s();
+    }
+    for (int i = 0; i < n; ++i)
+      if (err[i]) {
+        try {
+          std::rethrow_exception(err[i]);
+        } catch (const std::exception& e) {
+          throw std::runtime_error("batch member " + std::to_string(i) + ": " + e.what());
+        }
+      }
   })
 }
 

@@ -57,10 +57,10 @@ void Factor<T>::init_level(int j) {
   int nb = 1 << (2 * j);
   B[j].assign(nb, FBox());
   colors[j].assign(9, {});
-  std::vector<CvtDesc<T>> cv;
+  CvtBatch<T> cv;
   auto conv = [&](const Blk<double2>& s) {
     Blk<T> d = new_blk<T>(ar, s.r, s.c);
-    if (s.size()) cv.push_back(CvtDesc<T>{s.p, d.p, (int64_t)s.size()});
+    cv.add(s.p, d.p, (int64_t)s.size());
     return d;
   };
   for (int k = 0; k < nb; ++k) {
@@ -78,8 +78,7 @@ void Factor<T>::init_level(int j) {
     if (j == L)
       for (auto& e : pb.P2P[k]) F.P2P[e.first] = conv(e.second);
   }
-  auto* d = upload(sc, cv, pb.stream);
-  launch_cvt<T>((int)cv.size(), d, pb.stream, max_len(cv));
+  cv.run(sc, pb.stream);
   sc.reset();  // scratch reuse is ordered on the stream
 }
 
@@ -304,7 +303,7 @@ void Factor<T>::recompress(int j, int c) {
     int u, p, kind;
   };
   std::vector<Fill> cand;
-  std::vector<CopyDesc<T>> nd;
+  std::vector<const Blk<T>*> cb;
   auto blk_of = [&](const Fill& f) -> Blk<T>& {
     FBox& U = Bx[f.u];
     return f.kind == 0 ? U.P2P[f.p] : f.kind == 1 ? U.M2P[f.p] : U.P2L[f.p];
@@ -328,25 +327,21 @@ void Factor<T>::recompress(int j, int c) {
         continue;
       }
       cand.push_back(Fill{u, p, kind});
-      CopyDesc<T> d{};
-      d.src = it->second.p;
-      d.nr = it->second.r;
-      d.nc = it->second.c;
-      d.lds = it->second.ld;
-      nd.push_back(d);
+      cb.push_back(&it->second);
     }
   }
+  // (inside a merged run every member takes part in each launch round, so
+  // an early return comes after the rounds of this function's first launch)
+  FroBatch<T> fro;
+  double* dn = cand.empty() ? nullptr : sc.get<double>(cand.size());
+  for (size_t i = 0; i < cand.size(); ++i) fro.add(cb[i]->p, cb[i]->r, cb[i]->c, cb[i]->ld, dn + i);
+  fro.run(sc, pb.stream);
   if (cand.empty()) return;
   std::vector<double> norms(cand.size());
   {
-    auto* dd = upload(sc, nd, pb.stream);
-    double* dn = sc.get<double>(cand.size());
-    ProfScope ps(pb.stream, KNAME(T, "k_fro"), 0, 0);
-    launch_fro<T>((int)nd.size(), dd, dn, pb.stream);
     const double w0 = now_s();
     d2h(norms.data(), dn, 8 * cand.size(), pb.stream);
     t_wait_ += now_s() - w0;
-    HIP_OK(hipStreamSynchronize(pb.stream));
   }
   std::vector<Fill> fills;
   for (size_t i = 0; i < cand.size(); ++i) {
@@ -410,7 +405,8 @@ void Factor<T>::recompress(int j, int c) {
   }
   cp.run(sc, pb.stream);
   // CPQR of both stacks
-  std::vector<QRDesc<T>> qd;
+  QRBatch<T> qb;
+  std::vector<QRDesc<T>>& qd = qb.d;
   int* dr = sc.get<int>(2 * nt);
   std::vector<T*> taus(2 * nt);
   for (int i = 0; i < 2 * nt; ++i) {
@@ -434,16 +430,11 @@ void Factor<T>::recompress(int j, int c) {
     double mm = A.r, nn = A.c;
     st.flops += mm >= nn ? 8.0 * (mm * nn * nn - nn * nn * nn / 3) : 8.0 * (nn * mm * mm - mm * mm * mm / 3);
   }
-  {
-    auto* dq = upload(sc, qd, pb.stream);
-    ProfScope ps(pb.stream, KNAME(T, "k_cpqr"), 0, 0);
-    launch_cpqr<T>((int)qd.size(), dq, pb.stream);
-  }
+  qb.run(sc, pb.stream);
   std::vector<int> rk(2 * nt);
   const double w1 = now_s();
   d2h(rk.data(), dr, 8 * nt, pb.stream);
   t_wait_ += now_s() - w1;
-  HIP_OK(hipStreamSynchronize(pb.stream));
   std::vector<int> rnew(nb, -1);
   for (int i = 0; i < nt; ++i) {
     const int u = tl[i];
@@ -457,7 +448,8 @@ void Factor<T>::recompress(int j, int c) {
     fprintf(stderr, "\n");
   }
   std::vector<Blk<T>> QR(nt), QC(nt), Rrow(nt), Rcol(nt);
-  std::vector<QFormDesc<T>> qf;
+  QFormBatch<T> qfb;
+  std::vector<QFormDesc<T>>& qf = qfb.d;
   GemmBatch<T> g;
   for (int i = 0; i < nt; ++i) {
     int u = tl[i];
@@ -475,11 +467,7 @@ void Factor<T>::recompress(int j, int c) {
     Rrow[i] = new_blk<T>(ar, r, U.r);
     Rcol[i] = new_blk<T>(ar, U.r, r);
   }
-  {
-    auto* d = upload(sc, qf, pb.stream);
-    ProfScope ps(pb.stream, KNAME(T, "k_qform"), 0, 0);
-    launch_qform<T>((int)qf.size(), d, pb.stream);
-  }
+  qfb.run(sc, pb.stream);
   for (int i = 0; i < nt; ++i) {
     FBox& U = Bx[tl[i]];
     if (rnew[tl[i]] && U.r) {
@@ -863,15 +851,15 @@ struct Composer {
       g.run(sc, s);
       cp.run(sc, s);
     } else {
-      std::vector<WidenDesc> wd;
-      std::vector<CvtDesc<float2>> nd;
+      WidenBatch wd;
+      CvtBatch<float2> nd;
       std::map<const float2*, Blk<double2>> wide;  // each source widened once
       auto widen = [&](const Blk<float2>& x) {
         if (x.ld != x.r) throw std::runtime_error("sweep composition needs contiguous blocks");
         auto it = wide.find(x.p);
         if (it != wide.end()) return it->second;
         Blk<double2> w = new_blk<double2>(sc, x.r, x.c);
-        wd.push_back(WidenDesc{x.p, w.p, (int64_t)x.size()});
+        wd.add(x.p, w.p, (int64_t)x.size());
         wide[x.p] = w;
         return w;
       };
@@ -880,11 +868,11 @@ struct Composer {
         Blk<double2> t = new_blk<double2>(sc, j.dst.r, j.dst.c);
         g.add(a, 0, b, 0, t, ONE, ZERO);
         if (j.dst.ld != j.dst.r) throw std::runtime_error("sweep composition needs contiguous blocks");
-        nd.push_back(CvtDesc<float2>{t.p, j.dst.p, (int64_t)j.dst.size()});
+        nd.add(t.p, j.dst.p, (int64_t)j.dst.size());
       }
-      launch_widen((int)wd.size(), upload(sc, wd, s), s, max_len(wd));
+      wd.run(sc, s);
       g.run(sc, s);  // every product reads the widened couplings before any is replaced
-      launch_cvt<float2>((int)nd.size(), upload(sc, nd, s), s, max_len(nd));
+      nd.run(sc, s);
     }
     jobs.clear();
     bytes = 0;
@@ -1269,8 +1257,9 @@ void Factor<T>::dense_top_fill() {
     cp.add(x2, d2);
     cp.run(sc, pb.stream);
   } else {
-    std::vector<CvtDesc<T>> cv{CvtDesc<T>{x2.p, d2.p, (int64_t)n * n}};
-    launch_cvt<T>(1, upload(sc, cv, pb.stream), pb.stream, max_len(cv));
+    CvtBatch<T> cv;
+    cv.add(x2.p, d2.p, (int64_t)n * n);
+    cv.run(sc, pb.stream);
   }
   HIP_OK(hipStreamSynchronize(pb.stream));  // before P and its launches go
 }

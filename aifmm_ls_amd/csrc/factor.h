@@ -63,8 +63,8 @@ struct DenseOps<double2> {
 
 template <>
 struct DenseOps<float2> {
-  std::vector<WidenDesc> wf, ws;   // widen for factor / for solve
-  std::vector<CvtDesc<float2>> nb; // narrow solve results
+  WidenBatch wf, ws;               // widen for factor / for solve
+  CvtBatch<float2> nb;             // narrow solve results
   LUBatch<double2> lf;
   SolveBatch<double2> ls;
   static Blk<double2> wide(Arena& a, const Blk<float2>& b) { return new_blk<double2>(a, b.r, b.c); }
@@ -79,30 +79,27 @@ struct DenseOps<float2> {
     f.a64 = wide(ar, A);
     f.piv = ar.get<int>(A.r > 0 ? A.r : 1);
     if (!A.size()) return;
-    wf.push_back(WidenDesc{A.p, f.a64.p, (int64_t)A.size()});
+    wf.add(A.p, f.a64.p, (int64_t)A.size());
     lf.add(f.a64, f.piv);
   }
   void solve(Arena& sc, const DFact<float2>& f, const Blk<float2>& B) {
     need_contig(B);
     if (!f.a64.r || !B.c) return;
     Blk<double2> b64 = wide(sc, B);
-    ws.push_back(WidenDesc{B.p, b64.p, (int64_t)B.size()});
+    ws.add(B.p, b64.p, (int64_t)B.size());
     ls.add(f.a64, f.piv, b64);
-    nb.push_back(CvtDesc<float2>{b64.p, B.p, (int64_t)B.size()});
+    nb.add(b64.p, B.p, (int64_t)B.size());
   }
   // the FP64 array holding the factor (U on and above the diagonal)
   static Blk<double2> fp64_factor(const DFact<float2>& f) { return f.a64; }
   void run_factor(Arena& sc, hipStream_t s) {
-    if (!wf.empty()) launch_widen((int)wf.size(), upload(sc, wf, s), s, max_len(wf));
-    wf.clear();
+    wf.run(sc, s);
     lf.run(sc, s);
   }
   void run_solve(Arena& sc, hipStream_t s) {
-    if (!ws.empty()) launch_widen((int)ws.size(), upload(sc, ws, s), s, max_len(ws));
-    ws.clear();
+    ws.run(sc, s);
     ls.run(sc, s);
-    if (!nb.empty()) launch_cvt<float2>((int)nb.size(), upload(sc, nb, s), s, max_len(nb));
-    nb.clear();
+    nb.run(sc, s);
   }
 };
 

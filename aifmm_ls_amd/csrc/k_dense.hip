@@ -650,7 +650,8 @@ __global__ __launch_bounds__(NT) void k_copy(const CopyDesc<T>* __restrict__ ds)
   }
 }
 
-// Frobenius norms of a batch of blocks (one workgroup each).
+// Frobenius norms of a batch of blocks (one workgroup each): into out[i], or
+// through descriptor i's dst when out is null.
 template <class T>
 __global__ __launch_bounds__(NT) void k_fro(const CopyDesc<T>* __restrict__ ds, double* __restrict__ out) {
   using R = typename CplxOf<T>::real;
@@ -663,7 +664,8 @@ __global__ __launch_bounds__(NT) void k_fro(const CopyDesc<T>* __restrict__ ds, 
     s += cabs2(d.src[r + (size_t)d.lds * c]);
   }
   s = block_sumR(s, sh);
-  if (threadIdx.x == 0) out[blockIdx.x] = sqrt((double)s);
+  // out null: each descriptor names its own output in dst
+  if (threadIdx.x == 0) *(out ? out + blockIdx.x : (double*)d.dst) = sqrt((double)s);
 }
 
 // grid = (descriptor, chunk): the chunks of a descriptor stride its elements

@@ -209,6 +209,7 @@ template <class T>
 void launch_qform(int nb, const QFormDesc<T>* d, hipStream_t s);
 template <class T>
 void launch_copy(int nb, const CopyDesc<T>* d, hipStream_t s);
+// norms into out[i], or (out null) into *(double*)d[i].dst
 template <class T>
 void launch_fro(int nb, const CopyDesc<T>* d, double* out, hipStream_t s);
 // refinement vectors: r = b - ax with per-block partial squared norms

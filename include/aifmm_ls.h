@@ -80,6 +80,17 @@ int aifmm_ls_skeleton(aifmm_ls_problem* h, int level, int box, int which, int64_
 /* FMM2DTree::eliminate_phase_efficient (FMM2DTree_find_error.hpp:2407-2451), here the
  * level-synchronous colour schedule.  precision: 0 = complex128, 1 = complex64. */
 int aifmm_ls_factor(aifmm_ls_problem* h, int precision);
+/* The same for n independent problems together: FMM2DTree::assemble_M2L ..
+ * eliminate_phase_efficient (FMM2DTree_find_error.hpp:1338-1361, 2407-2451) of
+ * every problem, where the reference factorises one problem per process.
+ * Assembly runs concurrently per problem; the eliminations of all problems
+ * then go out as one level-synchronous launch sequence (problems of different
+ * depth start together at their leaf levels).  Each factorisation equals what
+ * aifmm_ls_factor gives.  info (optional, 6 doubles): wall seconds in all,
+ * wall seconds of the factorisations, launch rounds, kernel launches, seconds
+ * spent launching, seconds between rounds (the members' host work).  On
+ * failure the members that failed are left unfactored. */
+int aifmm_ls_factor_batch(aifmm_ls_problem* const* hs, int n, int precision, double* info);
 /* Zero-pivot epsilons of the next complex128 factorisation, in Eigen's
  * ColPivHouseholderQR "nonzero pivot" test (a pivot counts as zero below
  * eps x the largest column norm; the reference uses the FP64 epsilon,

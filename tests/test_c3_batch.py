@@ -117,6 +117,46 @@ def test_batch_fp64_direct_vs_oracle(batch):
         assert d < max(10 * 1e-6, 20 * sens)
 
 
+@pytest.mark.parametrize("prec", [A.PREC_C128, A.PREC_C64], ids=["c128", "c64"])
+def test_merged_factorisations_equal_alone(batch, prec):
+    """aifmm_ls_factor_batch: the eliminations of several problems of levels
+    3, 4 and 5 as one merged launch sequence.  Every descriptor of a merged
+    launch is computed as it is alone, so each problem's recompression
+    ranks, its zeroed pivots and its direct solve (launch tuning off, so
+    both sweeps sum in one order) must equal those of its own
+    factorisation bitwise."""
+    cfg, gs, bs = batch
+    sub = [i for lv in (3, 4, 5) for i in [k for k, c in enumerate(cfg) if c["levels"] == lv][:3]]
+    A.set_tuning(0)
+    try:
+        alone = []
+        for i in sub:
+            gs[i].factor(prec)
+            x, _ = gs[i].solve(bs[i], max_refine=0)
+            alone.append((x, gs[i].factor_trace(0), gs[i].factor_trace(1)))
+        info = A.factor_batch([gs[i] for i in sub], prec)
+        print(f"  merged factorisation of {len(sub)} problems: {info}", flush=True)
+        assert info["rounds"] > 0 and info["launches"] > 0
+        for i, (xa, ta, za) in zip(sub, alone):
+            x, _ = gs[i].solve(bs[i], max_refine=0)
+            assert np.array_equal(gs[i].factor_trace(0), ta)
+            assert np.array_equal(gs[i].factor_trace(1), za)
+            assert np.array_equal(x, xa), (i, rel(x, xa))
+        # refined from the merged factors, as from any factorisation
+        xr, sr = gs[sub[-1]].solve(bs[sub[-1]], tol=TOL, max_refine=40)
+        assert sr[2] <= TOL
+    finally:
+        A.set_tuning(1)
+
+
+def test_merged_factorisation_refusals(batch):
+    cfg, gs, bs = batch
+    with pytest.raises(A.AifmmError):
+        A.factor_batch([gs[0], gs[0]])
+    with pytest.raises(A.AifmmError):
+        A.factor_batch(gs[:2], precision=5)
+
+
 def test_batch_rejects_mixed_precision(batch):
     cfg, gs, bs = batch
     gs[0].factor(A.PREC_C64)

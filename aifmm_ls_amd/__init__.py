@@ -57,6 +57,7 @@ def lib():
         L.aifmm_ls_factor.argtypes = [P, ctypes.c_int]
         L.aifmm_ls_factor_batch.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_int, ctypes.c_int, dp]
         L.aifmm_ls_set_pivot_eps.argtypes = [P, ctypes.c_double, ctypes.c_double]
+        L.aifmm_ls_set_growth_gate.argtypes = [P, ctypes.c_double]
         L.aifmm_ls_factor_trace.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.c_int,
                                             ctypes.POINTER(ctypes.c_int)]
         L.aifmm_ls_solve.argtypes = [P, ctypes.c_int, dp, dp, ctypes.c_double, ctypes.c_int, dp]
@@ -149,7 +150,7 @@ def profile_read(reset=True, max_kernels=64):
 STAT_NAMES = ["t_tables", "t_assemble", "t_factor", "factor_flops", "factor_bytes", "base_n", "max_rank",
               "solve_bytes_1rhs", "recompressed_boxes", "operator_bytes", "t_solve", "t_nca", "ill_boxes",
               "sweep_tuned_time_ratio", "sweep_tuned_stages", "dense_top_level", "dense_top_n",
-              "base_rdiag_ratio", "zeroed_pivot_blocks"]
+              "base_rdiag_ratio", "zeroed_pivot_blocks", "pivot_blocks_c64", "pivot_blocks_fp64"]
 
 
 class LSProblem:
@@ -262,6 +263,12 @@ class LSProblem:
         pivot blocks (default 1e-12) and the level-2 base system (default 0 =
         Eigen's rule, the reference's)."""
         _chk(lib().aifmm_ls_set_pivot_eps(self.h, float(l2mf_eps), float(base_eps)))
+
+    def set_growth_gate(self, thr=1e4):
+        """Pivot blocks of the next complex64 factorisation: thr < 0 factors
+        all of them in FP64; thr >= 0 (default 1e4) keeps a block's complex64
+        LU when its growth factor times its condition estimate is at most thr."""
+        _chk(lib().aifmm_ls_set_growth_gate(self.h, float(thr)))
 
     def factor_trace(self, which=0):
         """which 0: recompressions of the factorisation (rows: level, colour
@@ -408,6 +415,16 @@ def selftest_lu_solve(A, B, c64=False):
     C = np.zeros(2 * m * n)
     _chk(lib().aifmm_ls_selftest(10 * c64 + 1, m, n, 0, _dp(_cm(A)), _dp(_cm(B)), _dp(C), None, 0.0))
     return C.view(np.complex128).reshape(n, m).T
+
+
+def selftest_gated_lu_solve(A, B, thr):
+    """A^{-1} B through the complex64 pivot-block growth gate at threshold thr;
+    returns (X, estimate, went_to_fp64)."""
+    m, n = B.shape
+    C = np.zeros(2 * m * n + 2)
+    fl = (ctypes.c_int * 1)()
+    _chk(lib().aifmm_ls_selftest(14, m, n, 0, _dp(_cm(A)), _dp(_cm(B)), _dp(C), fl, float(thr)))
+    return C[:2 * m * n].view(np.complex128).reshape(n, m).T, float(C[2 * m * n]), bool(fl[0])
 
 
 def selftest_qr_solve(A, B, c64=False):

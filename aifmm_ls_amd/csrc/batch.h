@@ -214,9 +214,9 @@ struct CopyBatch {
 template <class T>
 struct LUBatch {
   std::vector<LUDesc<T>> d;
-  void add(const Blk<T>& A, int* piv) {
+  void add(const Blk<T>& A, int* piv, const int* gate = nullptr, int want = 0) {
     if (A.r == 0) return;
-    d.push_back(LUDesc<T>{A.p, piv, A.r, A.ld});
+    d.push_back(LUDesc<T>{A.p, piv, A.r, A.ld, gate, want});
   }
   void run(Arena& scratch, hipStream_t s) {
     if (d.empty() && !merge_group()) return;
@@ -248,9 +248,9 @@ template <class T>
 struct SolveBatch {
   std::vector<SolveDesc<T>> d;
   int maxc = 0;
-  void add(const Blk<T>& LU, const int* piv, const Blk<T>& B) {
+  void add(const Blk<T>& LU, const int* piv, const Blk<T>& B, const int* gate = nullptr, int want = 0) {
     if (LU.r == 0 || B.c == 0) return;
-    d.push_back(SolveDesc<T>{LU.p, piv, B.p, LU.r, B.c, LU.ld, B.ld});
+    d.push_back(SolveDesc<T>{LU.p, piv, B.p, LU.r, B.c, LU.ld, B.ld, gate, want});
     int ch = (B.c + 63) / 64;
     if (ch > maxc) maxc = ch;
   }
@@ -438,8 +438,8 @@ struct FroBatch {
 template <class T>
 struct CvtBatch {
   std::vector<CvtDesc<T>> d;
-  void add(const double2* src, T* dst, int64_t n) {
-    if (n > 0) d.push_back(CvtDesc<T>{src, dst, n});
+  void add(const double2* src, T* dst, int64_t n, const int* gate = nullptr, int want = 0) {
+    if (n > 0) d.push_back(CvtDesc<T>{src, dst, n, gate, want});
   }
   void run(Arena& scratch, hipStream_t s) {
     if (d.empty() && !merge_group()) return;
@@ -456,8 +456,8 @@ struct CvtBatch {
 // Element-wise complex64 -> FP64 conversions.
 struct WidenBatch {
   std::vector<WidenDesc> d;
-  void add(const float2* src, double2* dst, int64_t n) {
-    if (n > 0) d.push_back(WidenDesc{src, dst, n});
+  void add(const float2* src, double2* dst, int64_t n, const int* gate = nullptr, int want = 0) {
+    if (n > 0) d.push_back(WidenDesc{src, dst, n, gate, want});
   }
   void run(Arena& scratch, hipStream_t s) {
     if (d.empty() && !merge_group()) return;
@@ -468,6 +468,53 @@ struct WidenBatch {
     if (d.empty()) return 0;
     launch_widen((int)d.size(), upload(scratch, d, s), s, max_len(d));
     return 1;
+  }
+};
+
+// Complex64 pivot-block LUs whose precision is chosen per block on the
+// device (GateDesc).
+struct GateBatch {
+  std::vector<GateDesc> d;
+  static int run_many(void* const* bs, int n, Arena& scratch, hipStream_t s) {
+    std::vector<GateDesc> d = take_all(bs, n, &GateBatch::d);
+    if (d.empty()) return 0;
+    const int lim = lu_lds_max(sizeof(float2));
+    int nmax = 0;
+    for (auto& g : d)
+      if (g.n <= lim) nmax = std::max(nmax, g.n);
+    auto* dd = upload(scratch, d, s);
+    double by = 0, fl = 0;
+    if (prof_on())
+      for (auto& g : d) by += 8.0 * 2.0 * g.n * g.n, fl += 8.0 * g.n * g.n * (double)g.n / 3;
+    ProfScope ps(s, "k_lu_gate<c64>", by, fl);
+    launch_lu_gate((int)d.size(), dd, nmax, s);
+    return 1;
+  }
+  void run(Arena& scratch, hipStream_t s) {
+    if (d.empty() && !merge_group()) return;
+    merge_or_run(this, scratch, s);
+  }
+};
+
+// Solves with gated complex64 pivot blocks (MixSolveDesc).
+struct MixSolveBatch {
+  std::vector<MixSolveDesc> d;
+  static int run_many(void* const* bs, int n, Arena& scratch, hipStream_t s) {
+    std::vector<MixSolveDesc> d = take_all(bs, n, &MixSolveBatch::d);
+    if (d.empty()) return 0;
+    int nmax = 0, cmax = 0;
+    for (auto& m : d) nmax = std::max(nmax, m.n), cmax = std::max(cmax, m.nrhs);
+    auto* dd = upload(scratch, d, s);
+    double by = 0, fl = 0;
+    if (prof_on())
+      for (auto& m : d) by += 8.0 * ((double)m.n * m.n + 2.0 * m.n * m.nrhs), fl += 8.0 * m.n * m.n * (double)m.nrhs;
+    ProfScope ps(s, "k_lu_solve_mixed", by, fl);
+    launch_lu_solve_mixed((int)d.size(), cmax, dd, nmax, s);
+    return 1;
+  }
+  void run(Arena& scratch, hipStream_t s) {
+    if (d.empty() && !merge_group()) return;
+    merge_or_run(this, scratch, s);
   }
 };
 

@@ -41,6 +41,7 @@ struct aifmm_ls_problem {
   int refine_method = 0;
   int precision = -1;
   double l2mf_eps = 1e-12, base_eps = 0;  // zero-pivot epsilons of the next factorisation
+  double gate_thr = 1e4;                  // growth gate of the next complex64 factorisation (aifmm_ls_set_growth_gate)
   Arena sc{size_t(16) << 20};
   Arena io{size_t(64) << 20};
   double2* rhs = nullptr;
@@ -250,6 +251,7 @@ int aifmm_ls_factor(aifmm_ls_problem* h, int precision) {
       std::unique_ptr<Factor<float2>> f(new Factor<float2>(*h->pb));
       f->l2mf_eps = h->l2mf_eps;
       f->base_eps = h->base_eps;
+      f->gate_thr = h->gate_thr;
       f->factor();
       h->f32 = std::move(f);
     }
@@ -326,6 +328,7 @@ int aifmm_ls_factor_batch(aifmm_ls_problem* const* hs, int n, int precision, dou
         std::unique_ptr<Factor<float2>> f(new Factor<float2>(pb));
         f->l2mf_eps = h->l2mf_eps;
         f->base_eps = h->base_eps;
+        f->gate_thr = h->gate_thr;
         f->factor();
         HIP_OK(hipStreamSynchronize(gs));
         h->f32 = std::move(f);
@@ -349,6 +352,14 @@ int aifmm_ls_factor_batch(aifmm_ls_problem* const* hs, int n, int precision, dou
           throw std::runtime_error("batch member " + std::to_string(i) + ": " + e.what());
         }
       }
+  })
+}
+
+int aifmm_ls_set_growth_gate(aifmm_ls_problem* h, double thr) {
+  GUARD({
+    if (!h) throw std::runtime_error("null handle");
+    if (std::isnan(thr)) throw std::runtime_error("growth gate threshold is NaN");
+    h->gate_thr = thr < 0 ? -1 : thr;
   })
 }
 
@@ -656,7 +667,7 @@ int aifmm_ls_device_bytes(aifmm_ls_problem* h, int64_t* bytes) {
 
 int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
   GUARD({
-    double v[19] = {0};
+    double v[21] = {0};
     Problem& P = *h->pb;
     v[0] = P.t_tables;
     v[1] = P.t_assemble;
@@ -673,6 +684,8 @@ int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
       v[16] = st->dense_n;
       v[17] = st->base_rdiag_ratio;
       v[18] = (double)(st->zero_piv.size() / 4);
+      v[19] = st->piv_c64;
+      v[20] = st->piv_fp64;
     }
     if (h->f64) v[7] = (double)h->f64->solve_bytes(1);
     if (h->f32) v[7] = (double)h->f32->solve_bytes(1);
@@ -687,7 +700,7 @@ int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
     };
     if (h->precision == 0 && h->f64) tuned(h->f64->sweep_tune());
     if (h->precision == 1 && h->f32) tuned(h->f32->sweep_tune());
-    for (int i = 0; i < n && i < 19; ++i) out[i] = v[i];
+    for (int i = 0; i < n && i < 21; ++i) out[i] = v[i];
   })
 }
 
@@ -772,6 +785,32 @@ static void selftest_t(int op, int m, int n, int k, const double* A, const doubl
     HIP_OK(hipStreamSynchronize(s));
     HIP_OK(hipMemcpy(iout, f.nz, 4, hipMemcpyDeviceToHost));
     down(dB, C);
+  } else if (op == 4) {
+    // complex64 pivot block through the growth gate at threshold thr:
+    // C[:m*n] = A^{-1} B, C[m*n].re = the block's estimate, iout[0] = 1 when
+    // the block went to FP64
+    if constexpr (std::is_same<T, float2>::value) {
+      Blk<float2> dA = up(A, m, m), dB = up(B, m, n);
+      DenseOps<float2> dn;
+      DFact<float2> f;
+      dn.factor(a, sc, f, dA, 0, thr);
+      dn.run_factor(sc, s);
+      dn.solve(sc, f, dB);
+      dn.run_solve(sc, s);
+      HIP_OK(hipStreamSynchronize(s));
+      float est = INFINITY;
+      iout[0] = 1;
+      if (f.flag) {
+        HIP_OK(hipMemcpy(iout, f.flag, 4, hipMemcpyDeviceToHost));
+        HIP_OK(hipMemcpy(&est, f.est, 4, hipMemcpyDeviceToHost));
+        iout[0] = iout[0] != 0;
+      }
+      down(dB, C);
+      C[2 * (size_t)m * n] = est;
+      C[2 * (size_t)m * n + 1] = 0;
+    } else {
+      throw std::runtime_error("self-test op 4 is complex64 only (op 14)");
+    }
   }
   HIP_OK(hipStreamSynchronize(s));
   stream_destroy(s);

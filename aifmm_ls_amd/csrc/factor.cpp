@@ -162,7 +162,7 @@ void Factor<T>::eliminate_color(int j, int c) {
       else cp.add(dst, dst, 4);
     }
     K.A = K.P2P[k];
-    dn.factor(ar, sc, K.fa, K.A);
+    dn.factor(ar, sc, K.fa, K.A, 0, gate_thr);
     fl += 16.0 * m * m * m / 3.0 + 16.0 * m * m * off;
   }
   cp.run(sc, pb.stream);
@@ -189,7 +189,7 @@ void Factor<T>::eliminate_color(int j, int c) {
     int r = K.r;
     X[t] = new_blk<T>(ar, r, ncols[t]);
     if (!r) continue;
-    dn.factor(ar, sc, K.fg, K.G.sub(0, 0, r, r), l2mf_eps);
+    dn.factor(ar, sc, K.fg, K.G.sub(0, 0, r, r), l2mf_eps, gate_thr);
     cp.add(X[t], X[t].sub(0, 0, r, r), 3, MONE);
     if (ncols[t] > r) cp.add(K.G.sub(0, r, r, ncols[t] - r), X[t].sub(0, r, r, ncols[t] - r));
     fl += 16.0 * r * r * r / 3.0 + 16.0 * r * r * ncols[t];
@@ -672,6 +672,28 @@ void Factor<T>::factor() {
       for (size_t i = 0; i < nz.size(); ++i)
         if (nz[i] < who[i][2]) st.zero_piv.insert(st.zero_piv.end(), {who[i][0], who[i][1], nz[i], who[i][2]});
     }
+  } else if (gate_thr >= 0) {
+    // which pivot blocks kept their complex64 LU (one gather, one readback)
+    std::vector<const int*> src;
+    for (int j = 2; j <= L; ++j)
+      for (auto& F : B[j]) {
+        if (F.fa.flag) src.push_back(F.fa.flag);
+        if (F.fg.flag) src.push_back(F.fg.flag);
+      }
+    st.piv_c64 = st.piv_fp64 = 0;
+    if (!src.empty()) {
+      auto* ds = upload(sc, src, pb.stream);
+      int* dn = sc.get<int>(src.size());
+      launch_gather_i32(ds, dn, (int)src.size(), pb.stream);
+      std::vector<int> fl(src.size());
+      d2h(fl.data(), dn, fl.size() * 4, pb.stream);
+      for (int v : fl) (v ? st.piv_fp64 : st.piv_c64) += 1;
+    }
+    for (int j = 2; j <= L; ++j)
+      for (auto& F : B[j]) {
+        if (F.m && !F.fa.flag) ++st.piv_fp64;
+        if (F.m && F.r && !F.fg.flag) ++st.piv_fp64;
+      }
   }
   tick(4, tt);
   sweep_operators();

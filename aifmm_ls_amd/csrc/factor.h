@@ -28,6 +28,9 @@ struct FactorStats {
   // below its size: level, box, nonzero pivots, size
   std::vector<int> zero_piv;
   double base_rdiag_ratio = 0;  // smallest over largest |R_ii| (complex128) or |U_ii| (complex64) of the base factor
+  // complex64 with a growth gate: pivot blocks that kept their complex64 LU,
+  // and those that went to FP64
+  int piv_c64 = 0, piv_fp64 = 0;
 };
 
 // Factorisation of a square block for repeated solves.  complex128: column-
@@ -41,6 +44,11 @@ struct DFact {
   QRFact<T> q;                  // complex128: q.A is the factored block
   Blk<double2> a64;             // complex64: FP64 LU factors
   int* piv = nullptr;
+  // complex64 with a growth gate (DenseOps<float2>::factor, gate_thr >= 0):
+  // *flag = 0 when the block kept its complex64 LU (in q.A, in place), 1 when
+  // it went to FP64 (a64); est = the block's growth-and-condition estimate
+  int* flag = nullptr;
+  float* est = nullptr;
 };
 
 template <class T>
@@ -50,7 +58,9 @@ template <>
 struct DenseOps<double2> {
   QRBatch<double2> qf;
   QRSolveBatch<double2> qs;
-  void factor(Arena& ar, Arena& sc, DFact<double2>& f, const Blk<double2>& A, double nz_eps = 0) {
+  void factor(Arena& ar, Arena& sc, DFact<double2>& f, const Blk<double2>& A, double nz_eps = 0,
+              double gate_thr = -1) {
+    (void)gate_thr;
     f.q = new_qr(ar, A);
     qf.add(f.q, sc, nz_eps);
   }
@@ -61,42 +71,75 @@ struct DenseOps<double2> {
   void run_solve(Arena& sc, hipStream_t s) { qs.run(sc, s); }
 };
 
+// Complex64 pivot blocks.  Without a gate (gate_thr < 0) every block is
+// widened and LU-factored in FP64 and its solves run in FP64.  With a gate
+// (gate_thr >= 0; blocks of up to lu_lds_max(8) = 128 rows) each block is
+// first LU-factored in complex64 in LDS and keeps that factor when its growth
+// factor times its condition estimate stays at or below gate_thr; the others
+// are factored in FP64, in the same launch when the FP64 factor fits the LDS
+// (up to 100 rows), else by the FP64 kernels gated on flag 2.  The choice is
+// made on the device, per block, and the solves follow it with no host round
+// trip: one mixed solve launch, plus the gated FP64 kernels for blocks above
+// 100 rows.
 template <>
 struct DenseOps<float2> {
   WidenBatch wf, ws;               // widen for factor / for solve
   CvtBatch<float2> nb;             // narrow solve results
   LUBatch<double2> lf;
   SolveBatch<double2> ls;
+  GateBatch gf;                    // gated complex64 LUs (and their FP64 fallback)
+  MixSolveBatch ms;                // solves with gated factors
   static Blk<double2> wide(Arena& a, const Blk<float2>& b) { return new_blk<double2>(a, b.r, b.c); }
   static void need_contig(const Blk<float2>& b) {
     if (b.size() && b.ld != b.r) throw std::runtime_error("FP64 pivot block path needs contiguous blocks");
   }
   // nz_eps applies to the QR path only: LU has no zero-pivot test
-  void factor(Arena& ar, Arena& sc, DFact<float2>& f, const Blk<float2>& A, double nz_eps = 0) {
+  void factor(Arena& ar, Arena& sc, DFact<float2>& f, const Blk<float2>& A, double nz_eps = 0,
+              double gate_thr = -1) {
     (void)nz_eps;
     need_contig(A);
     f.q.A = A;
     f.a64 = wide(ar, A);
     f.piv = ar.get<int>(A.r > 0 ? A.r : 1);
+    f.flag = nullptr;
+    f.est = nullptr;
     if (!A.size()) return;
+    if (gate_thr >= 0 && A.r <= lu_lds_max(sizeof(float2))) {
+      f.flag = ar.get<int>(1);
+      f.est = ar.get<float>(1);
+      gf.d.push_back(GateDesc{A.p, f.piv, f.flag, f.a64.p, f.est, (float)gate_thr, A.r, A.ld, f.a64.ld});
+      if (A.r > lu_lds_max(sizeof(double2))) {  // FP64 factor too large for the gate's LDS
+        wf.add(A.p, f.a64.p, (int64_t)A.size(), f.flag, 2);
+        lf.add(f.a64, f.piv, f.flag, 2);
+      }
+      return;
+    }
     wf.add(A.p, f.a64.p, (int64_t)A.size());
     lf.add(f.a64, f.piv);
   }
   void solve(Arena& sc, const DFact<float2>& f, const Blk<float2>& B) {
     need_contig(B);
     if (!f.a64.r || !B.c) return;
+    if (f.flag) {
+      ms.d.push_back(MixSolveDesc{f.q.A.p, f.a64.p, f.piv, f.flag, B.p, f.a64.r, B.c, f.q.A.ld, f.a64.ld, B.ld});
+      if (f.a64.r <= lu_lds_max(sizeof(double2))) return;
+    }
     Blk<double2> b64 = wide(sc, B);
-    ws.add(B.p, b64.p, (int64_t)B.size());
-    ls.add(f.a64, f.piv, b64);
-    nb.add(b64.p, B.p, (int64_t)B.size());
+    const int want = f.flag ? 2 : 0;
+    ws.add(B.p, b64.p, (int64_t)B.size(), f.flag, want);
+    ls.add(f.a64, f.piv, b64, f.flag, want);
+    nb.add(b64.p, B.p, (int64_t)B.size(), f.flag, want);
   }
-  // the FP64 array holding the factor (U on and above the diagonal)
+  // the FP64 array holding the factor's diagonal (U on and above it in FP64;
+  // a block that kept its complex64 LU has only its diagonal there)
   static Blk<double2> fp64_factor(const DFact<float2>& f) { return f.a64; }
   void run_factor(Arena& sc, hipStream_t s) {
+    gf.run(sc, s);
     wf.run(sc, s);
     lf.run(sc, s);
   }
   void run_solve(Arena& sc, hipStream_t s) {
+    ms.run(sc, s);
     ws.run(sc, s);
     ls.run(sc, s);
     nb.run(sc, s);
@@ -160,6 +203,9 @@ class Factor {
   // FP64 one; 0 = the FP64 epsilon, the reference's rule): the L2M_f pivot
   // blocks, and the level-2 base system.  Set before factor().
   double l2mf_eps = 1e-12, base_eps = 0;
+  // complex64 only: growth gate of the pivot blocks (DenseOps<float2>;
+  // negative: every pivot block in FP64).  Set before factor().
+  double gate_thr = -1;
   std::vector<std::vector<FBox>> B;
   std::vector<std::vector<std::vector<int>>> colors;
 

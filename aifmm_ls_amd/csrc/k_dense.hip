@@ -4,6 +4,7 @@
 // formation and strided block copies.  One workgroup (4 wave64) per matrix
 // or per output tile; every batch is described by a device array of
 // descriptors built on the host.
+#include <algorithm>
 #include <stdexcept>
 #include <type_traits>
 
@@ -36,6 +37,23 @@ __device__ void block_argmax_first(R v, int i, R* shv, int* shi, R& outv, int& o
   __syncthreads();
   outv = bv;
   outi = bi;
+}
+
+// a gated descriptor runs only when its gate holds the wanted value
+__device__ __forceinline__ bool gated_off(const int* gate, int want) { return gate && *gate != want; }
+
+template <class R, int NB = NT>
+__device__ R block_maxR(R v, R* sh) {
+  for (int o = 32; o > 0; o >>= 1) {
+    R u = __shfl_down(v, o);
+    v = u > v ? u : v;
+  }
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  R r = sh[0];
+  for (int k = 1; k < NB / 64; ++k) r = sh[k] > r ? sh[k] : r;
+  __syncthreads();
+  return r;
 }
 
 template <class R, int NB = NT>
@@ -202,6 +220,7 @@ __global__ __launch_bounds__(NT) void k_lu(const LUDesc<T>* __restrict__ ds) {
   const LUDesc<T> d = ds[blockIdx.x];
   __shared__ R shv[4];
   __shared__ int shi[4];
+  if (gated_off(d.gate, d.want)) return;
   const int n = d.n, ld = d.lda;
   T* A = d.A;
   for (int k = 0; k < n; ++k) {
@@ -238,21 +257,14 @@ __global__ __launch_bounds__(NT) void k_lu(const LUDesc<T>* __restrict__ ds) {
   }
 }
 
-// The same LU for a block that fits the LDS (n x n elements, n <= 64): the
-// block is loaded once, factored in LDS with the same operations in the same
-// order (bitwise the global-memory kernel's result) and written back.
+// The same LU for a block held in LDS (S: n x n, leading dimension n), with
+// the same operations in the same order (bitwise the global-memory kernel's
+// result).  Every thread of the workgroup calls it.
 template <class T>
-__global__ __launch_bounds__(NT) void k_lu_lds(const LUDesc<T>* __restrict__ ds, int nmax) {
+__device__ void lu_in_lds(T* S, int n, int* piv) {
   using R = typename CplxOf<T>::real;
-  extern __shared__ __align__(16) unsigned char lu_lds[];
-  T* S = (T*)lu_lds;
   __shared__ R shv[4];
   __shared__ int shi[4];
-  const LUDesc<T> d = ds[blockIdx.x];
-  const int n = d.n, ld = d.lda;
-  if (n > nmax) return;  // the launcher gives such blocks to k_lu
-  for (int e = threadIdx.x; e < n * n; e += NT) S[e] = d.A[e % n + (size_t)ld * (e / n)];
-  __syncthreads();
   for (int k = 0; k < n; ++k) {
     R bv = -1;
     int bi = -1;
@@ -270,7 +282,7 @@ __global__ __launch_bounds__(NT) void k_lu_lds(const LUDesc<T>* __restrict__ ds,
         S[k + n * j] = S[p + n * j];
         S[p + n * j] = t;
       }
-    if (threadIdx.x == 0) d.piv[k] = p;
+    if (threadIdx.x == 0) piv[k] = p;
     __syncthreads();
     T dk = S[k + n * k];
     bool nz = dk.x != 0 || dk.y != 0;
@@ -285,7 +297,87 @@ __global__ __launch_bounds__(NT) void k_lu_lds(const LUDesc<T>* __restrict__ ds,
       }
     __syncthreads();
   }
+}
+
+// LU of a block that fits the LDS (n <= nmax <= lu_lds_max): loaded once,
+// factored in LDS, written back.
+template <class T>
+__global__ __launch_bounds__(NT) void k_lu_lds(const LUDesc<T>* __restrict__ ds, int nmax) {
+  extern __shared__ __align__(16) unsigned char lu_lds[];
+  T* S = (T*)lu_lds;
+  const LUDesc<T> d = ds[blockIdx.x];
+  const int n = d.n, ld = d.lda;
+  if (n > nmax) return;  // the launcher gives such blocks to k_lu
+  if (gated_off(d.gate, d.want)) return;
+  for (int e = threadIdx.x; e < n * n; e += NT) S[e] = d.A[e % n + (size_t)ld * (e / n)];
+  __syncthreads();
+  lu_in_lds(S, n, d.piv);
   for (int e = threadIdx.x; e < n * n; e += NT) d.A[e % n + (size_t)ld * (e / n)] = S[e];
+}
+
+// Complex64 pivot-block LU with the precision chosen per block (GateDesc):
+// the complex64 LU in LDS, then for a block that fails the gate the FP64 LU
+// of the widened block in the same LDS (n <= n64max), or flag 2 for the
+// launcher's FP64 kernels (larger blocks).  The FP64 factor is bitwise that
+// of widening the block and running k_lu_lds<double2> on it.
+__global__ __launch_bounds__(NT) void k_lu_gate(const GateDesc* __restrict__ ds, int nmax, int n64max) {
+  extern __shared__ __align__(16) unsigned char lug_lds[];
+  float2* S = (float2*)lug_lds;
+  double2* S64 = (double2*)lug_lds;
+  __shared__ float shr[4];
+  const GateDesc d = ds[blockIdx.x];
+  const int n = d.n, ld = d.lda;
+  if (n > nmax) {  // the launcher gates only blocks that fit; FP64 elsewhere
+    if (threadIdx.x == 0) {
+      *d.flag = 2;
+      if (d.est) *d.est = INFINITY;
+    }
+    return;
+  }
+  float amax = 0;
+  for (int e = threadIdx.x; e < n * n; e += NT) {
+    const float2 a = d.A[e % n + (size_t)ld * (e / n)];
+    S[e] = a;
+    amax = fmaxf(amax, hypotf(a.x, a.y));
+  }
+  amax = block_maxR(amax, shr);  // (its barriers also order the loads)
+  lu_in_lds(S, n, d.piv);
+  float umax = 0, dmax = 0, dmin = INFINITY;
+  for (int e = threadIdx.x; e < n * n; e += NT) {
+    const int i = e % n, j = e / n;
+    if (i > j) continue;
+    const float u = hypotf(S[e].x, S[e].y);
+    umax = fmaxf(umax, u);
+    if (i == j) dmax = fmaxf(dmax, u), dmin = fminf(dmin, u);
+  }
+  umax = block_maxR(umax, shr);
+  dmax = block_maxR(dmax, shr);
+  dmin = -block_maxR(-dmin, shr);
+  // a zero or non-finite pivot gives an infinite (or NaN) estimate: FP64
+  const float est = (umax / amax) * (dmax / dmin);
+  const bool keep = amax > 0 && est <= d.thr;  // uniform over the workgroup
+  int flag = 0;
+  if (keep) {
+    for (int e = threadIdx.x; e < n * n; e += NT) d.A[e % n + (size_t)ld * (e / n)] = S[e];
+    for (int i = threadIdx.x; i < n; i += NT)
+      d.a64[i + (size_t)d.ld64 * i] = mk((double)S[i + n * i].x, (double)S[i + n * i].y);
+  } else if (n <= n64max) {
+    // the block in global memory is untouched: widen it into the LDS
+    for (int e = threadIdx.x; e < n * n; e += NT) {
+      const float2 a = d.A[e % n + (size_t)ld * (e / n)];
+      S64[e] = mk((double)a.x, (double)a.y);
+    }
+    __syncthreads();
+    lu_in_lds(S64, n, d.piv);
+    for (int e = threadIdx.x; e < n * n; e += NT) d.a64[e % n + (size_t)d.ld64 * (e / n)] = S64[e];
+    flag = 1;
+  } else {
+    flag = 2;
+  }
+  if (threadIdx.x == 0) {
+    *d.flag = flag;
+    if (d.est) *d.est = est;
+  }
 }
 
 template <class T>
@@ -295,51 +387,117 @@ __device__ __forceinline__ T shfl_c(T v, int lane) {
   return v;
 }
 
-// LU solves of a block with n <= 64 rows: the factors and the row
-// permutation in LDS, one right-hand-side column per wave at a time with row
-// i in lane i, so the substitutions need no workgroup barriers (the same
-// operations per element in the same order as k_lu_solve, bitwise).
-// grid = (matrix, group of LUS_COLS columns)
-constexpr int LUS_COLS = 16;
-template <class T>
-__global__ __launch_bounds__(NT) void k_lu_solve_wave(const SolveDesc<T>* __restrict__ ds, int nmax) {
-  extern __shared__ __align__(16) unsigned char lus_lds[];
-  T* S = (T*)lus_lds;
-  __shared__ int perm[64];
-  const SolveDesc<T> d = ds[blockIdx.x];
-  const int n = d.n, la = d.lda, lb = d.ldb;
-  const int c0 = blockIdx.y * LUS_COLS;
-  if (n > nmax || c0 >= d.nrhs) return;  // the launcher gives larger blocks to k_lu_solve
-  const int nc = d.nrhs - c0 < LUS_COLS ? d.nrhs - c0 : LUS_COLS;
-  for (int e = threadIdx.x; e < n * n; e += NT) S[e] = d.A[e % n + (size_t)la * (e / n)];
+template <class TF, class TB>
+__device__ __forceinline__ TF to_f(TB b);
+template <>
+__device__ __forceinline__ double2 to_f<double2, double2>(double2 b) { return b; }
+template <>
+__device__ __forceinline__ float2 to_f<float2, float2>(float2 b) { return b; }
+template <>
+__device__ __forceinline__ double2 to_f<double2, float2>(float2 b) { return mk((double)b.x, (double)b.y); }
+template <class TB, class TF>
+__device__ __forceinline__ TB to_b(TF v);
+template <>
+__device__ __forceinline__ double2 to_b<double2, double2>(double2 v) { return v; }
+template <>
+__device__ __forceinline__ float2 to_b<float2, float2>(float2 v) { return v; }
+template <>
+__device__ __forceinline__ float2 to_b<float2, double2>(double2 v) { return mkf((float)v.x, (float)v.y); }
+
+// The substitutions of nc right-hand-side columns (B, leading dimension lb)
+// with the factors S (n x n in LDS, n <= 128) and the row order perm (in
+// LDS): one column per wave at a time, rows i and i + 64 in lane i, so no
+// workgroup barriers; arithmetic in TF, the columns stored as TB.  The same
+// operations per element in the same order as k_lu_solve.
+template <class TF, class TB>
+__device__ void wave_solve(const TF* S, int n, const int* perm, TB* B0, int lb, int nc) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int hi = lane + 64;  // this lane's second row
+  for (int c = w; c < nc; c += NT / 64) {
+    TB* B = B0 + (size_t)lb * c;
+    TF b0 = czero<TF>(), b1 = czero<TF>();
+    if (lane < n) b0 = to_f<TF, TB>(B[perm[lane]]);
+    if (hi < n) b1 = to_f<TF, TB>(B[perm[hi]]);
+    for (int k = 0; k < n - 1; ++k) {
+      const TF bk = shfl_c(k < 64 ? b0 : b1, k & 63);
+      if (lane > k && lane < n) b0 = cfms(S[lane + n * k], bk, b0);
+      if (hi > k && hi < n) b1 = cfms(S[hi + n * k], bk, b1);
+    }
+    for (int k = n - 1; k >= 0; --k) {
+      const TF dk = S[k + n * k];
+      const bool nzp = dk.x != 0 || dk.y != 0;
+      if (k < 64) {
+        if (lane == k) b0 = nzp ? cdiv(b0, dk) : czero<TF>();
+      } else if (lane == k - 64) {
+        b1 = nzp ? cdiv(b1, dk) : czero<TF>();
+      }
+      const TF bk = shfl_c(k < 64 ? b0 : b1, k & 63);
+      if (lane < k) b0 = cfms(S[lane + n * k], bk, b0);
+      if (hi < k) b1 = cfms(S[hi + n * k], bk, b1);
+    }
+    if (lane < n) B[lane] = to_b<TB, TF>(b0);
+    if (hi < n) B[hi] = to_b<TB, TF>(b1);
+  }
+}
+
+// the row order of the interchanges k <-> piv[k], k = 0..n-1, applied in order
+__device__ __forceinline__ void perm_from_piv(const int* piv, int n, int* perm) {
   if (threadIdx.x == 0) {
-    // the row interchanges k <-> piv[k], k = 0..n-1, applied in order
     for (int i = 0; i < n; ++i) perm[i] = i;
     for (int k = 0; k < n; ++k) {
-      const int p = d.piv[k];
+      const int p = piv[k];
       const int t = perm[k];
       perm[k] = perm[p];
       perm[p] = t;
     }
   }
+}
+
+// LU solves of a block with n <= 128 rows, the factors in LDS (bitwise
+// k_lu_solve).  grid = (matrix, group of LUS_COLS columns)
+constexpr int LUS_COLS = 16;
+template <class T>
+__global__ __launch_bounds__(NT) void k_lu_solve_wave(const SolveDesc<T>* __restrict__ ds, int nmax) {
+  extern __shared__ __align__(16) unsigned char lus_lds[];
+  T* S = (T*)lus_lds;
+  __shared__ int perm[128];
+  const SolveDesc<T> d = ds[blockIdx.x];
+  const int n = d.n, la = d.lda;
+  const int c0 = blockIdx.y * LUS_COLS;
+  if (n > nmax || c0 >= d.nrhs) return;  // the launcher gives larger blocks to k_lu_solve
+  if (gated_off(d.gate, d.want)) return;
+  const int nc = d.nrhs - c0 < LUS_COLS ? d.nrhs - c0 : LUS_COLS;
+  for (int e = threadIdx.x; e < n * n; e += NT) S[e] = d.A[e % n + (size_t)la * (e / n)];
+  perm_from_piv(d.piv, n, perm);
   __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int c = w; c < nc; c += NT / 64) {
-    T* B = d.B + (size_t)lb * (c0 + c);
-    T b = czero<T>();
-    if (lane < n) b = B[perm[lane]];
-    for (int k = 0; k < n - 1; ++k) {
-      const T bk = shfl_c(b, k);
-      if (lane > k && lane < n) b = cfms(S[lane + n * k], bk, b);
-    }
-    for (int k = n - 1; k >= 0; --k) {
-      const T dk = S[k + n * k];
-      const bool nzp = dk.x != 0 || dk.y != 0;
-      if (lane == k) b = nzp ? cdiv(b, dk) : czero<T>();
-      const T bk = shfl_c(b, k);
-      if (lane < k) b = cfms(S[lane + n * k], bk, b);
-    }
-    if (lane < n) B[lane] = b;
+  wave_solve<T, T>(S, n, perm, d.B + (size_t)d.ldb * c0, d.ldb, nc);
+}
+
+// Solves with the gated complex64 pivot blocks (MixSolveDesc): flag 0, the
+// complex64 factor; flag 1, the FP64 factor with the complex64 columns
+// widened on load and rounded on store (bitwise widen + k_lu_solve_wave
+// <double2> + narrow); flag 2 blocks are left to the launcher's FP64 kernels.
+__global__ __launch_bounds__(NT) void k_lu_solve_mixed(const MixSolveDesc* __restrict__ ds, int nmax) {
+  extern __shared__ __align__(16) unsigned char lum_lds[];
+  __shared__ int perm[128];
+  const MixSolveDesc d = ds[blockIdx.x];
+  const int n = d.n;
+  const int c0 = blockIdx.y * LUS_COLS;
+  if (n > nmax || c0 >= d.nrhs) return;
+  const int f = *d.flag;
+  if (f == 2) return;
+  const int nc = d.nrhs - c0 < LUS_COLS ? d.nrhs - c0 : LUS_COLS;
+  perm_from_piv(d.piv, n, perm);
+  if (f == 0) {
+    float2* S = (float2*)lum_lds;
+    for (int e = threadIdx.x; e < n * n; e += NT) S[e] = d.A32[e % n + (size_t)d.lda32 * (e / n)];
+    __syncthreads();
+    wave_solve<float2, float2>(S, n, perm, d.B + (size_t)d.ldb * c0, d.ldb, nc);
+  } else {
+    double2* S = (double2*)lum_lds;
+    for (int e = threadIdx.x; e < n * n; e += NT) S[e] = d.A64[e % n + (size_t)d.ld64 * (e / n)];
+    __syncthreads();
+    wave_solve<double2, float2>(S, n, perm, d.B + (size_t)d.ldb * c0, d.ldb, nc);
   }
 }
 
@@ -349,6 +507,7 @@ __global__ __launch_bounds__(NT) void k_lu_solve(const SolveDesc<T>* __restrict_
   const SolveDesc<T> d = ds[blockIdx.x];
   int c0 = blockIdx.y * 64;
   if (c0 >= d.nrhs) return;
+  if (gated_off(d.gate, d.want)) return;
   int nc = d.nrhs - c0 < 64 ? d.nrhs - c0 : 64;
   const int n = d.n, la = d.lda, lb = d.ldb;
   const T* A = d.A;
@@ -672,6 +831,7 @@ __global__ __launch_bounds__(NT) void k_fro(const CopyDesc<T>* __restrict__ ds, 
 template <class T>
 __global__ __launch_bounds__(NT) void k_cvt(const CvtDesc<T>* __restrict__ ds) {
   const CvtDesc<T> d = ds[blockIdx.x];
+  if (gated_off(d.gate, d.want)) return;
   for (int64_t e = threadIdx.x + (int64_t)blockIdx.y * NT; e < d.n; e += (int64_t)NT * gridDim.y)
     d.dst[e] = from2<T>(d.src[e]);
 }
@@ -708,24 +868,66 @@ void launch_lu_solve(int nb, int max_chunks, const SolveDesc<T>* d, hipStream_t 
   hipLaunchKernelGGL(k_lu_solve<T>, dim3(nb, max_chunks), dim3(NT), 0, s, d);
   HIP_OK(hipGetLastError());
 }
+// LDS per workgroup of the current device (160 KiB on MI355X), less room for
+// the kernels' static shared arrays
+static size_t lds_budget() {
+  static const size_t v = [] {
+    int dev = 0, b = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&b, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        b <= 0)
+      b = 64 * 1024;
+    return (size_t)b - 2048;
+  }();
+  return v;
+}
 int lu_lds_max(int elem_bytes) {
-  int n = 64;
-  while ((size_t)n * n * elem_bytes > 60 * 1024) --n;
+  int n = 128;
+  while ((size_t)n * n * elem_bytes > lds_budget()) --n;
   return n;
+}
+// dynamic LDS above the default 64 KiB must be allowed per kernel
+template <class K>
+static void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 64 * 1024) HIP_OK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 template <class T>
 void launch_lu_lds(int nb, const LUDesc<T>* d, int nmax, hipStream_t s) {
   if (!nb) return;
   if (nmax > lu_lds_max(sizeof(T))) throw std::runtime_error("LU block too large for the LDS kernel");
-  hipLaunchKernelGGL(k_lu_lds<T>, dim3(nb), dim3(NT), (size_t)nmax * nmax * sizeof(T), s, d, nmax);
+  const size_t lds = (size_t)nmax * nmax * sizeof(T);
+  allow_lds(k_lu_lds<T>, lds);
+  hipLaunchKernelGGL(k_lu_lds<T>, dim3(nb), dim3(NT), lds, s, d, nmax);
   HIP_OK(hipGetLastError());
 }
 template <class T>
 void launch_lu_solve_lds(int nb, int max_cols, const SolveDesc<T>* d, int nmax, hipStream_t s) {
   if (!nb || !max_cols) return;
   if (nmax > lu_lds_max(sizeof(T))) throw std::runtime_error("LU block too large for the LDS kernel");
-  hipLaunchKernelGGL(k_lu_solve_wave<T>, dim3(nb, (max_cols + LUS_COLS - 1) / LUS_COLS), dim3(NT),
-                     (size_t)nmax * nmax * sizeof(T), s, d, nmax);
+  const size_t lds = (size_t)nmax * nmax * sizeof(T);
+  allow_lds(k_lu_solve_wave<T>, lds);
+  hipLaunchKernelGGL(k_lu_solve_wave<T>, dim3(nb, (max_cols + LUS_COLS - 1) / LUS_COLS), dim3(NT), lds, s, d, nmax);
+  HIP_OK(hipGetLastError());
+}
+// LDS of the gated kernels: the complex64 factor of the largest block, or the
+// FP64 factor of the largest block that takes the FP64 path in LDS
+static size_t gate_lds(int nmax) {
+  const int n64 = std::min(nmax, lu_lds_max(sizeof(double2)));
+  return std::max((size_t)nmax * nmax * sizeof(float2), (size_t)n64 * n64 * sizeof(double2)) + 16;
+}
+void launch_lu_gate(int nb, const GateDesc* d, int nmax, hipStream_t s) {
+  if (!nb) return;
+  if (nmax > lu_lds_max(sizeof(float2))) throw std::runtime_error("gated LU: block too large for the LDS kernel");
+  const size_t lds = gate_lds(nmax);
+  allow_lds(k_lu_gate, lds);
+  hipLaunchKernelGGL(k_lu_gate, dim3(nb), dim3(NT), lds, s, d, nmax, lu_lds_max(sizeof(double2)));
+  HIP_OK(hipGetLastError());
+}
+void launch_lu_solve_mixed(int nb, int max_cols, const MixSolveDesc* d, int nmax, hipStream_t s) {
+  if (!nb || !max_cols) return;
+  if (nmax > lu_lds_max(sizeof(float2))) throw std::runtime_error("gated LU solve: block too large for the LDS kernel");
+  const size_t lds = gate_lds(nmax);
+  allow_lds(k_lu_solve_mixed, lds);
+  hipLaunchKernelGGL(k_lu_solve_mixed, dim3(nb, (max_cols + LUS_COLS - 1) / LUS_COLS), dim3(NT), lds, s, d, nmax);
   HIP_OK(hipGetLastError());
 }
 template <class T>
@@ -773,6 +975,7 @@ void launch_cvt(int nb, const CvtDesc<T>* d, hipStream_t s, int64_t max_n) {
 }
 __global__ __launch_bounds__(NT) void k_widen(const WidenDesc* __restrict__ ds) {
   const WidenDesc d = ds[blockIdx.x];
+  if (gated_off(d.gate, d.want)) return;
   for (int64_t i = threadIdx.x + (int64_t)blockIdx.y * NT; i < d.n; i += (int64_t)NT * gridDim.y)
     d.dst[i] = mk((double)d.src[i].x, (double)d.src[i].y);
 }

@@ -96,11 +96,16 @@ struct TileTask {
   int g, tm, tn;
 };
 
+// Gates: a descriptor with gate set runs only when *gate == want (the
+// per-block precision choice of the complex64 pivot blocks, made on the
+// device; null runs always).
 template <class T>
 struct LUDesc {
   T* A;
   int* piv;
   int n, lda;
+  const int* gate;
+  int want;
 };
 
 template <class T>
@@ -109,7 +114,42 @@ struct SolveDesc {
   const int* piv;
   T* B;
   int n, nrhs, lda, ldb;
+  const int* gate;
+  int want;
 };
+
+// Complex64 LU of a pivot block in LDS with partial pivoting, kept in place
+// when its error estimate  rho * cond  stays at or below thr (per block),
+// where rho = max|U| / max|A| is the growth factor and cond = max|u_ii| /
+// min|u_ii| the condition estimate from the diagonal of U.  Otherwise the
+// block (left as it was) is widened and LU-factored in FP64 into a64, in the
+// same launch when its FP64 factor fits the LDS.  *flag: 0 kept in complex64,
+// 1 FP64 factor in a64, 2 the FP64 factor is still to be formed (by the
+// launcher's widen + LU kernels gated on flag 2).  A kept factor also writes
+// its diagonal, widened, into the diagonal of a64 (where the callers read
+// condition estimates); *est receives the estimate when est is set.
+struct GateDesc {
+  float2* A;
+  int* piv;
+  int* flag;
+  double2* a64;
+  float* est;
+  float thr;
+  int n, lda, ld64;
+};
+// Solves with such factors: B (complex64) <- A^{-1} B with the complex64 factor
+// (flag 0) or the FP64 one (flag 1, the columns widened and rounded back);
+// flag 2 blocks are skipped (solved by the FP64 kernels gated on flag 2).
+struct MixSolveDesc {
+  const float2* A32;
+  const double2* A64;
+  const int* piv;
+  const int* flag;
+  float2* B;
+  int n, nrhs, lda32, ld64, ldb;
+};
+void launch_lu_solve_mixed(int nb, int max_cols, const MixSolveDesc* d, int nmax, hipStream_t s);
+void launch_lu_gate(int nb, const GateDesc* d, int nmax, hipStream_t s);
 
 template <class T>
 struct QRDesc {
@@ -170,6 +210,8 @@ struct CvtDesc {
   const double2* src;
   T* dst;
   int64_t n;
+  const int* gate;
+  int want;
 };
 // max_n: the longest descriptor's n (sets the chunks per descriptor)
 template <class T>
@@ -179,6 +221,8 @@ struct WidenDesc {
   const float2* src;
   double2* dst;
   int64_t n;
+  const int* gate;
+  int want;
 };
 void launch_widen(int nb, const WidenDesc* d, hipStream_t s, int64_t max_n);
 // the longest n of a batch of conversion descriptors
@@ -197,7 +241,9 @@ template <class T>
 void launch_lu_solve(int nb, int max_chunks, const SolveDesc<T>* d, hipStream_t s);
 // the same for blocks of at most nmax <= lu_lds_max(sizeof(T)) rows (larger
 // descriptors are skipped): factors in LDS; the solve keeps one column per
-// wave in registers (bitwise the results of the two kernels above)
+// wave in registers, rows i and i + 64 in lane i (bitwise the results of the
+// two kernels above).  lu_lds_max: the largest n whose n x n block fits the
+// device's LDS per workgroup, at most 128.
 int lu_lds_max(int elem_bytes);
 template <class T>
 void launch_lu_lds(int nb, const LUDesc<T>* d, int nmax, hipStream_t s);

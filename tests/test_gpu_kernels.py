@@ -22,7 +22,11 @@ def test_gemm(m, n, k):
     assert np.abs(c - ref).max() < 1e-5 * np.abs(ref).max() * np.sqrt(k)
 
 
-@pytest.mark.parametrize("m,n", [(1, 1), (36, 1), (36, 290), (77, 65), (200, 3)])
+# 64/65, 100/101 and 128/129 rows: the LDS kernels' one- and two-row-per-lane
+# solves and their limits (complex128 up to 100 rows, complex64 up to 128 in
+# 160 KiB of LDS), then the global-memory kernels
+@pytest.mark.parametrize("m,n", [(1, 1), (36, 1), (36, 290), (64, 5), (65, 5), (77, 65), (100, 20), (101, 3),
+                                 (128, 17), (129, 2), (200, 3)])
 def test_lu_solve(m, n):
     rng = np.random.default_rng(m + 7 * n)
     a, b = rnd(rng, m, m), rnd(rng, m, n)
@@ -31,6 +35,60 @@ def test_lu_solve(m, n):
     assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-11
     x = A.selftest_lu_solve(a, b, c64=True)
     assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-3
+
+
+def lu_estimate(a):
+    """Growth factor max|U|/max|A| times max|u_ii|/min|u_ii| of the LU with
+    partial pivoting by modulus (the pivot rule of the kernels), in FP64."""
+    u = np.array(a, np.complex128)
+    n = u.shape[0]
+    for k in range(n):
+        p = k + int(np.argmax(np.abs(u[k:, k])))
+        u[[k, p]] = u[[p, k]]
+        if u[k, k] != 0:
+            u[k + 1:, k] /= u[k, k]
+            u[k + 1:, k + 1:] -= np.outer(u[k + 1:, k], u[k, k + 1:])
+    U = np.triu(u)
+    d = np.abs(np.diag(U))
+    return np.abs(U).max() / np.abs(a).max() * d.max() / d.min() if d.min() > 0 else np.inf
+
+
+def test_growth_gate():
+    """The complex64 pivot-block gate: a block keeps its complex64 LU when its
+    growth factor times condition estimate is at most the threshold, else it
+    is factored and solved in FP64; blocks above 128 rows and singular ones
+    always go to FP64."""
+    rng = np.random.default_rng(5)
+    a, b = rnd(rng, 40, 40), rnd(rng, 40, 3)
+    a32 = a.astype(np.complex64).astype(np.complex128)  # what the kernels see
+    ref = np.linalg.solve(a32, b.astype(np.complex64).astype(np.complex128))
+    est_ref = lu_estimate(a32)
+    x, est, fp64 = A.selftest_gated_lu_solve(a, b, 1e8)
+    assert not fp64
+    assert abs(est - est_ref) < 1e-3 * est_ref
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-3
+    x, est, fp64 = A.selftest_gated_lu_solve(a, b, 0.5 * est_ref)
+    assert fp64
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-6  # FP64 solve, result rounded to complex64
+    # badly conditioned: diagonal scaling over 1e6
+    s = np.logspace(0, 6, 40)
+    x, est, fp64 = A.selftest_gated_lu_solve(a * s[None, :], b, 1e4)
+    assert fp64 and est > 1e4
+    # 101-128 rows: the complex64 factor fits the LDS, the FP64 one does not
+    # (formed by the FP64 kernels gated on the block's flag)
+    a2, b2 = rnd(rng, 110, 110), rnd(rng, 110, 4)
+    ref2 = np.linalg.solve(a2.astype(np.complex64).astype(np.complex128), b2.astype(np.complex64).astype(np.complex128))
+    x, est, fp64 = A.selftest_gated_lu_solve(a2, b2, 1e30)
+    assert not fp64 and np.linalg.norm(x - ref2) / np.linalg.norm(ref2) < 1e-3
+    x, est, fp64 = A.selftest_gated_lu_solve(a2, b2, 0.0)
+    assert fp64 and np.linalg.norm(x - ref2) / np.linalg.norm(ref2) < 1e-6
+    # above 128 rows and exactly singular blocks: FP64
+    x, est, fp64 = A.selftest_gated_lu_solve(rnd(rng, 129, 129), rnd(rng, 129, 1), 1e30)
+    assert fp64 and np.isinf(est)
+    z = a.copy()
+    z[:, 7] = 0
+    x, est, fp64 = A.selftest_gated_lu_solve(z, b, 1e30)
+    assert fp64
 
 
 @pytest.mark.parametrize("m,n", [(1, 1), (36, 1), (36, 290), (77, 65), (200, 3), (130, 17)])

@@ -40,6 +40,9 @@ def lib():
         L.aifmm_ls_create.argtypes = [ctypes.POINTER(_Params), ctypes.POINTER(ctypes.c_void_p)]
         L.aifmm_ls_create_cached.argtypes = [ctypes.POINTER(_Params), ctypes.c_char_p, ctypes.POINTER(ctypes.c_void_p)]
         L.aifmm_ls_write_precomputations.argtypes = [P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)]
+        L.aifmm_ls_create_adaptive.argtypes = [ctypes.POINTER(_Params), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.aifmm_ls_leaves.argtypes = [P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int64), dp,
+                                      ctypes.POINTER(ctypes.c_int)]
         L.aifmm_ls_table.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int, dp]
         L.aifmm_ls_destroy.argtypes = [P]
         L.aifmm_ls_last_error.restype = ctypes.c_char_p
@@ -162,10 +165,14 @@ class LSProblem:
     holding the reference's precomputations/ text cache: the tables are read
     from it instead of computed (LS_Assembly.hpp:1217-1228, 1422-1459; the
     reference with its three precomputation lines commented out, LS/Readme.md).
+    ``tree_adaptivity`` = t: the adaptive quadtree with 2:1 level restriction
+    instead of the uniform one, refined until the contrast interpolates to
+    10^-t (aifmm_ls_create_adaptive; n_levels unused; entries, the dense
+    product and the fields only).
     """
 
     def __init__(self, n_levels, kappa, tol_pow=8, qchoice=0, n_cheb=6, degree=6, L=1.0, bumps=None,
-                 quantize=True, precomputations=None):
+                 quantize=True, precomputations=None, tree_adaptivity=None):
         p = _Params()
         p.n_cheb, p.degree, p.n_levels, p.tol_pow, p.qchoice = n_cheb, degree, n_levels, tol_pow, qchoice
         p.quantize_tables = int(quantize)
@@ -176,7 +183,10 @@ class LSProblem:
             p.bumps = _dp(self._bumps)
             p.n_bumps = self._bumps.shape[0]
         h = ctypes.c_void_p()
-        if precomputations is None:
+        if tree_adaptivity is not None:
+            # the adaptive quadtree (n_levels unused): aifmm_ls_create_adaptive
+            _chk(lib().aifmm_ls_create_adaptive(ctypes.byref(p), int(tree_adaptivity), ctypes.byref(h)))
+        elif precomputations is None:
             _chk(lib().aifmm_ls_create(ctypes.byref(p), ctypes.byref(h)))
         else:
             _chk(lib().aifmm_ls_create_cached(ctypes.byref(p), os.fsencode(precomputations), ctypes.byref(h)))
@@ -192,6 +202,17 @@ class LSProblem:
 
     def __del__(self):
         self.close()
+
+    def leaves(self):
+        """The adaptive tree's leaves in order: (levels, box numbers, centres)."""
+        n = ctypes.c_int(0)
+        _chk(lib().aifmm_ls_leaves(self.h, None, None, None, ctypes.byref(n)))
+        lv = np.zeros(n.value, np.int32)
+        bx = np.zeros(n.value, np.int64)
+        c = np.zeros(2 * n.value)
+        _chk(lib().aifmm_ls_leaves(self.h, lv.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                   bx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), _dp(c), ctypes.byref(n)))
+        return lv, bx, c.reshape(-1, 2)
 
     def points(self):
         xy = np.zeros(2 * self.N)

@@ -17,7 +17,7 @@ ARCH = os.environ.get("AIFMM_ARCH", "gfx950")
 FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
          "-Wno-unused-variable", "-Wno-unused-but-set-variable"]
-SOURCES = ["k_entry.hip", "k_dense.hip", "k_tables.hip", "problem.cpp", "factor.cpp", "h2.cpp", "prof.cpp", "prog.cpp", "k_prog.hip", "k_vec.hip", "krylov.cpp", "capi.cpp", "serve.cpp", "noalloc.hip", "precomp.cpp", "merge.cpp"]
+SOURCES = ["k_entry.hip", "k_dense.hip", "k_tables.hip", "problem.cpp", "factor.cpp", "h2.cpp", "prof.cpp", "prog.cpp", "k_prog.hip", "k_vec.hip", "krylov.cpp", "capi.cpp", "serve.cpp", "noalloc.hip", "precomp.cpp", "merge.cpp", "adaptive.cpp"]
 
 
 def _compile(src):

@@ -94,7 +94,8 @@ extern "C" {
 
 const char* aifmm_ls_last_error(void) { return g_err.c_str(); }
 
-static aifmm_ls_problem* create_problem(const aifmm_ls_params* p, const std::string& precomp_dir) {
+static aifmm_ls_problem* create_problem(const aifmm_ls_params* p, const std::string& precomp_dir,
+                                        int adaptive = 0, int tree_adaptivity = 1) {
   if (!p) throw std::runtime_error("null argument");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) throw std::runtime_error("no HIP device available");
@@ -108,6 +109,8 @@ static aifmm_ls_problem* create_problem(const aifmm_ls_params* p, const std::str
   P.kappa = p->kappa;
   P.L = p->L;
   if (p->bumps && p->n_bumps > 0) P.bumps.assign(p->bumps, p->bumps + 4 * p->n_bumps);
+  P.adaptive = adaptive;
+  P.tree_adaptivity = tree_adaptivity;
   if (P.n_cheb < 1 || P.degree < 1 || P.degree * (P.degree + 1) / 2 > P.n_cheb * P.n_cheb)
     throw std::runtime_error("invalid n_cheb/degree");
   auto* h = new aifmm_ls_problem;
@@ -133,6 +136,30 @@ int aifmm_ls_create_cached(const aifmm_ls_params* p, const char* dir, aifmm_ls_p
   GUARD({
     if (!out || !dir || !*dir) throw std::runtime_error("null argument");
     *out = create_problem(p, dir);
+  })
+}
+
+int aifmm_ls_create_adaptive(const aifmm_ls_params* p, int tree_adaptivity, aifmm_ls_problem** out) {
+  GUARD({
+    if (!out) throw std::runtime_error("null argument");
+    *out = create_problem(p, std::string(), 1, tree_adaptivity);
+  })
+}
+
+int aifmm_ls_leaves(aifmm_ls_problem* h, int* levels, int64_t* boxes, double* centres, int* n) {
+  GUARD({
+    if (!h || !n) throw std::runtime_error("null argument");
+    const Problem& P = *h->pb;
+    if (!P.P.adaptive) throw std::runtime_error("leaves: the problem has the uniform tree");
+    if (levels || boxes || centres) {
+      if (*n < P.nleaf) throw std::runtime_error("leaf buffers too small");
+      for (int k = 0; k < P.nleaf; ++k) {
+        if (levels) levels[k] = P.leaves[k].level;
+        if (boxes) boxes[k] = P.leaves[k].num;
+        if (centres) centres[2 * k] = P.leaves[k].cx, centres[2 * k + 1] = P.leaves[k].cy;
+      }
+    }
+    *n = P.nleaf;
   })
 }
 
@@ -172,6 +199,7 @@ int aifmm_ls_points(aifmm_ls_problem* h, double* xy, double* q) {
 
 int aifmm_ls_tables(aifmm_ls_problem* h, double* M, double* T, int* capped) {
   GUARD({
+    h->pb->need_uniform("tables of the leaf level");
     if (M) std::memcpy(M, h->pb->M_h.data(), h->pb->M_h.size() * 16);
     if (T) std::memcpy(T, h->pb->T_h.data(), h->pb->T_h.size() * 16);
     if (capped) *capped = h->pb->quad_capped;

@@ -84,6 +84,21 @@ struct EntryCtx {
   // LS2DTree::findPhi (LS_Assembly.hpp:507-512, 530-535): 1, entries of the LS
   // operator I + kappa^2 q G; 0, of the volume potential G alone
   int find_phi = 1;
+  // Adaptive discretisation (adaptive = 1; the fields above then leave M, T,
+  // L and rleaf unused): leaf k at level leaf_lv[k], centre (lcx, lcy)[k];
+  // rad[j] the box radius at level j; Mlev the far table of level j at
+  // j * NTERM * R; the near leaves of leaf k are near_leaf[near_off[k] ..
+  // near_off[k + 1]) with their tables Ttab + near_tab[..] * R * R.
+  int adaptive = 0;
+  const int* leaf_lv = nullptr;
+  const double* lcx = nullptr;
+  const double* lcy = nullptr;
+  const double* rad = nullptr;
+  const double2* Mlev = nullptr;
+  const int* near_off = nullptr;
+  const int* near_leaf = nullptr;
+  const int* near_tab = nullptr;
+  const double2* Ttab = nullptr;
 };
 
 }  // namespace aifmm

@@ -94,8 +94,52 @@ __device__ __forceinline__ double2 graf_sum(const double2* __restrict__ Mcol, do
   return acc;
 }
 
+// The near table of target leaf li for source leaf lj (adaptive), or null
+// when the pair is far.
+__device__ __forceinline__ const double2* adaptive_near_table(const EntryCtx& C, int li, int lj) {
+  for (int e = C.near_off[li]; e < C.near_off[li + 1]; ++e)
+    if (C.near_leaf[e] == lj) return C.Ttab + (size_t)C.near_tab[e] * C.R * C.R;
+  return nullptr;
+}
+
+// getMatrixEntry between leaves of an adaptive tree (LS_Assembly.hpp:474-541):
+// the near table of the pair's neighbour type at the target's level, or the
+// Graf series with the far table of the source leaf's level about its centre.
+__device__ double2 ls_entry_adaptive(const EntryCtx& C, int64_t i, int64_t j) {
+#pragma clang fp contract(off)
+  const int R = C.R;
+  const int li = (int)(i / R), ii = (int)(i - (int64_t)li * R);
+  const int lj = (int)(j / R), jj = (int)(j - (int64_t)lj * R);
+  const double qi = C.q[i];
+  const double2* T = adaptive_near_table(C, li, lj);
+  if (T) {
+    double2 e = T[ii + (size_t)R * jj];
+    if (C.find_phi) {
+      e = cscale(e, C.kappa);
+      e = cscale(e, C.kappa);
+      e = cscale(e, qi);
+      if (i == j) e.x = 1.0 + e.x;
+    }
+    if (hypot(e.x, e.y) < 1e-16) e = mk(0.0, 0.0);
+    return e;
+  }
+  const double ddy = C.py[i] - C.lcy[lj], ddx = C.px[i] - C.lcx[lj];
+  const double phi = atan2(ddy, ddx);
+  const double R2 = ddy * ddy + ddx * ddx;
+  const double arg = C.kappa * sqrt(R2);
+  double2 s = graf_sum(C.Mlev + (size_t)C.leaf_lv[lj] * NTERM * R + (size_t)NTERM * jj, arg, phi);
+  s = cmul(s, mk(0.0, 0.25));
+  if (C.find_phi) {
+    s = cscale(s, C.kappa * C.kappa * qi);
+    if (i == j) s.x += 1.0;
+  }
+  if (hypot(s.x, s.y) < 1e-16) s = mk(0.0, 0.0);
+  return s;
+}
+
 __device__ double2 ls_entry(const EntryCtx& C, int64_t i, int64_t j) {
 #pragma clang fp contract(off)
+  if (C.adaptive) return ls_entry_adaptive(C, i, j);
   int R = C.R;
   int li = (int)(i / R), ii = (int)(i - (int64_t)li * R);
   int lj = (int)(j / R), jj = (int)(j - (int64_t)lj * R);
@@ -489,8 +533,10 @@ __global__ __launch_bounds__(128) void k_dmv_moments(EntryCtx C, const double2* 
   const int l = blockIdx.x, n = threadIdx.x;
   if (n >= NTERM) return;
   const double2* xl = x + (size_t)l * C.R;
+  // the far table of the leaf's level (adaptive) or of the leaf level
+  const double2* M = C.adaptive ? C.Mlev + (size_t)C.leaf_lv[l] * NTERM * C.R : C.M;
   double2 acc = mk(0.0, 0.0);
-  for (int jj = 0; jj < C.R; ++jj) acc = cadd(acc, cmul(C.M[n + (size_t)NTERM * jj], xl[jj]));
+  for (int jj = 0; jj < C.R; ++jj) acc = cadd(acc, cmul(M[n + (size_t)NTERM * jj], xl[jj]));
   m[(size_t)l * NTERM + n] = acc;
 }
 
@@ -638,15 +684,28 @@ __global__ __launch_bounds__(DMV_T) void k_dmv_far(EntryCtx C, const double2* __
   const int c = blockIdx.y;
   if (i >= N) return;
   const int l0 = c * lpc, l1 = min(nleaf, l0 + lpc);
+  const int li = (int)(i / C.R);
   int xi, yi;
-  box_xy(C.L, (int)(i / C.R), xi, yi);
+  box_xy(C.L, li, xi, yi);
   const double px = C.px[i], py = C.py[i];
+  // adaptive: getIfNeighbor's near test between the two leaves
+  const double ai = C.adaptive ? 3.0 * C.rad[C.leaf_lv[li]] - 1e-16 : 0.0;
+  const double cxi = C.adaptive ? C.lcx[li] : 0.0, cyi = C.adaptive ? C.lcy[li] : 0.0;
   double2 acc = mk(0.0, 0.0);
   for (int l = l0; l < l1; ++l) {
-    int xj, yj;
-    box_xy(C.L, l, xj, yj);
-    if (xj - xi >= -1 && xj - xi <= 1 && yj - yi >= -1 && yj - yi <= 1) continue;
-    const double cx = -C.Ldom + (2 * xj + 1) * C.rleaf, cy = -C.Ldom + (2 * yj + 1) * C.rleaf;
+    double cx, cy;
+    if (C.adaptive) {
+      cx = C.lcx[l];
+      cy = C.lcy[l];
+      const double lim = ai + C.rad[C.leaf_lv[l]];
+      if (!(fabs(cxi - cx) >= lim || fabs(cyi - cy) >= lim)) continue;
+    } else {
+      int xj, yj;
+      box_xy(C.L, l, xj, yj);
+      if (xj - xi >= -1 && xj - xi <= 1 && yj - yi >= -1 && yj - yi <= 1) continue;
+      cx = -C.Ldom + (2 * xj + 1) * C.rleaf;
+      cy = -C.Ldom + (2 * yj + 1) * C.rleaf;
+    }
     const double ddy = py - cy, ddx = px - cx;
     const double rho = sqrt(ddy * ddy + ddx * ddx);
     const double ir = 1.0 / rho;
@@ -668,6 +727,26 @@ __global__ __launch_bounds__(DMV_T) void k_dmv_sum(EntryCtx C, const double2* __
   const double qi = C.q[i];
   double2 acc = cmul(far, mk(0.0, 0.25));
   if (C.find_phi) acc = cscale(acc, C.kappa * C.kappa * qi);
+  if (C.adaptive) {
+    for (int e = C.near_off[li]; e < C.near_off[li + 1]; ++e) {
+      const int lj = C.near_leaf[e];
+      const double2* Tt = C.Ttab + (size_t)C.near_tab[e] * R * R + ii;
+      const double2* xl = x + (size_t)lj * R;
+      for (int jj = 0; jj < R; ++jj) {
+        double2 v = Tt[(size_t)R * jj];
+        if (C.find_phi) {
+          v = cscale(v, C.kappa);
+          v = cscale(v, C.kappa);
+          v = cscale(v, qi);
+          if (lj == li && jj == ii) v.x = 1.0 + v.x;
+        }
+        if (hypot(v.x, v.y) < 1e-16) v = mk(0.0, 0.0);
+        acc = cadd(acc, cmul(v, xl[jj]));
+      }
+    }
+    y[i] = acc;
+    return;
+  }
   int xi, yi;
   box_xy(C.L, li, xi, yi);
   const int side = 1 << C.L;

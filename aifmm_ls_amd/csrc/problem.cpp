@@ -3,6 +3,9 @@
 // L2P/P2M transfer operators, M2L and leaf P2P blocks.
 #include "problem.h"
 
+#include <algorithm>
+#include <array>
+
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -149,6 +152,11 @@ Problem::Problem(const Params& Pin, const std::string& precomp_dir) : P(Pin) {
 void Problem::init(const std::string& precomp_dir) {
   R = P.n_cheb * P.n_cheb;
   nbasis = P.degree * (P.degree + 1) / 2;
+  if (P.adaptive) {
+    if (!precomp_dir.empty()) throw std::runtime_error("adaptive discretisation: tables are computed, not read");
+    init_adaptive();
+    return;
+  }
   L = P.n_levels;
   if (L < 2 || L > 9) throw std::runtime_error("n_levels must be in [2, 9]");
   nleaf = 1 << (2 * L);
@@ -431,6 +439,124 @@ void Problem::install_tables() {
   ctx.Ldom = P.L;
 }
 
+void Problem::need_uniform(const char* what) const {
+  if (P.adaptive)
+    throw std::runtime_error(std::string(what) +
+                             ": needs the uniform tree (the reference's FMM2DTree solver works on it); this problem "
+                             "has the adaptive discretisation");
+}
+
+// The adaptive discretisation (Params::adaptive): the leaves of the adaptive
+// quadtree with 2:1 level restriction in the reference's leafNodes order,
+// points per leaf as assignLeafChargeLocations places them
+// (LS_Assembly.hpp:1074-1087), each leaf's near leaves with the neighbour
+// table getIfNeighbor picks, and the tables: the far M at every leaf level
+// and every near table a pair uses, by device quadrature, rounded to 6
+// digits with quantize as the text cache would.
+void Problem::init_adaptive() {
+  double t0 = now_s();
+  leaves = adaptive_leaves(P, P.tree_adaptivity, &L);
+  nleaf = (int)leaves.size();
+  N = (int64_t)nleaf * R;
+  persist.set_chunk(Arena::chunk_for(N, 64, 4 << 20, 64 << 20));
+  scratch.set_chunk(Arena::chunk_for(N, 2 << 10, 16 << 20, 256 << 20));
+  std::vector<double> cheb(P.n_cheb);
+  for (int k = 0; k < P.n_cheb; ++k) cheb[k] = -cos((k + 0.5) / P.n_cheb * PI);
+  px.resize(N);
+  py.resize(N);
+  q.resize(N);
+  std::vector<double> rad(L + 1);
+  for (int j = 0; j <= L; ++j) rad[j] = box_radius(j);
+  rleaf = rad[L];
+  for (int k = 0; k < nleaf; ++k) {
+    const AdaptiveLeaf& f = leaves[k];
+    const double r = rad[f.level];
+    for (int jy = 0; jy < P.n_cheb; ++jy)
+      for (int kx = 0; kx < P.n_cheb; ++kx) {
+        int64_t i = (int64_t)k * R + jy * P.n_cheb + kx;
+        px[i] = r * cheb[kx] + f.cx;
+        py[i] = r * cheb[jy] + f.cy;
+      }
+  }
+  for (int64_t i = 0; i < N; ++i) q[i] = contrast(P, px[i], py[i]);
+  // near leaves: candidates from a grid of cells of the coarsest leaf size
+  int lmin = L;
+  for (auto& f : leaves) lmin = std::min(lmin, f.level);
+  const int side = 1 << lmin;
+  const double cell = 2.0 * rad[lmin];
+  auto cell_of = [&](double v) { return std::min(side - 1, std::max(0, (int)floor((v + P.L) / cell))); };
+  std::vector<std::vector<int>> grid((size_t)side * side);
+  for (int k = 0; k < nleaf; ++k) grid[(size_t)cell_of(leaves[k].cy) * side + cell_of(leaves[k].cx)].push_back(k);
+  std::map<std::array<int, 3>, int> tab_of;
+  near_off.assign(nleaf + 1, 0);
+  near_leaf.clear();
+  near_tab.clear();
+  near_specs.clear();
+  for (int a = 0; a < nleaf; ++a) {
+    const AdaptiveLeaf& fa = leaves[a];
+    const double w = 3.0 * rad[fa.level] + rad[lmin];
+    const int x0 = cell_of(fa.cx - w), x1 = cell_of(fa.cx + w), y0 = cell_of(fa.cy - w), y1 = cell_of(fa.cy + w);
+    std::vector<int> cand;
+    for (int cy = y0; cy <= y1; ++cy)
+      for (int cx = x0; cx <= x1; ++cx)
+        for (int b : grid[(size_t)cy * side + cx]) cand.push_back(b);
+    std::sort(cand.begin(), cand.end());
+    for (int b : cand) {
+      int slot = -1;
+      const int fam = classify_near(fa, leaves[b], rad[fa.level], rad[leaves[b].level], slot);
+      if (fam < 0) continue;
+      const std::array<int, 3> key{fam, fa.level, slot};
+      auto it = tab_of.find(key);
+      if (it == tab_of.end()) {
+        it = tab_of.emplace(key, (int)near_specs.size()).first;
+        near_specs.push_back(TableSpec{fam, fa.level, slot});
+      }
+      near_leaf.push_back(b);
+      near_tab.push_back(it->second);
+    }
+    near_off[a + 1] = (int)near_leaf.size();
+  }
+  // tables: the far M of every level 0..L (levels without leaves are never read)
+  std::vector<TableSpec> specs;
+  for (int j = 0; j <= L; ++j) specs.push_back({-1, j, 0});
+  specs.insert(specs.end(), near_specs.begin(), near_specs.end());
+  auto tabs = quad_tables(specs, &quad_capped);
+  if (P.quantize)
+    for (auto& t : tabs)
+      for (auto& v : t) v = make_double2(round6(v.x), round6(v.y));
+  std::vector<double2> Mlev, Ttab;
+  for (int j = 0; j <= L; ++j) Mlev.insert(Mlev.end(), tabs[j].begin(), tabs[j].end());
+  for (size_t t = 0; t < near_specs.size(); ++t) Ttab.insert(Ttab.end(), tabs[L + 1 + t].begin(), tabs[L + 1 + t].end());
+  M_h = tabs[L];
+  std::vector<int> lv(nleaf);
+  std::vector<double> lcx(nleaf), lcy(nleaf);
+  for (int k = 0; k < nleaf; ++k) lv[k] = leaves[k].level, lcx[k] = leaves[k].cx, lcy[k] = leaves[k].cy;
+  dpx = upload(persist, px, stream);
+  dpy = upload(persist, py, stream);
+  dq = upload(persist, q, stream);
+  ctx = EntryCtx{};
+  ctx.px = dpx;
+  ctx.py = dpy;
+  ctx.q = dq;
+  ctx.R = R;
+  ctx.L = L;
+  ctx.kappa = P.kappa;
+  ctx.rleaf = rleaf;
+  ctx.Ldom = P.L;
+  ctx.adaptive = 1;
+  ctx.leaf_lv = upload(persist, lv, stream);
+  ctx.lcx = upload(persist, lcx, stream);
+  ctx.lcy = upload(persist, lcy, stream);
+  ctx.rad = upload(persist, rad, stream);
+  ctx.Mlev = upload(persist, Mlev, stream);
+  ctx.near_off = upload(persist, near_off, stream);
+  ctx.near_leaf = near_leaf.empty() ? nullptr : upload(persist, near_leaf, stream);
+  ctx.near_tab = near_tab.empty() ? nullptr : upload(persist, near_tab, stream);
+  ctx.Ttab = Ttab.empty() ? nullptr : upload(persist, Ttab, stream);
+  HIP_OK(hipStreamSynchronize(stream));
+  t_tables = now_s() - t0;
+}
+
 void Problem::entries(int64_t n, const int64_t* I, const int64_t* J, double2* out) {
   std::vector<int64_t> Iv(I, I + n), Jv(J, J + n);
   int64_t* dI = upload(scratch, Iv, stream);
@@ -680,6 +806,7 @@ void Problem::nca_level(int j) {
 
 void Problem::assemble() {
   if (assembled) return;
+  need_uniform("assembly");
   double t0 = now_s();
   sk.assign(L + 1, {});
   for (int j = 2; j <= L; ++j) sk[j].resize(1 << (2 * j));

@@ -14,7 +14,25 @@ struct Params {
   int n_cheb = 6, degree = 6, n_levels = 5, tol_pow = 8, qchoice = 0, quantize = 1;
   double kappa = 10.0, L = 1.0;
   std::vector<double> bumps;  // a, w, cx, cy per bump (qchoice 6)
+  // adaptive = 1: the adaptive quadtree with 2:1 level restriction in place of
+  // the uniform one (n_levels unused), refined until the contrast is
+  // interpolated to 10^-tree_adaptivity (LS_Assembly.hpp:201, 628-742)
+  int adaptive = 0, tree_adaptivity = 1;
 };
+
+// A leaf of the adaptive quadtree: level, box number (children 4b + c, as
+// the uniform tree numbers them) and centre.
+struct AdaptiveLeaf {
+  int level;
+  int64_t num;
+  double cx, cy;
+};
+// The leaves in the reference's leafNodes order; *n_levels = the tree depth.
+std::vector<AdaptiveLeaf> adaptive_leaves(const Params& P, int tree_adaptivity, int* n_levels);
+// getIfNeighbor between target leaf a (radius ra) and source leaf b (rb):
+// -1 far, else the family (0 colleague, 1 fine, 2 separated fine, 3 coarse)
+// with its slot.
+int classify_near(const AdaptiveLeaf& a, const AdaptiveLeaf& b, double ra, double rb, int& slot);
 
 // One precomputation table: family -1 the far M at level (NTERM x R);
 // 0..3 the colleague / fine / separated-fine / coarse neighbour operator n of
@@ -62,6 +80,12 @@ class Problem {
   std::vector<std::map<int, Blk<double2>>> P2P;  // leaf near-field blocks
   bool assembled = false;
   int quad_capped = 0;
+  // adaptive discretisation: its leaves (nleaf of them, L = the depth), and
+  // per leaf the near leaves with their table (family, level, slot) index
+  std::vector<AdaptiveLeaf> leaves;
+  std::vector<int> near_off, near_leaf, near_tab;
+  std::vector<TableSpec> near_specs;
+  void need_uniform(const char* what) const;
   double t_tables = 0, t_nca = 0, t_assemble = 0;
   Arena persist{size_t(64) << 20};
   Arena op{size_t(1) << 30};
@@ -99,6 +123,7 @@ class Problem {
 
  private:
   void init(const std::string& precomp_dir);
+  void init_adaptive();
   void nca_level(int j);
 };
 

@@ -46,6 +46,23 @@ int aifmm_ls_create_cached(const aifmm_ls_params* p, const char* dir, aifmm_ls_p
  * at levels 2..n_levels by device quadrature, written as the reference's text
  * cache under dir; *seconds (may be NULL) = time of the quadrature batch. */
 int aifmm_ls_write_precomputations(aifmm_ls_problem* h, const char* dir, double* seconds);
+/* The LS2DTree ctor with createAdaptiveTree + makeLevelRestriction_Tree in place of
+ * createUniformTree (LS_Assembly.hpp:255-275, 609-742, 892-1072): a box splits
+ * until the contrast interpolates from its Chebyshev nodes to its children's
+ * to 10^-tree_adaptivity (the reference driver passes 1) and kappa r / (2 pi)
+ * <= 1; then leaves two or more levels apart that are near are refined (2:1
+ * restriction).  p->n_levels is unused.  Points, entries (getMatrixEntry with
+ * the colleague, fine, separated-fine and coarse tables, LS_Assembly.hpp:474-541),
+ * the right-hand side, the dense product and the fields work on it; assembly,
+ * factorisation and the solves refuse it, since the reference's FMM2DTree
+ * solver works on the uniform tree.  The reference's own tables stop at level 2
+ * (it would read outside them for a leaf at level 1); here every leaf level's
+ * tables are formed. */
+int aifmm_ls_create_adaptive(const aifmm_ls_params* p, int tree_adaptivity, aifmm_ls_problem** out);
+/* The adaptive tree's leaves in the reference's leafNodes order (leaf k holds
+ * points 36k .. 36k+35): level, box number (children 4b + c), centre (x, y).
+ * Any output may be null; *n = capacity in, the leaf count out. */
+int aifmm_ls_leaves(aifmm_ls_problem* h, int* levels, int64_t* boxes, double* centres, int* n);
 /* One precomputation table, unrounded (FarFieldInteraction::evaluateM for
  * family -1, LS2DTree::getOperator for families 0..3 = colleague, fine,
  * separated fine, coarse; FarFieldInteraction.hpp:153-160, LS_Assembly.hpp:1290-1317):

@@ -303,3 +303,86 @@ void orc_lists(void* p, int level, int box, int* out) {
   for (int s = 0; s < 24; ++s) out[24 + s] = B.outer[s];
 }
 }
+
+// ---------------------------------------------------------- adaptive tree
+extern "C" {
+// params as orc_create; tree_adaptivity as LS2DTree's (epsilon 10^-t)
+void* orc_adaptive_create(const int* ip, double kappa, double L, const double* bumps, int nbumps,
+                          int tree_adaptivity) {
+  Params P;
+  P.n_cheb = ip[0];
+  P.degree = ip[1];
+  P.n_levels = ip[2];
+  P.tol_pow = ip[3];
+  P.qchoice = ip[4];
+  P.quantize = ip[5];
+  P.exact_bessel = ip[6];
+  P.kappa = kappa;
+  P.L = L;
+  if (bumps && nbumps > 0) P.bumps.assign(bumps, bumps + 4 * nbumps);
+  AdaptiveDisc* a = new AdaptiveDisc;
+  a->build(P, tree_adaptivity);
+  return a;
+}
+void orc_adaptive_destroy(void* p) { delete (AdaptiveDisc*)p; }
+long long orc_adaptive_n(void* p) { return ((AdaptiveDisc*)p)->N; }
+int orc_adaptive_depth(void* p) { return ((AdaptiveDisc*)p)->n_levels; }
+int orc_adaptive_splits(void* p) { return ((AdaptiveDisc*)p)->restriction_splits; }
+// leaves in leafNodes order: level, box number, centre (x, y); returns the count
+int orc_adaptive_leaves(void* p, int* levels, long long* boxes, double* centres, int cap) {
+  AdaptiveDisc* a = (AdaptiveDisc*)p;
+  const int n = (int)a->leafNodes.size();
+  for (int k = 0; k < n && k < cap; ++k) {
+    const auto& b = a->tree[a->leafNodes[k].first][a->leafNodes[k].second];
+    levels[k] = b.level;
+    boxes[k] = b.boxNumber;
+    centres[2 * k] = b.cx;
+    centres[2 * k + 1] = b.cy;
+  }
+  return n;
+}
+void orc_adaptive_points(void* p, double* xy, double* q) {
+  AdaptiveDisc* a = (AdaptiveDisc*)p;
+  for (int64_t i = 0; i < a->N; ++i) xy[2 * i] = a->px[i], xy[2 * i + 1] = a->py[i], q[i] = a->q[i];
+}
+// getIfNeighbor of leaf pairs: type (-1 far, 0 colleague, 1 coarse, 2 fine,
+// 3 separated fine) and slot
+void orc_adaptive_neighbor(void* p, int n, const int* li, const int* lj, int* type, int* num) {
+  AdaptiveDisc* a = (AdaptiveDisc*)p;
+  for (int t = 0; t < n; ++t) type[t] = a->neighbor(li[t], lj[t], num[t]);
+}
+void orc_adaptive_set_find_phi(void* p, int on) { ((AdaptiveDisc*)p)->find_phi = on != 0; }
+int orc_adaptive_entries(void* p, long long n, const long long* I, const long long* J, double* out) {
+  AdaptiveDisc* a = (AdaptiveDisc*)p;
+  try {
+    for (long long t = 0; t < n; ++t) {
+      cd e = a->entry(I[t], J[t]);
+      out[2 * t] = e.real();
+      out[2 * t + 1] = e.imag();
+    }
+  } catch (const std::exception&) {
+    return -1;
+  }
+  return 0;
+}
+// y = A x over every entry (small N only)
+int orc_adaptive_dense_matvec(void* p, const double* x, double* y) {
+  AdaptiveDisc* a = (AdaptiveDisc*)p;
+  const int64_t N = a->N;
+  try {
+    // tables first, serially (the caches are not thread-safe)
+    for (int64_t i = 0; i < N; i += a->rank)
+      for (int64_t j = 0; j < N; j += a->rank) a->entry(i, j);
+  } catch (const std::exception&) {
+    return -1;
+  }
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t i = 0; i < N; ++i) {
+    cd s = 0;
+    for (int64_t j = 0; j < N; ++j) s += a->entry(i, j) * cd(x[2 * j], x[2 * j + 1]);
+    y[2 * i] = s.real();
+    y[2 * i + 1] = s.imag();
+  }
+  return 0;
+}
+}

@@ -313,22 +313,7 @@ Mat Disc::table(int family, int level, int n) const {
 }
 
 void Disc::build(const Params& Pin) {
-  P = Pin;
-  rank = P.n_cheb * P.n_cheb;
-  nb_basis = P.degree * (P.degree + 1) / 2;
-  n_leaf = 1 << (2 * P.n_levels);
-  N = (int64_t)n_leaf * rank;
-  double rad = P.L;
-  for (int j = 0; j < P.n_levels; ++j) rad *= 0.5;
-  r_leaf = rad;
-  // Chebyshev nodes (LS_Assembly.hpp:567-580)
-  cheb1d.resize(P.n_cheb);
-  for (int k = 0; k < P.n_cheb; ++k) cheb1d[k] = -cos((k + 0.5) / P.n_cheb * PI);
-  // basis order (FarFieldInteraction.hpp:46-54)
-  basis_a.clear();
-  basis_b.clear();
-  for (int d = 0; d < P.degree; ++d)
-    for (int a = 0; a < d + 1; ++a) basis_a.push_back(a), basis_b.push_back(d - a);
+  init_basis(Pin);
   // points: leaf k, index jy*n+kx at center + r*(xi_kx, xi_jy) (LS_Assembly.hpp:1074-1087, 543-550)
   px.resize(N);
   py.resize(N);
@@ -345,6 +330,37 @@ void Disc::build(const Params& Pin) {
       }
   }
   for (int64_t i = 0; i < N; ++i) q[i] = contrast(P, px[i], py[i]);
+  // the tables the uniform tree reads: M and the colleague operators at the
+  // leaf level (LS_Assembly.hpp:496, 527)
+  M = table(-1, P.n_levels, 0);
+  for (int nb = 0; nb < 9; ++nb) T[nb] = table(0, P.n_levels, nb);
+  if (P.quantize) {
+    for (auto& v : M.a) v = round6(v);
+    for (int nb = 0; nb < 9; ++nb)
+      for (auto& v : T[nb].a) v = round6(v);
+  }
+}
+
+cd round6c(cd v) { return round6(v); }
+
+// Chebyshev nodes, the basis order and Q^+ (no points, no tables).
+void Disc::init_basis(const Params& Pin) {
+  P = Pin;
+  rank = P.n_cheb * P.n_cheb;
+  nb_basis = P.degree * (P.degree + 1) / 2;
+  n_leaf = 1 << (2 * P.n_levels);
+  N = (int64_t)n_leaf * rank;
+  double rad = P.L;
+  for (int j = 0; j < P.n_levels; ++j) rad *= 0.5;
+  r_leaf = rad;
+  // Chebyshev nodes (LS_Assembly.hpp:567-580)
+  cheb1d.resize(P.n_cheb);
+  for (int k = 0; k < P.n_cheb; ++k) cheb1d[k] = -cos((k + 0.5) / P.n_cheb * PI);
+  // basis order (FarFieldInteraction.hpp:46-54)
+  basis_a.clear();
+  basis_b.clear();
+  for (int d = 0; d < P.degree; ++d)
+    for (int a = 0; a < d + 1; ++a) basis_a.push_back(a), basis_b.push_back(d - a);
   // Q and its pseudo-inverse (FarFieldInteraction.hpp:138-150)
   Mat Qm(rank, nb_basis);
   for (int i = 0; i < rank; ++i) {
@@ -369,15 +385,6 @@ void Disc::build(const Params& Pin) {
     for (int i = 0; i < nb_basis; ++i)
       for (int col = 0; col < rank; ++col) Qpinv(f.perm[i], col) = Rinv_Q1H(i, col);
   }
-  // the tables the uniform tree reads: M and the colleague operators at the
-  // leaf level (LS_Assembly.hpp:496, 527)
-  M = table(-1, P.n_levels, 0);
-  for (int nb = 0; nb < 9; ++nb) T[nb] = table(0, P.n_levels, nb);
-  if (P.quantize) {
-    for (auto& v : M.a) v = round6(v);
-    for (int nb = 0; nb < 9; ++nb)
-      for (auto& v : T[nb].a) v = round6(v);
-  }
 }
 
 // Graf far-field sum for target point i and source leaf box (LS_Assembly.hpp:518-529).
@@ -385,13 +392,20 @@ cd Disc::far_sum(int64_t i, int leaf_j, int idx_j) const {
   int ix, iy;
   box_xy(P.n_levels, leaf_j, ix, iy);
   double cx = -P.L + (2 * ix + 1) * r_leaf, cy = -P.L + (2 * iy + 1) * r_leaf;
-  double phi = atan2(py[i] - cy, px[i] - cx);
-  double R2 = (py[i] - cy) * (py[i] - cy) + (px[i] - cx) * (px[i] - cx);
-  double R = P.kappa * sqrt(R2);
+  return graf_series(M, idx_j, px[i], py[i], cx, cy, P.kappa, series, P.exact_bessel != 0);
+}
+
+// sum_{n=-series}^{series} M(n + series, idx_j) H_n(kappa rho) e^{i n phi}, with
+// (rho, phi) the polar form of (x - cx, y - cy) (LS_Assembly.hpp:518-529)
+cd graf_series(const Mat& M, int idx_j, double x, double y, double cx, double cy, double kappa, int series,
+               bool exact) {
+  double phi = atan2(y - cy, x - cx);
+  double R2 = (y - cy) * (y - cy) + (x - cx) * (x - cx);
+  double R = kappa * sqrt(R2);
   double J[64], Y[64];
-  bessel_jy(R, series, J, Y, P.exact_bessel != 0);
+  bessel_jy(R, series, J, Y, exact);
   cd s = 0;
-  if (P.exact_bessel) {
+  if (exact) {
     for (int n = -series; n <= series; ++n) {
       int an = n < 0 ? -n : n;
       double sg = (n < 0 && (an % 2)) ? -1.0 : 1.0;

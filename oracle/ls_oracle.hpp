@@ -118,6 +118,7 @@ struct Disc {
   // (the scattered field from the solution, hybrid.cpp:264-281)
   bool find_phi = true;
   void build(const Params& P);
+  void init_basis(const Params& P);  // nodes, basis, Q^+ only
   double box_radius(int j) const;
   // a precomputation table, unrounded: family -1 the far M at `level`
   // (121 x rank); 0..3 the colleague / fine / separated-fine / coarse
@@ -125,6 +126,49 @@ struct Disc {
   Mat table(int family, int level, int n) const;
   cd entry(int64_t i, int64_t j) const;
   cd far_sum(int64_t i, int leaf_j, int idx_j) const;
+};
+
+// The 6-significant-digit rounding of the precomputation text cache.
+cd round6c(cd v);
+// The Graf series of a far entry: sum_n M(n + series, idx_j) H_n(kappa rho)
+// e^{i n phi}, (rho, phi) the polar form of (x - cx, y - cy).
+cd graf_series(const Mat& M, int idx_j, double x, double y, double cx, double cy, double kappa, int series,
+               bool exact);
+
+// Adaptive quadtree with 2:1 level restriction and the entries between its
+// leaves (LS2DTree::createAdaptiveTree, makeLevelRestriction_Tree,
+// getIfNeighbor, getMatrixEntry; LS_Assembly.hpp:283-541, 609-742, 892-1072).
+// Laid out as the reference keeps it: per level the boxes in creation order,
+// indexTree holding their box numbers, leafNodes the (level, index) pairs.
+struct AdaptiveDisc {
+  struct ABox {
+    double cx = 0, cy = 0, radius = 0;
+    int level = 0, boxNumber = -1;
+    bool isLeaf = false;
+  };
+  Disc D;  // nodes, basis, Q^+ and the table quadrature
+  double epsilon_tree = 0.1;
+  int n_levels = 0, rank = 0;
+  int restriction_splits = 0;  // boxes refineBox split
+  std::vector<std::vector<ABox>> tree;
+  std::vector<std::vector<int>> indexTree;
+  std::vector<std::pair<int, int>> leafNodes;
+  std::vector<double> boxRadius;
+  std::vector<double> px, py, q;
+  int64_t N = 0;
+  bool find_phi = true;
+  void build(const Params& P, int tree_adaptivity);
+  // getIfNeighbor between leaves li (target) and lj: the reference's
+  // typeOfNeighbor (-1 far, 0 colleague, 1 coarse, 2 fine, 3 separated fine)
+  // and NumOfNeighbor (-1 when no slot matched)
+  int neighbor(int li, int lj, int& num) const;
+  cd entry(int64_t i, int64_t j);
+  const Mat& near_table(int type, int level, int num);
+  const Mat& far_table(int level);
+
+ private:
+  std::map<std::vector<int>, Mat> ntab_;  // (type, level, num) -> table, rounded as the cache holds it
+  std::map<int, Mat> ftab_;               // level -> M
 };
 
 // Quadtree index helpers (reference child order 0=(-,-) 1=(+,-) 2=(+,+)

@@ -66,6 +66,25 @@ def lib():
         L.orc_qr_solve.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp, dp]
         L.orc_lu_solve.argtypes = [ctypes.c_int, dp, ctypes.c_int, dp, dp]
         L.orc_lists.argtypes = [P, ctypes.c_int, ctypes.c_int, ip]
+        L.orc_adaptive_create.restype = P
+        L.orc_adaptive_create.argtypes = [ip, ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_int,
+                                          ctypes.c_int]
+        L.orc_adaptive_destroy.argtypes = [P]
+        L.orc_adaptive_n.restype = ctypes.c_longlong
+        L.orc_adaptive_n.argtypes = [P]
+        L.orc_adaptive_depth.restype = ctypes.c_int
+        L.orc_adaptive_depth.argtypes = [P]
+        L.orc_adaptive_splits.restype = ctypes.c_int
+        L.orc_adaptive_splits.argtypes = [P]
+        L.orc_adaptive_leaves.restype = ctypes.c_int
+        L.orc_adaptive_leaves.argtypes = [P, ip, lp, dp, ctypes.c_int]
+        L.orc_adaptive_points.argtypes = [P, dp, dp]
+        L.orc_adaptive_neighbor.argtypes = [P, ctypes.c_int, ip, ip, ip, ip]
+        L.orc_adaptive_set_find_phi.argtypes = [P, ctypes.c_int]
+        L.orc_adaptive_entries.restype = ctypes.c_int
+        L.orc_adaptive_entries.argtypes = [P, ctypes.c_longlong, lp, lp, dp]
+        L.orc_adaptive_dense_matvec.restype = ctypes.c_int
+        L.orc_adaptive_dense_matvec.argtypes = [P, dp, dp]
         _lib = L
     return _lib
 
@@ -257,3 +276,68 @@ def lu_solve(A, B):
     X = np.zeros(2 * n * nr)
     lib().orc_lu_solve(n, _c(A.T.copy()), nr, _c(B.T.copy()), X)
     return X.view(np.complex128).reshape(nr, n).T
+
+
+class AdaptiveProblem:
+    """The adaptive quadtree with 2:1 level restriction and the entries
+    between its leaves (LS2DTree::createAdaptiveTree, makeLevelRestriction_Tree,
+    getIfNeighbor, getMatrixEntry) in the oracle."""
+
+    def __init__(self, kappa, tree_adaptivity=1, qchoice=0, n_cheb=6, degree=6, L=1.0, bumps=None, quantize=True,
+                 exact_bessel=False):
+        ip = np.array([n_cheb, degree, 0, 8, qchoice, int(quantize), int(exact_bessel)], np.int32)
+        nb = 0
+        self._bumps = None
+        if bumps is not None:
+            self._bumps = np.ascontiguousarray(np.asarray(bumps, np.float64).reshape(-1, 4))
+            nb = self._bumps.shape[0]
+        ptr = self._bumps.ctypes.data if self._bumps is not None else None
+        self.h = lib().orc_adaptive_create(ip, float(kappa), float(L), ptr, nb, int(tree_adaptivity))
+        self.kappa, self.L, self.rank = kappa, L, n_cheb * n_cheb
+        self.N = int(lib().orc_adaptive_n(self.h))
+        self.depth = int(lib().orc_adaptive_depth(self.h))
+        self.restriction_splits = int(lib().orc_adaptive_splits(self.h))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_adaptive_destroy(self.h)
+            self.h = None
+
+    def leaves(self):
+        """(levels, box numbers, centres) in leafNodes order."""
+        n = self.N // self.rank
+        lv, bx, c = np.zeros(n, np.int32), np.zeros(n, np.int64), np.zeros(2 * n)
+        lib().orc_adaptive_leaves(self.h, lv, bx, c, n)
+        return lv, bx, c.reshape(-1, 2)
+
+    def points(self):
+        xy = np.zeros(2 * self.N)
+        q = np.zeros(self.N)
+        lib().orc_adaptive_points(self.h, xy, q)
+        return xy.reshape(-1, 2), q
+
+    def neighbor(self, li, lj):
+        """getIfNeighbor of leaf pairs: (type, slot); type -1 far, 0 colleague,
+        1 coarse, 2 fine, 3 separated fine."""
+        li = np.ascontiguousarray(li, np.int32)
+        lj = np.ascontiguousarray(lj, np.int32)
+        t, s = np.zeros(li.size, np.int32), np.zeros(li.size, np.int32)
+        lib().orc_adaptive_neighbor(self.h, li.size, li, lj, t, s)
+        return t, s
+
+    def set_find_phi(self, on):
+        lib().orc_adaptive_set_find_phi(self.h, int(bool(on)))
+
+    def entries(self, I, J):
+        I = np.ascontiguousarray(I, np.int64)
+        J = np.ascontiguousarray(J, np.int64)
+        out = np.zeros(2 * I.size)
+        if lib().orc_adaptive_entries(self.h, I.size, I, J, out):
+            raise RuntimeError("adaptive oracle: a near pair matched no neighbour slot")
+        return out.view(np.complex128)
+
+    def dense_matvec(self, x):
+        y = np.zeros(2 * self.N)
+        if lib().orc_adaptive_dense_matvec(self.h, _c(x), y):
+            raise RuntimeError("adaptive oracle: a near pair matched no neighbour slot")
+        return y.view(np.complex128)

@@ -88,6 +88,7 @@ class Server {
 
   Server(int64_t budget, int slots, int max_batch_rhs = 64, int max_batch_problems = 32)
       : budget_(budget), max_batch_rhs_(max_batch_rhs), max_batch_problems_(max_batch_problems) {
+    max_builds_ = slots / 2 > 1 ? slots / 2 : 1;
     for (int i = 0; i < slots; ++i) workers_.emplace_back([this] { worker(); });
   }
 
@@ -240,6 +241,7 @@ class Server {
   };
   struct Work {
     int kind = -1;  // 0 factorise, 1 solve one problem (factorising first if not resident), 2 batch of problems
+    bool build = false;  // a factorisation runs first (new or evicted problem)
     std::vector<int64_t> problems;
     std::vector<std::shared_ptr<Req>> reqs;  // kind 2: one per problem, same order
   };
@@ -256,6 +258,11 @@ class Server {
   int max_batch_rhs_, max_batch_problems_;
   bool stop_ = false;
   int running_ = 0;
+  // Work that factorises (a new problem, or a solve of an evicted one) holds
+  // a slot for the length of a factorisation: at most max_builds_ slots do at
+  // once (half the slots, at least one), so solves of resident problems keep
+  // slots while factorisations queue.
+  int builds_running_ = 0, max_builds_ = 1;
   Counters c_;
   std::vector<std::thread> workers_;
   // largest measured footprint per (levels, precision): the admission
@@ -330,7 +337,8 @@ class Server {
     return true;
   }
 
-  // Choose the next runnable work in arrival order.  Caller holds mu_.
+  // Choose the next runnable work in arrival order; work that factorises is
+  // passed over while max_builds_ factorisations run.  Caller holds mu_.
   bool pick(Work& w) {
     for (auto it = queue_.begin(); it != queue_.end();) {
       std::shared_ptr<Req> r = *it;
@@ -342,6 +350,11 @@ class Server {
       }
       Entry& e = ce->second;
       if (e.busy) {
+        ++it;
+        continue;
+      }
+      const bool needs_build = e.h == Handle();
+      if (needs_build && builds_running_ >= max_builds_) {
         ++it;
         continue;
       }
@@ -357,6 +370,8 @@ class Server {
       w = Work();
       w.problems.push_back(r->problem);
       e.busy = true;
+      w.build = needs_build;
+      builds_running_ += needs_build;
       if (r->kind == 0) {
         w.kind = 0;
         w.reqs.push_back(r);
@@ -517,6 +532,7 @@ class Server {
       ++c_.completed;
     }
     --running_;
+    builds_running_ -= w.build;
     reap(lk);
     cv_.notify_all();
     cv_done_.notify_all();

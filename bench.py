@@ -70,13 +70,15 @@ def measured_traffic(kernel):
 
 
 def serving_trace(A, seconds, rate, budget_gib, slots, seed, refine_tol, sample):
-    """A shortened serving trace: Poisson arrivals for `seconds`; 30 % are new
+    """The serving trace: Poisson arrivals for `seconds`; 30 % are new
     factorisations (levels 4-6, kappa ~ U[10,60], 2-6 random bumps, complex64),
-    70 % solve 1-64 plane waves against a problem submitted earlier.  The server
+    70 % solve 1-64 plane waves against a problem submitted earlier (3 in 4
+    against one of the 8 most recent, the rest against any).  The server
     keeps factorisations resident within the device-memory budget (LRU
-    eviction).  Latency is submission to completion, measured by the server;
-    `sample` solves are checked against an FP64 solution (complex128 factor
-    refined to 1e-13)."""
+    eviction; an evicted problem is factorised again when a solve needs it).
+    Latency is submission to completion, measured by the server; `sample`
+    solves are checked against an FP64 solution (complex128 factor refined to
+    1e-13)."""
     import threading
     rng = np.random.default_rng(seed)
     pts = {}
@@ -135,7 +137,12 @@ def serving_trace(A, seconds, rate, budget_gib, slots, seed, refine_tol, sample)
             waves.append(W)
             tickets.append(("factor", t, {"levels": lv}))
         else:
-            j = int(rng.integers(len(probs)))
+            # solves against cached factors: 3 in 4 go to one of the 8 most
+            # recently submitted problems, the rest to any earlier one
+            if rng.uniform() < 0.75:
+                j = len(probs) - 1 - int(rng.integers(min(8, len(probs))))
+            else:
+                j = int(rng.integers(len(probs)))
             k = int(rng.integers(1, 65))
             B = waves[j][:, np.arange(k) % 8]
             check = n_sample < sample and probs[j]["levels"] <= 5 and k <= 8
@@ -426,7 +433,7 @@ def main():
     ap.add_argument("--many-rhs", type=int, default=256, help="RHS count of the 7-level many-RHS line (0: skip)")
     ap.add_argument("--serve-seconds", type=float, default=20.0, help="length of the serving trace (0: skip)")
     ap.add_argument("--serve-rate", type=float, default=4.0, help="serving arrivals per second")
-    ap.add_argument("--serve-budget-gib", type=float, default=64.0, help="serving device-memory budget")
+    ap.add_argument("--serve-budget-gib", type=float, default=160.0, help="serving device-memory budget")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} refused: this solver runs on one GPU; "

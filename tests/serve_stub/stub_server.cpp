@@ -37,6 +37,7 @@ std::mutex reg_mu;
 std::set<Stub*> live;
 std::atomic<int64_t> live_bytes{0}, peak_bytes{0};
 std::atomic<int> violations{0}, builds{0}, destroys{0}, batch_calls{0};
+std::atomic<int> building{0}, peak_building{0};  // builds running at once
 
 void sleep_ms(int ms) { std::this_thread::sleep_for(std::chrono::milliseconds(ms)); }
 
@@ -62,7 +63,12 @@ struct StubOps {
     while (now > pk && !peak_bytes.compare_exchange_weak(pk, now)) {
     }
     ++builds;
+    const int nb = ++building;
+    int pb = peak_building.load();
+    while (nb > pb && !peak_building.compare_exchange_weak(pb, nb)) {
+    }
     sleep_ms(cfg.build_ms);
+    --building;
     return 0;
   }
   static int solve(Handle h, int nrhs, const double* b, double* x, double, int, double* st, std::string&) {
@@ -117,7 +123,8 @@ extern "C" {
 
 // out: completed, failed, max submit latency (ms), peak live bytes, budget,
 // evictions, violations, cross-problem batches, wrong results, builds,
-// destroys, live bytes at the end, stub batch calls, solves refused because
+// destroys, live bytes at the end, stub batch calls, peak concurrent builds
+// (out[14]), solves refused because
 // their problem was released after they were queued
 int stub_trace(int seed, int nreq, int slots, int nsub, int build_ms, int solve_ms, int destroy_ms,
                int64_t budget_problems, double* out) {
@@ -128,6 +135,7 @@ int stub_trace(int seed, int nreq, int slots, int nsub, int build_ms, int solve_
   violations = 0;
   peak_bytes = 0;
   builds = destroys = batch_calls = 0;
+  building = peak_building = 0;
   const int64_t budget = budget_problems * cfg.bytes_per_problem;
   const int64_t N = 64;  // n_levels 2, n_cheb 2
   std::atomic<int> completed{0}, failed{0}, wrong{0}, refused{0};
@@ -247,6 +255,7 @@ int stub_trace(int seed, int nreq, int slots, int nsub, int build_ms, int solve_
   out[11] = (double)live_bytes.load();
   out[12] = batch_calls;
   out[13] = refused;
+  out[14] = peak_building;
   return 0;
 }
 }

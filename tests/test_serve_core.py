@@ -33,11 +33,11 @@ def stub(tmp_path_factory):
 
 
 def run(L, seed, nreq=60, slots=4, nsub=4, build_ms=4, solve_ms=2, destroy_ms=25, budget_problems=3):
-    out = np.zeros(14)
+    out = np.zeros(15)
     L.stub_trace(seed, nreq, slots, nsub, build_ms, solve_ms, destroy_ms, budget_problems,
                  out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
     keys = ["completed", "failed", "max_submit_ms", "peak_bytes", "budget", "evictions", "violations",
-            "cross_batches", "wrong", "builds", "destroys", "live_bytes_end", "batch_calls", "refused"]
+            "cross_batches", "wrong", "builds", "destroys", "live_bytes_end", "batch_calls", "refused", "peak_builds"]
     return dict(zip(keys, out.tolist()))
 
 
@@ -70,3 +70,13 @@ def test_cross_problem_batches_form(stub):
     print(f"\n  {r}")
     assert r["violations"] == 0 and r["wrong"] == 0 and r["failed"] == 0
     assert r["cross_batches"] > 0 and r["batch_calls"] == r["cross_batches"]
+
+
+def test_factorisations_leave_slots_to_solves(stub):
+    """Four slots, slow builds: at most two slots factorise at once (the
+    others keep serving resident problems in arrival order), and every
+    request still ends."""
+    r = run(stub, 5, nreq=80, slots=4, nsub=4, build_ms=20, solve_ms=2, destroy_ms=5, budget_problems=6)
+    print(f"\n  {r}")
+    assert r["violations"] == 0 and r["wrong"] == 0 and r["failed"] == 0
+    assert 1 <= r["peak_builds"] <= 2

@@ -466,6 +466,19 @@ def selftest_cpqr(A, thr, k, c64=False):
     return r.value, C.view(np.complex128).reshape(k, m).T
 
 
+def selftest_cpqr_raw(A, thr, reg, early=False, c64=False):
+    """Column-pivoted QR of A by one kernel (reg: the register kernel, else
+    the global-memory one): (R and reflectors as one m x n array, tau,
+    permutation, rank, nonzero pivots)."""
+    m, n = A.shape
+    C = np.zeros(2 * (m * n + min(m, n)))
+    io = np.zeros(2 + n, dtype=np.int32)
+    _chk(lib().aifmm_ls_selftest(10 * c64 + 5, m, n, int(bool(reg)) + 2 * int(bool(early)), _dp(_cm(A)),
+                                 _dp(np.zeros(2)), _dp(C), io.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), thr))
+    z = C.view(np.complex128)
+    return z[:m * n].reshape(n, m).T, z[m * n:], io[2:].copy(), int(io[0]), int(io[1])
+
+
 def _blocks(a):
     """Stacked blocks (..., r, c) to consecutive column-major interleaved complex."""
     return np.ascontiguousarray(np.swapaxes(np.asarray(a, np.complex128), -1, -2)).view(np.float64)

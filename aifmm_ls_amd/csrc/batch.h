@@ -333,15 +333,21 @@ struct QRBatch {
     if (d.empty() && !merge_group()) return;
     merge_or_run(this, scratch, s);
   }
+  // the matrices that fit k_cpqr_reg first, the rest after; one launch per
+  // kernel (AIFMM_CPQR_REG=0: every matrix through k_cpqr)
   static int run_many(void* const* bs, int n, Arena& scratch, hipStream_t s) {
     std::vector<QRDesc<T>> d = take_all(bs, n, &QRBatch::d);
     if (d.empty()) return 0;
+    static const bool reg = !getenv("AIFMM_CPQR_REG") || atoi(getenv("AIFMM_CPQR_REG")) != 0;
+    const auto small = [](const QRDesc<T>& q) { return cpqr_shape(q.m, q.n) == CPQR_REG; };
+    const int ns = reg ? (int)(std::stable_partition(d.begin(), d.end(), small) - d.begin()) : 0;
     auto* dd = upload(scratch, d, s);
     double fl = 0;
     if (prof_on())
       for (auto& q : d) fl += 8.0 * ((double)q.m * q.n * q.n - (double)q.n * q.n * q.n / 3);
     ProfScope ps(s, KNAME(T, "k_cpqr"), 0, fl);
-    launch_cpqr<T>((int)d.size(), dd, s);
+    launch_cpqr<T>(ns, dd, s, CPQR_REG);
+    launch_cpqr<T>((int)d.size() - ns, dd + ns, s, CPQR_GLOBAL);
     return 1;
   }
 };

@@ -739,6 +739,10 @@ int aifmm_ls_stats(aifmm_ls_problem* h, double* out, int n) {
 // op 2: CPQR of A (m x n) at threshold thr -> iout[0] = rank, C = Q[:, :k]
 // op 3: C = A(m x m)^{-1} B(m x n) by CPQR + ColPivHouseholderQR::solve,
 //       iout[0] = number of nonzero pivots
+// op 4 (complex64 only, op 14): the pivot-block growth gate (below)
+// op 5: CPQR of A (m x n) at threshold thr by the kernel k & 1 (0 k_cpqr,
+//       1 k_cpqr_reg), with early stop if k & 2: C = the overwritten A then
+//       tau, iout = rank, nonzero pivots, perm[0..n)
 // op +10: the same in complex64.  Column-major interleaved complex.
 template <class T>
 static void selftest_t(int op, int m, int n, int k, const double* A, const double* B, double* C, int* iout,
@@ -794,13 +798,43 @@ static void selftest_t(int op, int m, int n, int k, const double* A, const doubl
     d.rank_out = a.get<int>(1);
     d.maxpiv_out = nullptr;
     std::vector<QRDesc<T>> v{d};
-    launch_cpqr<T>(1, upload(sc, v, s), s);
+    launch_cpqr<T>(1, upload(sc, v, s), s, cpqr_shape(m, n));
     Blk<T> Q = new_blk<T>(a, m, k);
     std::vector<QFormDesc<T>> f{QFormDesc<T>{dA.p, d.tau, Q.p, m, n, k, m, m}};
     launch_qform<T>(1, upload(sc, f, s), s);
     HIP_OK(hipStreamSynchronize(s));
     HIP_OK(hipMemcpy(iout, d.rank_out, 4, hipMemcpyDeviceToHost));
     down(Q, C);
+  } else if (op == 5) {
+    using Rl = typename CplxOf<T>::real;
+    const int shape = k & 1;
+    if (shape == CPQR_REG && cpqr_shape(m, n) != CPQR_REG)
+      throw std::runtime_error("k_cpqr_reg takes at most 36 rows and 256 columns");
+    Blk<T> dA = up(A, m, n);
+    const int mn = std::min(m, n);
+    Blk<T> tau = new_blk<T>(a, std::max(mn, 1), 1);
+    QRDesc<T> d{};
+    d.A = dA.p;
+    d.m = m;
+    d.n = n;
+    d.lda = m;
+    d.tau = tau.p;
+    d.perm = a.get<int>(n);
+    d.work = a.get<Rl>(2 * (size_t)n);
+    d.thr = thr;
+    d.early = (k & 2) ? 1 : 0;
+    int* outs = a.get<int>(2);
+    d.rank_out = outs;
+    d.nz_out = outs + 1;
+    d.maxpiv_out = nullptr;
+    HIP_OK(hipMemsetAsync(tau.p, 0, sizeof(T) * std::max(mn, 1), s));
+    std::vector<QRDesc<T>> v{d};
+    launch_cpqr<T>(1, upload(sc, v, s), s, shape);
+    HIP_OK(hipStreamSynchronize(s));
+    down(dA, C);
+    if (mn) down(tau.sub(0, 0, mn, 1), C + 2 * (size_t)m * n);
+    HIP_OK(hipMemcpy(iout, outs, 8, hipMemcpyDeviceToHost));
+    HIP_OK(hipMemcpy(iout + 2, d.perm, 4 * (size_t)n, hipMemcpyDeviceToHost));
   } else if (op == 3) {
     Blk<T> dA = up(A, m, m), dB = up(B, m, n);
     QRFact<T> f = new_qr(a, dA);

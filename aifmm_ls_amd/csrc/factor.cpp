@@ -134,6 +134,7 @@ void Factor<T>::eliminate_color(int j, int c) {
   DenseOps<T> dn;
   GemmBatch<T> g;
   double fl = 0;
+  double h0 = now_s();
   // 1. W = [L2P | row blocks], LU of A, W <- A^{-1} W
   for (size_t t = 0; t < S.size(); ++t) {
     int k = S[t];
@@ -172,6 +173,7 @@ void Factor<T>::eliminate_color(int j, int c) {
     if (K.m) dn.solve(sc, K.fa, K.W);
   }
   dn.run_solve(sc, pb.stream);
+  hs_[0] += now_s() - h0, h0 = now_s();
   // 2. G = -P2M W
   for (size_t t = 0; t < S.size(); ++t) {
     FBox& K = Bx[S[t]];
@@ -229,6 +231,7 @@ void Factor<T>::eliminate_color(int j, int c) {
     }
   }
   g.run(sc, pb.stream);
+  hs_[1] += now_s() - h0, h0 = now_s();
   // 6. neighbour equations: target -= C_nn V[:, block]
   CopyBatch<T> zero;
   auto ensure = [&](std::map<int, Blk<T>>& mp, int key, int rows, int cols) -> Blk<T> {
@@ -283,6 +286,7 @@ void Factor<T>::eliminate_color(int j, int c) {
   }
   zero.run(sc, pb.stream);
   g.run(sc, pb.stream);
+  hs_[2] += now_s() - h0;
   for (int k : S) Bx[k].elim = true;
   sc.reset();  // scratch reuse is ordered on the stream
   st.flops += fl;
@@ -302,6 +306,7 @@ void Factor<T>::recompress(int j, int c) {
   struct Fill {
     int u, p, kind;
   };
+  double h0 = now_s();
   std::vector<Fill> cand;
   std::vector<const Blk<T>*> cb;
   auto blk_of = [&](const Fill& f) -> Blk<T>& {
@@ -336,6 +341,7 @@ void Factor<T>::recompress(int j, int c) {
   double* dn = cand.empty() ? nullptr : sc.get<double>(cand.size());
   for (size_t i = 0; i < cand.size(); ++i) fro.add(cb[i]->p, cb[i]->r, cb[i]->c, cb[i]->ld, dn + i);
   fro.run(sc, pb.stream);
+  hs_[3] += now_s() - h0;
   if (cand.empty()) return;
   std::vector<double> norms(cand.size());
   {
@@ -343,6 +349,7 @@ void Factor<T>::recompress(int j, int c) {
     d2h(norms.data(), dn, 8 * cand.size(), pb.stream);
     t_wait_ += now_s() - w0;
   }
+  h0 = now_s();
   std::vector<Fill> fills;
   for (size_t i = 0; i < cand.size(); ++i) {
     if (norms[i] > 1e-10) fills.push_back(cand[i]);
@@ -431,6 +438,7 @@ void Factor<T>::recompress(int j, int c) {
     st.flops += mm >= nn ? 8.0 * (mm * nn * nn - nn * nn * nn / 3) : 8.0 * (nn * mm * mm - mm * mm * mm / 3);
   }
   qb.run(sc, pb.stream);
+  hs_[4] += now_s() - h0;
   std::vector<int> rk(2 * nt);
   const double w1 = now_s();
   d2h(rk.data(), dr, 8 * nt, pb.stream);
@@ -444,9 +452,12 @@ void Factor<T>::recompress(int j, int c) {
   static const bool dbg = getenv("AIFMM_DEBUG") != nullptr;
   if (dbg) {
     fprintf(stderr, "[gpu] level %d: %zu candidates, %zu fills, %d touched:", j, cand.size(), fills.size(), nt);
-    for (int i = 0; i < nt; ++i) fprintf(stderr, " %d(%d,%d|r%d)", tl[i], rk[2 * i], rk[2 * i + 1], Bx[tl[i]].r);
+    for (int i = 0; i < nt; ++i)
+      fprintf(stderr, " %d(%d,%d|r%d|%dx%d,%d)", tl[i], rk[2 * i], rk[2 * i + 1], Bx[tl[i]].r, Arow[i].r, Arow[i].c,
+              Acol[i].c);
     fprintf(stderr, "\n");
   }
+  h0 = now_s();
   std::vector<Blk<T>> QR(nt), QC(nt), Rrow(nt), Rcol(nt);
   QFormBatch<T> qfb;
   std::vector<QFormDesc<T>>& qf = qfb.d;
@@ -483,6 +494,7 @@ void Factor<T>::recompress(int j, int c) {
     for (int s = 0; s < 24; ++s)
       if (LS[u].outer[s] != -1) v.push_back(LS[u].outer[s]);
   };
+  hs_[5] += now_s() - h0, h0 = now_s();
   std::vector<int> il;
   // row rotation of M2L[u][*]
   for (int i = 0; i < nt; ++i) {
@@ -510,6 +522,7 @@ void Factor<T>::recompress(int j, int c) {
     }
   }
   g.run(sc, pb.stream);
+  hs_[6] += now_s() - h0, h0 = now_s();
   // fill-ins projected onto M2L
   std::vector<Blk<T>> tmp(fills.size());
   for (size_t q = 0; q < fills.size(); ++q) {
@@ -577,6 +590,7 @@ void Factor<T>::recompress(int j, int c) {
     FBox& U = Bx[f.u];
     (f.kind == 0 ? U.P2P : f.kind == 1 ? U.M2P : U.P2L).erase(f.p);
   }
+  hs_[7] += now_s() - h0;
   sc.reset();  // scratch reuse is ordered on the stream
 }
 
@@ -696,11 +710,16 @@ void Factor<T>::factor() {
       }
   }
   tick(4, tt);
+  double h0 = now_s();
   sweep_operators();
+  hs_[8] += now_s() - h0, h0 = now_s();
   compose_sweep();
+  hs_[9] += now_s() - h0;
   tick(5, tt);
   dense_top_alloc();
+  h0 = now_s();
   build_plan(1);
+  hs_[10] += now_s() - h0;
   tick(6, tt);
   dense_top_fill();
   HIP_OK(hipStreamSynchronize(pb.stream));
@@ -713,6 +732,13 @@ void Factor<T>::factor() {
             "%.1f, plan %.1f, dense top %.1f; recompress waits %.1f ms\n",
             1e3 * tph[0], 1e3 * tph[1], 1e3 * tph[2], 1e3 * tph[3], 1e3 * tph[4], 1e3 * tph[5], 1e3 * tph[6],
             1e3 * tph[7], 1e3 * t_wait_);
+  if (timers)
+    fprintf(stderr,
+            "[factor] host: eliminate W/LU %.1f ms, G/X/V %.1f, neighbour updates %.1f; recompress scan %.1f, "
+            "stacks %.1f, Q/R %.1f, M2L rotations %.1f, fills/parents/bases %.1f; sweep operators %.1f, compose "
+            "%.1f, plan %.1f\n",
+            1e3 * hs_[0], 1e3 * hs_[1], 1e3 * hs_[2], 1e3 * hs_[3], 1e3 * hs_[4], 1e3 * hs_[5], 1e3 * hs_[6],
+            1e3 * hs_[7], 1e3 * hs_[8], 1e3 * hs_[9], 1e3 * hs_[10]);
   st.bytes = ar.used();
   for (int j = 2; j <= L; ++j)
     for (auto& F : B[j]) st.max_rank = std::max(st.max_rank, F.r);

@@ -254,6 +254,7 @@ class Factor {
   void base_factor();
   Blk<T> nb_blk(int j, int row, int col);  // coupling block (row eq, col unknown)
   double t_wait_ = 0;                       // host seconds waiting at recompression readbacks
+  double hs_[12] = {};                      // host seconds per sub-phase (AIFMM_FACTOR_TIMERS)
 };
 
 }  // namespace aifmm

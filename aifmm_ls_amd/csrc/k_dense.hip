@@ -588,7 +588,8 @@ __global__ __launch_bounds__(NQ) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
   // element types (a complex64 factor that dropped pivots below the float
   // epsilon would lose the badly scaled rows of low-contrast boxes)
   double maxnorm = 0;
-  for (int j = 0; j < n; ++j) maxnorm = fmax(maxnorm, (double)nrm[j]);
+  for (int j = threadIdx.x; j < n; j += NQ) maxnorm = fmax(maxnorm, (double)nrm[j]);
+  maxnorm = block_maxR<double, NQ>(maxnorm, shs);
   const double eps_r = d.nz_eps > 0 ? d.nz_eps : 2.220446049250313e-16;
   const double th_helper = (maxnorm * eps_r) * (maxnorm * eps_r) / (m > 0 ? m : 1);
   int nonzero = size;
@@ -708,6 +709,209 @@ __global__ __launch_bounds__(NQ) void k_cpqr(const QRDesc<T>* __restrict__ ds) {
     double pre = maxpiv * d.thr;
     for (int i = 0; i < stop; ++i)
       if (hypot((double)A[i + (size_t)ld * i].x, (double)A[i + (size_t)ld * i].y) > pre) ++r;
+    *d.rank_out = r;
+    if (d.maxpiv_out) *d.maxpiv_out = maxpiv;
+    if (d.nz_out) *d.nz_out = nonzero;
+  }
+}
+
+// The same column-pivoted QR for a matrix of at most MR rows and NQ columns
+// with one thread per column and the column in registers: pivot choice (the
+// largest running norm, the first position on ties), reflectors, early stop,
+// norm downdates with exact recomputation and every output as k_cpqr, but a
+// step costs two workgroup barriers and reads the reflector from LDS instead
+// of making several dependent round trips to global memory.
+//  - Columns keep their thread; a swap exchanges positions.  The pivot column
+//    of step k is final after the step and is written to A right away; the
+//    others are written to their positions at the end.
+//  - The reflector sits in LDS as a full column: 0 above row k and from row m
+//    on, 1 at row k, v below.  Every thread then runs the same unpredicated
+//    loops: s = t (v^H c), c <- c - s v, which leaves rows above k alone
+//    (s = 0 for columns that take no update).
+//  - Sums run down the rows in order (k_cpqr sums across lanes), and the
+//    exact norm of a column is formed only when its downdate needs it.
+template <class T, int MR, int NQ>
+__global__ __launch_bounds__(NQ, 1) void k_cpqr_reg(const QRDesc<T>* __restrict__ ds) {
+  using R = typename CplxOf<T>::real;
+  const QRDesc<T> d = ds[blockIdx.x];
+  __shared__ double shv[NQ / 64];
+  __shared__ int shi[NQ / 64];
+  __shared__ T sv[MR];          // the pivot column, then the reflector
+  __shared__ double sdiag[MR];  // |R_kk| as stored
+  __shared__ double ssc[6];     // beta, t, step kind
+  const int m = d.m, n = d.n, ld = d.lda;
+  const int j = threadIdx.x;
+  const bool own = j < n;
+  const double dd_thr = sqrt((double)(sizeof(R) == 8 ? 2.220446049250313e-16 : 1.1920928955078125e-07));
+  // every load in flight at once: rows past m read row m - 1 (m >= 1) and
+  // are zeroed, threads past n read column 0 and are zeroed
+  // (real and imaginary parts as two plain arrays: they stay in registers)
+  R cre[MR], cim[MR];
+  auto cget = [&](int i) {
+    T v;
+    v.x = cre[i];
+    v.y = cim[i];
+    return v;
+  };
+  auto cset = [&](int i, T v) {
+    cre[i] = v.x;
+    cim[i] = v.y;
+  };
+  {
+    const T* col = d.A + (size_t)ld * (own ? j : 0);
+#pragma unroll
+    for (int i = 0; i < MR; ++i) cset(i, col[i < m ? i : m - 1]);
+  }
+  double s0 = 0;
+#pragma unroll
+  for (int i = 0; i < MR; ++i) {
+    if (!own || i >= m) cset(i, czero<T>());
+    s0 += (double)cre[i] * cre[i] + (double)cim[i] * cim[i];
+  }
+  R nrm = (R)sqrt(s0), dir = nrm;
+  int pos = j;
+  bool done = false;  // this column was a pivot and is written
+  const double maxnorm = block_maxR<double, NQ>(own ? (double)nrm : 0.0, shv);
+  const double eps_r = d.nz_eps > 0 ? d.nz_eps : 2.220446049250313e-16;
+  const double th_helper = (maxnorm * eps_r) * (maxnorm * eps_r) / (m > 0 ? m : 1);
+  const int size = m < n ? m : n;
+  int nonzero = size, stop = size;
+  double maxpiv = 0;
+  for (int k = 0; k < size; ++k) {
+    double pv;
+    int big;
+    const bool cand = own && pos >= k;
+    block_argmax_first<double, NQ>(cand ? (double)nrm : -1.0, cand ? pos : -1, shv, shi, pv, big);
+    if (big < 0) big = k;
+    if (nonzero == size && pv * pv < th_helper * (m - k)) nonzero = k;
+    const bool piv = own && pos == big;
+    if (own && pos == k && big != k) pos = big;
+    if (piv) {
+      pos = k;
+#pragma unroll
+      for (int i = 0; i < MR; ++i) sv[i] = cget(i);  // the raw pivot column
+    }
+    __syncthreads();
+    // wave 0, one row per lane: tail norm, beta and the reflector's scalars
+    // (every lane alike), the reflector into LDS and the final pivot column
+    // (R above row k, beta, v below) into A
+    if (threadIdx.x < 64) {
+      const int i = threadIdx.x;
+      const T x = i < MR ? sv[i] : czero<T>();
+      const T c0 = sv[k];
+      double tl = (i > k && i < m) ? (double)x.x * x.x + (double)x.y * x.y : 0.0;
+      for (int o = 32; o > 0; o >>= 1) tl += __shfl_xor(tl, o);
+      const double c0x = c0.x, c0y = c0.y;
+      double beta = sqrt(c0x * c0x + c0y * c0y + tl), tx = 0, ty = 0;
+      int kind;
+      if (d.early && k > 0 && beta <= maxpiv * d.thr) {
+        kind = 2;  // stop: the rank is k
+      } else if ((tl == 0 && c0y == 0) || beta == 0) {
+        kind = 1;  // nothing to annihilate (or a column below the double range)
+        beta = c0x;
+      } else {
+        kind = 0;
+        if (c0x >= 0) beta = -beta;
+        tx = (beta - c0x) / beta;
+        ty = c0y / beta;
+      }
+      if (kind != 2) {
+        // v_i = x_i / (c0 - beta) below row k (the raw column when there is
+        // nothing to annihilate); the reflector in LDS: 1 at row k, 0 above
+        // row k and from row m on
+        T v = x;
+        if (kind == 0 && i > k && i < m) {
+          const double2 q = cdiv(mk((double)x.x, (double)x.y), mk(c0x - beta, c0y));
+          v.x = (R)q.x;
+          v.y = (R)q.y;
+        }
+        T bb = czero<T>();
+        bb.x = (R)beta;
+        if (i < m) d.A[i + (size_t)ld * k] = i < k ? x : i == k ? bb : v;
+        if (i < MR) {
+          T w = czero<T>();
+          if (i == k) w.x = (R)1;
+          if (i > k && i < m) w = v;
+          sv[i] = w;
+        }
+        if (i == 0) {
+          T t;
+          t.x = (R)tx;
+          t.y = (R)ty;
+          d.tau[k] = t;
+          sdiag[k] = hypot((double)bb.x, (double)bb.y);
+        }
+      }
+      if (i == 0) {
+        ssc[0] = beta;
+        ssc[1] = tx;
+        ssc[2] = ty;
+        ssc[5] = kind;
+      }
+    }
+    __syncthreads();
+    const double beta = ssc[0];
+    const int kind = (int)ssc[5];
+    if (kind == 2) {
+      for (int i = k + threadIdx.x; i < size; i += NQ) d.tau[i] = czero<T>();
+      stop = k;
+      break;
+    }
+    if (piv) {
+      d.perm[k] = j;
+      done = true;  // its column is in A
+    }
+    T t;
+    t.x = (R)ssc[1];
+    t.y = (R)ssc[2];
+    if (fabs(beta) > maxpiv) maxpiv = fabs(beta);
+    // every thread: s = t (v^H c), c <- c - s v; only trailing columns keep s
+    const bool act = own && pos > k;
+    T sd = czero<T>(), ck = czero<T>();
+    // (the scheduling fences keep a few reflector rows in registers at a
+    // time, not all of them next to the column)
+#pragma unroll
+    for (int i = 0; i < MR; ++i) {
+      ck = i == k ? cget(i) : ck;
+      sd = cfma(cconj(sv[i]), cget(i), sd);
+      if (i % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+    sd = cmul(t, sd);
+    if (!act) sd = czero<T>();
+#pragma unroll
+    for (int i = 0; i < MR; ++i) {
+      cset(i, cfms(sd, sv[i], cget(i)));
+      if (i % 8 == 7) __builtin_amdgcn_sched_barrier(0);
+    }
+    const T cknew = csub(ck, sd);
+    const double u = nrm;
+    if (act && u != 0) {
+      double tq = hypot((double)cknew.x, (double)cknew.y) / u;
+      tq = (1 + tq) * (1 - tq);
+      if (tq < 0) tq = 0;
+      const double rr = u / (double)dir;
+      if (tq * rr * rr <= dd_thr) {
+        double nn = 0;
+#pragma unroll
+        for (int i = 0; i < MR; ++i)
+          if (i > k && i < m) nn += (double)cre[i] * cre[i] + (double)cim[i] * cim[i];
+        nrm = dir = (R)sqrt(nn);
+      } else {
+        nrm = (R)(u * sqrt(tq));
+      }
+    }
+  }
+  if (own && !done) {
+#pragma unroll
+    for (int i = 0; i < MR; ++i)
+      if (i < m) d.A[i + (size_t)ld * pos] = cget(i);
+    d.perm[pos] = j;
+  }
+  if (threadIdx.x == 0) {
+    int r = 0;
+    const double pre = maxpiv * d.thr;
+    for (int i = 0; i < stop; ++i)
+      if (sdiag[i] > pre) ++r;
     *d.rank_out = r;
     if (d.maxpiv_out) *d.maxpiv_out = maxpiv;
     if (d.nz_out) *d.nz_out = nonzero;
@@ -939,9 +1143,10 @@ void launch_qr_solve(int nb, int max_chunks, const QSolveDesc<T>* d, hipStream_t
 // 1024 threads: the trailing update of a wide stack (hundreds of columns)
 // runs one wave per column, so more waves per matrix shorten every pivot step
 template <class T>
-void launch_cpqr(int nb, const QRDesc<T>* d, hipStream_t s) {
+void launch_cpqr(int nb, const QRDesc<T>* d, hipStream_t s, int shape) {
   if (!nb) return;
-  hipLaunchKernelGGL((k_cpqr<T, 1024>), dim3(nb), dim3(1024), 0, s, d);
+  if (shape == CPQR_REG) hipLaunchKernelGGL((k_cpqr_reg<T, CPQR_REG_ROWS, 256>), dim3(nb), dim3(256), 0, s, d);
+  else hipLaunchKernelGGL((k_cpqr<T, 1024>), dim3(nb), dim3(1024), 0, s, d);
   HIP_OK(hipGetLastError());
 }
 template <class T>
@@ -1001,7 +1206,7 @@ void launch_f2d(const float2* src, double2* dst, int64_t n, hipStream_t s) {
   template void launch_lu_solve<T>(int, int, const SolveDesc<T>*, hipStream_t);         \
   template void launch_lu_lds<T>(int, const LUDesc<T>*, int, hipStream_t);               \
   template void launch_lu_solve_lds<T>(int, int, const SolveDesc<T>*, int, hipStream_t);  \
-  template void launch_cpqr<T>(int, const QRDesc<T>*, hipStream_t);                     \
+  template void launch_cpqr<T>(int, const QRDesc<T>*, hipStream_t, int);                \
   template void launch_qr_solve<T>(int, int, const QSolveDesc<T>*, hipStream_t);         \
   template void launch_qform<T>(int, const QFormDesc<T>*, hipStream_t);                 \
   template void launch_copy<T>(int, const CopyDesc<T>*, hipStream_t);                   \

@@ -249,8 +249,14 @@ template <class T>
 void launch_lu_lds(int nb, const LUDesc<T>* d, int nmax, hipStream_t s);
 template <class T>
 void launch_lu_solve_lds(int nb, int max_cols, const SolveDesc<T>* d, int nmax, hipStream_t s);
+// CPQR launch shapes: k_cpqr (global memory, any size) or k_cpqr_reg (one
+// thread per column, the column in registers: at most CPQR_REG_ROWS rows, the
+// 36 points of a leaf box, and 256 columns).
+enum { CPQR_GLOBAL = 0, CPQR_REG = 1 };
+constexpr int CPQR_REG_ROWS = 36;
+inline int cpqr_shape(int m, int n) { return m <= CPQR_REG_ROWS && n <= 256 ? CPQR_REG : CPQR_GLOBAL; }
 template <class T>
-void launch_cpqr(int nb, const QRDesc<T>* d, hipStream_t s);
+void launch_cpqr(int nb, const QRDesc<T>* d, hipStream_t s, int shape);
 template <class T>
 void launch_qform(int nb, const QFormDesc<T>* d, hipStream_t s);
 template <class T>

@@ -244,7 +244,12 @@ int aifmm_ls_aca(aifmm_ls_problem* h, int tol_pow, int n1, const int64_t* rows, 
  * op + 10 runs the complex64 kernels.  op 14 (complex64 only): C = A^{-1} B
  * through the pivot-block growth gate at threshold thr (see
  * aifmm_ls_set_growth_gate), C[m n] (real part) = the block's growth-and-
- * condition estimate, iout[0] = 1 when the block went to FP64. */
+ * condition estimate, iout[0] = 1 when the block went to FP64.  op 5 (op 15
+ * in complex64): column-pivoted QR of A (m x n) at threshold thr by the
+ * global-memory kernel (k & 1 = 0) or the register kernel (k & 1 = 1, at most
+ * 36 rows and 256 columns), early stop when k & 2: C = the overwritten A (R
+ * and the reflectors) followed by tau[min(m, n)], iout = rank, nonzero
+ * pivots, then the column permutation (n entries). */
 int aifmm_ls_selftest(int op, int m, int n, int k, const double* A, const double* B, double* C, int* iout,
                       double thr);
 /* Box-program self-test (no reference counterpart; for the test suite): one

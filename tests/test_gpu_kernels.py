@@ -126,3 +126,77 @@ def test_cpqr_rank_and_basis(m, n, r):
     assert rank == r
     np.testing.assert_allclose(Q.conj().T @ Q, np.eye(r), atol=1e-12)
     assert np.linalg.norm(a - Q @ (Q.conj().T @ a)) < 1e-9 * np.linalg.norm(a)
+
+
+def q_times_r(F, tau, k):
+    """Q R from the first k reflectors and the upper triangle of F's first k
+    rows.  The kernels apply H_j = I - tau_j v v^H (v_j = 1 at row j, F below
+    it) to A P, so A P = H_0^H ... H_{k-1}^H R."""
+    m, n = F.shape
+    Rm = np.triu(F)[:k]
+    Q = np.eye(m, dtype=complex)[:, :k]
+    for j in range(k - 1, -1, -1):  # Q = H_0 ... H_{k-1} applied to I[:, :k]
+        v = np.zeros(m, complex)
+        v[j] = 1
+        v[j + 1:] = F[j + 1:, j]
+        Q = Q - np.outer(np.conj(tau[j]) * v, v.conj() @ Q)
+    return Q @ Rm
+
+
+def cpqr_cases():
+    rng = np.random.default_rng(21)
+    out = []
+    for m, n in [(1, 5), (17, 3), (36, 1), (36, 36), (36, 120), (36, 198), (36, 256), (24, 24)]:
+        out.append((f"full{m}x{n}", rnd(rng, m, n), False))
+    # low rank + noise with the early stop, as the recompression stacks are
+    a = rnd(rng, 36, 12) @ rnd(rng, 12, 150) + 1e-9 * rnd(rng, 36, 150)
+    out.append(("lowrank36x150", a, True))
+    # duplicated and zero columns (ties in the pivot choice), badly scaled rows
+    b = rnd(rng, 36, 40)
+    b[:, 7] = b[:, 3]
+    b[:, 20] = 0
+    b[:, 30] = b[:, 3]
+    b[30:] *= 1e-25
+    out.append(("ties36x40", b, True))
+    c = rnd(rng, 36, 8) @ rnd(rng, 8, 60)
+    c[:, ::7] = 0
+    out.append(("zeros36x60", c, True))
+    return out
+
+
+@pytest.mark.parametrize("c64", [False, True], ids=["c128", "c64"])
+def test_cpqr_register_kernel_equals_global(c64):
+    """k_cpqr_reg (one thread per column) against k_cpqr on the same input:
+    the same rank, nonzero-pivot count and pivot order, factors equal to
+    rounding (the two kernels sum dot products in different orders), and
+    A P = Q R from the register kernel's own outputs."""
+    tol = 2e-5 if c64 else 1e-12
+    # complex64 rounding leaves ~1e-7 ||A|| in the trailing columns, so its
+    # rank test needs a threshold above that to be a decision about A
+    thr = 1e-4 if c64 else 1e-7
+    for name, a, early in cpqr_cases():
+        Fg, tg, pg, rg, zg = A.selftest_cpqr_raw(a, thr, reg=False, early=early, c64=c64)
+        Fr, tr, pr, rr, zr = A.selftest_cpqr_raw(a, thr, reg=True, early=early, c64=c64)
+        m, n = a.shape
+        k = rr if early else min(m, n)
+        print(f"  {name}: rank {rr}, nonzero {zr}", flush=True)
+        assert rr == rg and zr == zg, name
+        assert np.array_equal(pr[:k], pg[:k]), name
+        assert sorted(pr.tolist()) == list(range(n)), name
+        sc = np.linalg.norm(a)
+        assert np.linalg.norm(Fr[:, :k] - Fg[:, :k]) <= tol * sc + 1e-300, name
+        # tau_j moves with the rounding of R_jj's direction: |d tau_j| ~ eps ||A|| / |R_jj|
+        eps = 6e-8 if c64 else 1.1e-16
+        rd = np.abs(np.diag(Fg)[:k])
+        assert np.all(np.abs(tr[:k] - tg[:k]) <= 100 * eps * sc / np.maximum(rd, 1e-300)), name
+        if early:
+            assert np.all(tr[k:] == 0), name
+        QR = q_times_r(Fr, tr, k)
+        assert np.linalg.norm(a[:, pr[:k]] - QR[:, :k]) <= 10 * tol * sc, name
+        if not early and m <= n:
+            assert np.linalg.norm(a[:, pr] - QR) <= 10 * tol * sc, name
+    # the register kernel refuses what it cannot hold
+    with pytest.raises(A.AifmmError):
+        A.selftest_cpqr_raw(rnd(np.random.default_rng(1), 37, 4), 1e-7, reg=True, c64=c64)
+    with pytest.raises(A.AifmmError):
+        A.selftest_cpqr_raw(rnd(np.random.default_rng(1), 4, 257), 1e-7, reg=True, c64=c64)

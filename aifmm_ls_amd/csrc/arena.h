@@ -2,6 +2,7 @@
 // alignment and released together.  Factorisations allocate every block
 // from their own arena, so freeing a factorisation is one call.
 #pragma once
+#include <utility>
 #include <vector>
 
 #include "common.h"
@@ -65,6 +66,14 @@ class Arena {
     return s;
   }
   size_t used() const { return total_; }
+  void swap(Arena& o) {
+    std::swap(chunk_, o.chunk_);
+    chunks_.swap(o.chunks_);
+    sizes_.swap(o.sizes_);
+    std::swap(cur_, o.cur_);
+    std::swap(used_, o.used_);
+    std::swap(total_, o.total_);
+  }
 
   // chunk size for an arena holding about per_unknown bytes per unknown
   static size_t chunk_for(int64_t n, size_t per_unknown, size_t lo, size_t hi) {

@@ -48,6 +48,8 @@ Blk<T> new_blk(Arena& a, int r, int c) {
 // (the source may be released then); d2h returns when the data is in dst and
 // synchronises stream s only.  Neither waits for other streams.
 void h2d(void* dst, const void* src, size_t bytes, hipStream_t s);
+// two host arrays in one copy: a at dst, b at dst + na rounded up to 256 bytes
+void h2d2(void* dst, const void* a, size_t na, const void* b, size_t nb, hipStream_t s);
 void d2h(void* dst, const void* src, size_t bytes, hipStream_t s);
 // Streams of the library.  The staging rings and the profiler keep events
 // recorded on them, and a HIP event refers to the stream it was last recorded
@@ -155,14 +157,21 @@ struct GemmBatch {
   static int run_many(void* const* bs, int n, Arena& scratch, hipStream_t s) {
     std::vector<GemmDesc<T>> d = take_all(bs, n, &GemmBatch::d);
     if (d.empty()) return 0;
+    size_t nt = 0;
+    for (auto& g : d) nt += (size_t)((g.m + 31) / 32) * ((g.n + 31) / 32);
     std::vector<TileTask> tiles;
+    tiles.reserve(nt);
     for (int i = 0; i < (int)d.size(); ++i) {
       int tm = (d[i].m + 31) / 32, tn = (d[i].n + 31) / 32;
       for (int a = 0; a < tm; ++a)
         for (int b = 0; b < tn; ++b) tiles.push_back(TileTask{i, a, b});
     }
-    auto* dd = upload(scratch, d, s);
-    auto* dt = upload(scratch, tiles, s);
+    // descriptors and tiles in one copy
+    const size_t bd = d.size() * sizeof(GemmDesc<T>), od = (bd + 255) & ~size_t(255);
+    char* dev = (char*)scratch.alloc(od + tiles.size() * sizeof(TileTask));
+    h2d2(dev, d.data(), bd, tiles.data(), tiles.size() * sizeof(TileTask), s);
+    auto* dd = (GemmDesc<T>*)dev;
+    auto* dt = (TileTask*)(dev + od);
     double by = 0, fl = 0;
     if (prof_on())
       for (auto& g : d) {

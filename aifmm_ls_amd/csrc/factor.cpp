@@ -45,11 +45,12 @@ Factor<T>::Factor(Problem& p) : pb(p), L(p.L), R(p.R) {
   sc.set_chunk(Arena::chunk_for(p.N, 2 << 10, 16 << 20, 256 << 20));
   sv.set_chunk(Arena::chunk_for(p.N, 2 << 10, 16 << 20, 256 << 20));
   pv.set_chunk(Arena::chunk_for(p.N, 1 << 10, 16 << 20, 256 << 20));
+  spare_.pv.set_chunk(Arena::chunk_for(p.N, 1 << 10, 16 << 20, 256 << 20));
 }
 
 template <class T>
 size_t Factor<T>::device_bytes() const {
-  return ar.reserved() + sc.reserved() + sv.reserved() + pv.reserved();
+  return ar.reserved() + sc.reserved() + sv.reserved() + pv.reserved() + spare_.pv.reserved();
 }
 
 template <class T>
@@ -1345,9 +1346,32 @@ void Factor<T>::build_plan(int cap) {
 }
 
 template <class T>
+void Factor<T>::swap_sweep() {
+  pv.swap(spare_.pv);
+  plan.swap(spare_.plan);
+  std::swap(plan_cap, spare_.plan_cap);
+  std::swap(Bt, spare_.Bt);
+  std::swap(Xt, spare_.Xt);
+  std::swap(yb, spare_.yb);
+  std::swap(xb, spare_.xb);
+  bk.swap(spare_.bk);
+  bs.swap(spare_.bs);
+  mr.swap(spare_.mr);
+  xk.swap(spare_.xk);
+  yk.swap(spare_.yk);
+  yt.swap(spare_.yt);
+  vk.swap(spare_.vk);
+}
+
+// One-column sweeps run the program built for one column; wider sweeps run
+// one built for at least their width (rebuilt when a wider one comes).  The
+// two programs and their vectors stay side by side.
+template <class T>
 Program<T>& Factor<T>::sweep_program(int nrhs) {
   if (B.empty()) throw std::runtime_error("solve before factor");
-  if (nrhs > plan_cap) build_plan(std::max(nrhs, 1));
+  nrhs = std::max(nrhs, 1);
+  if ((nrhs == 1) != (plan_cap == 1) && plan) swap_sweep();
+  if (nrhs == 1 ? plan_cap != 1 : nrhs > plan_cap) build_plan(nrhs);
   return *plan;
 }
 

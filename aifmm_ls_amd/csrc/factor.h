@@ -230,6 +230,17 @@ class Factor {
   std::vector<std::vector<Blk<double2>>> bk, bs, mr, xk, yk;
   std::vector<std::vector<Blk<double2>>> yt;  // phase-1 vectors of the back-substitution
   std::vector<std::vector<Blk<double2>>> vk;  // y_k - P2M A^{-1} x0 of the ill boxes
+  // The other sweep program with its vectors: one-column sweeps keep their
+  // own program (16-row tiles, the dense top) while wider ones use one built
+  // for their width; swap_sweep exchanges the active set with this one.
+  struct SweepSet {
+    Arena pv{size_t(256) << 20};
+    std::unique_ptr<Program<T>> plan;
+    int plan_cap = 0;
+    Blk<double2> Bt, Xt, yb, xb;
+    std::vector<std::vector<Blk<double2>>> bk, bs, mr, xk, yk, yt, vk;
+  } spare_;
+  void swap_sweep();
   Blk<T> eye;                           // identity, rank x rank of the largest box
   // the sweep at levels jd..2 and the base solve as one dense matrix (n2 x
   // n2; n2 = 0: the staged form), level-jd box offsets in its stacked vectors

@@ -239,6 +239,21 @@ void h2d(void* dst, const void* src, size_t bytes, hipStream_t s) {
   }
 }
 
+void h2d2(void* dst, const void* a, size_t na, const void* b, size_t nb, hipStream_t s) {
+  const size_t oa = (na + 255) & ~size_t(255);
+  if (oa + nb > kChunk) {
+    h2d(dst, a, na, s);
+    h2d((char*)dst + oa, b, nb, s);
+    return;
+  }
+  Ring& R = ring();
+  char* p = take(R, oa + nb);
+  std::memcpy(p, a, na);
+  std::memcpy(p + oa, b, nb);
+  HIP_OK(hipMemcpyAsync(dst, p, oa + nb, hipMemcpyHostToDevice, s));
+  mark(R, s);
+}
+
 void d2h(void* dst, const void* src, size_t bytes, hipStream_t s) {
   Ring& R = ring();
   for (size_t o = 0; o < bytes; o += kChunk) {

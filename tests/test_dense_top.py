@@ -56,3 +56,25 @@ def test_dense_top_refined_solution_against_fp64():
     print(f"refined complex64 with dense top: {int(st[1])} steps, rel.err vs FP64 {err:.2e}", flush=True)
     assert err < 1e-6
     g.close()
+
+
+def test_one_column_program_kept_after_wider_solves():
+    """A one-column solve after a wider one runs the one-column program again
+    (its 16-row tiles and the dense top), not the wider staged one: the result
+    is bitwise the first one-column solve's."""
+    g = A.LSProblem(5, 20.0, 7, qchoice=6, bumps=gaussian_bumps(7, 4))
+    g.assemble()
+    g.factor(A.PREC_C64)
+    b = g.rhs(0.3)
+    x1, _ = g.solve(b, tol=1e-7, max_refine=0)
+    xw, _ = g.solve(np.stack([b] * 80, axis=1), tol=1e-7, max_refine=0)  # above the dense top's 64 columns
+    d = float(np.linalg.norm(x1 - xw[:, 0]) / np.linalg.norm(x1))
+    x1b, _ = g.solve(b, tol=1e-7, max_refine=0)
+    x4, _ = g.solve(np.stack([b] * 4, axis=1), tol=1e-7, max_refine=0)
+    x1c, _ = g.solve(b, tol=1e-7, max_refine=0)
+    print(f"one column after 80: dense vs staged {d:.2e}", flush=True)
+    assert np.array_equal(x1, x1b) and np.array_equal(x1, x1c)
+    assert d < 1e-3
+    # the 80-column program again (its column grouping may differ with the width)
+    assert all(np.linalg.norm(x4[:, c] - xw[:, 0]) <= 1e-10 * np.linalg.norm(xw[:, 0]) for c in range(4))
+    g.close()

@@ -962,6 +962,8 @@ __global__ __launch_bounds__(NT) void k_qr_solve(const QSolveDesc<T>* __restrict
 }
 
 // Q[:, :k] = H_0^H H_1^H ... H_{s-1}^H I[:, :k]; one wave per column.
+// Column j starts at reflector j: H_h^H (h > j) leaves e_j as it is (v_h is 0
+// above row h), and skipping them is exact (their updates add zeros).
 template <class T>
 __global__ __launch_bounds__(NT) void k_qform(const QFormDesc<T>* __restrict__ ds) {
   const QFormDesc<T> d = ds[blockIdx.x];
@@ -975,7 +977,7 @@ __global__ __launch_bounds__(NT) void k_qform(const QFormDesc<T>* __restrict__ d
       if (i == j) z.x = 1;
       q[i] = z;
     }
-    for (int h = size - 1; h >= 0; --h) {
+    for (int h = (j < size ? j : size - 1); h >= 0; --h) {
       const T* v = d.A + (size_t)lda * h;
       T s = czero<T>();
       for (int i = h + 1 + lane; i < m; i += 64) s = cfma(cconj(v[i]), q[i], s);

@@ -404,39 +404,78 @@ __device__ __forceinline__ float2 to_b<float2, float2>(float2 v) { return v; }
 template <>
 __device__ __forceinline__ float2 to_b<float2, double2>(double2 v) { return mkf((float)v.x, (float)v.y); }
 
-// The substitutions of nc right-hand-side columns (B, leading dimension lb)
-// with the factors S (n x n in LDS, n <= 128) and the row order perm (in
-// LDS): one column per wave at a time, rows i and i + 64 in lane i, so no
-// workgroup barriers; arithmetic in TF, the columns stored as TB.  The same
-// operations per element in the same order as k_lu_solve.
+// The substitutions of nc right-hand-side columns (B, leading dimension lb;
+// nc <= WS_COLS * the workgroup's waves) with the factors S (n x n in LDS,
+// n <= 128) and the row order perm (in LDS): wave w takes columns w, w + 4,
+// ... together (their shuffles and the factor's LDS reads overlap; each
+// factor element is read once for all of them), rows i and i + 64 in lane i,
+// so no workgroup barriers; arithmetic in TF, the columns stored as TB.  The
+// same operations per element in the same order as k_lu_solve.
+constexpr int WS_COLS = 4;
 template <class TF, class TB>
 __device__ void wave_solve(const TF* S, int n, const int* perm, TB* B0, int lb, int nc) {
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int hi = lane + 64;  // this lane's second row
-  for (int c = w; c < nc; c += NT / 64) {
-    TB* B = B0 + (size_t)lb * c;
-    TF b0 = czero<TF>(), b1 = czero<TF>();
-    if (lane < n) b0 = to_f<TF, TB>(B[perm[lane]]);
-    if (hi < n) b1 = to_f<TF, TB>(B[perm[hi]]);
-    for (int k = 0; k < n - 1; ++k) {
-      const TF bk = shfl_c(k < 64 ? b0 : b1, k & 63);
-      if (lane > k && lane < n) b0 = cfms(S[lane + n * k], bk, b0);
-      if (hi > k && hi < n) b1 = cfms(S[hi + n * k], bk, b1);
+  constexpr int NW = NT / 64;
+  TF b0[WS_COLS], b1[WS_COLS];
+#pragma unroll
+  for (int u = 0; u < WS_COLS; ++u) {
+    const int c = w + NW * u;
+    b0[u] = b1[u] = czero<TF>();
+    if (c < nc) {
+      const TB* B = B0 + (size_t)lb * c;
+      if (lane < n) b0[u] = to_f<TF, TB>(B[perm[lane]]);
+      if (hi < n) b1[u] = to_f<TF, TB>(B[perm[hi]]);
     }
-    for (int k = n - 1; k >= 0; --k) {
-      const TF dk = S[k + n * k];
-      const bool nzp = dk.x != 0 || dk.y != 0;
+  }
+  for (int k = 0; k < n - 1; ++k) {
+    TF bk[WS_COLS];
+#pragma unroll
+    for (int u = 0; u < WS_COLS; ++u) bk[u] = shfl_c(k < 64 ? b0[u] : b1[u], k & 63);
+    if (lane > k && lane < n) {
+      const TF a = S[lane + n * k];
+#pragma unroll
+      for (int u = 0; u < WS_COLS; ++u) b0[u] = cfms(a, bk[u], b0[u]);
+    }
+    if (hi > k && hi < n) {
+      const TF a = S[hi + n * k];
+#pragma unroll
+      for (int u = 0; u < WS_COLS; ++u) b1[u] = cfms(a, bk[u], b1[u]);
+    }
+  }
+  for (int k = n - 1; k >= 0; --k) {
+    const TF dk = S[k + n * k];
+    const bool nzp = dk.x != 0 || dk.y != 0;
+#pragma unroll
+    for (int u = 0; u < WS_COLS; ++u) {
       if (k < 64) {
-        if (lane == k) b0 = nzp ? cdiv(b0, dk) : czero<TF>();
+        if (lane == k) b0[u] = nzp ? cdiv(b0[u], dk) : czero<TF>();
       } else if (lane == k - 64) {
-        b1 = nzp ? cdiv(b1, dk) : czero<TF>();
+        b1[u] = nzp ? cdiv(b1[u], dk) : czero<TF>();
       }
-      const TF bk = shfl_c(k < 64 ? b0 : b1, k & 63);
-      if (lane < k) b0 = cfms(S[lane + n * k], bk, b0);
-      if (hi < k) b1 = cfms(S[hi + n * k], bk, b1);
     }
-    if (lane < n) B[lane] = to_b<TB, TF>(b0);
-    if (hi < n) B[hi] = to_b<TB, TF>(b1);
+    TF bk[WS_COLS];
+#pragma unroll
+    for (int u = 0; u < WS_COLS; ++u) bk[u] = shfl_c(k < 64 ? b0[u] : b1[u], k & 63);
+    if (lane < k) {
+      const TF a = S[lane + n * k];
+#pragma unroll
+      for (int u = 0; u < WS_COLS; ++u) b0[u] = cfms(a, bk[u], b0[u]);
+    }
+    if (hi < k) {
+      const TF a = S[hi + n * k];
+#pragma unroll
+      for (int u = 0; u < WS_COLS; ++u) b1[u] = cfms(a, bk[u], b1[u]);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < WS_COLS; ++u) {
+    const int c = w + NW * u;
+    if (c < nc) {
+      TB* B = B0 + (size_t)lb * c;
+      if (lane < n) B[lane] = to_b<TB, TF>(b0[u]);
+      if (hi < n) B[hi] = to_b<TB, TF>(b1[u]);
+    }
   }
 }
 
@@ -455,7 +494,7 @@ __device__ __forceinline__ void perm_from_piv(const int* piv, int n, int* perm) 
 
 // LU solves of a block with n <= 128 rows, the factors in LDS (bitwise
 // k_lu_solve).  grid = (matrix, group of LUS_COLS columns)
-constexpr int LUS_COLS = 16;
+constexpr int LUS_COLS = WS_COLS * (NT / 64);
 template <class T>
 __global__ __launch_bounds__(NT) void k_lu_solve_wave(const SolveDesc<T>* __restrict__ ds, int nmax) {
   extern __shared__ __align__(16) unsigned char lus_lds[];

@@ -22,6 +22,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <type_traits>
+#include <unordered_map>
 
 namespace aifmm {
 
@@ -902,7 +903,8 @@ struct Composer {
     } else {
       WidenBatch wd;
       CvtBatch<float2> nd;
-      std::map<const float2*, Blk<double2>> wide;  // each source widened once
+      std::unordered_map<const float2*, Blk<double2>> wide;  // each source widened once
+      wide.reserve(2 * jobs.size());
       auto widen = [&](const Blk<float2>& x) {
         if (x.ld != x.r) throw std::runtime_error("sweep composition needs contiguous blocks");
         auto it = wide.find(x.p);

@@ -7,7 +7,8 @@ profiler sees that part on its own:
 
   sweep   a complex64 factorisation, then three direct sweeps
           (``solve_resident(tol, 0)``: no refinement)
-  factor  one assembly, then two complex64 factorisations
+  factor  one assembly, then two complex64 and two complex128
+          factorisations
   solve   a complex64 factorisation, the residual after 1..5 refinement
           steps, then three refined solves (tolerance 1e-7)
   stream  no problem: three copies of 1 GiB between two fresh 1 GiB device
@@ -15,9 +16,8 @@ profiler sees that part on its own:
           for calibrating FETCH_SIZE / WRITE_SIZE against a stream of known
           size far beyond the L2 and MALL capacities
 
-Run under ``rocprofv3`` with ``AIFMM_GRAPH=0`` (launches issued one by one):
-the profiler crashes in the host runtime during HIP stream capture on this
-ROCm image."""
+Run under ``rocprofv3``; with ``AIFMM_GRAPH=0`` the sweeps' launches are
+issued one by one instead of replayed as HIP graphs."""
 import argparse
 import sys
 
@@ -50,9 +50,10 @@ def main(argv=None):
     g = A.LSProblem(args.levels, args.kappa, args.tol_pow, qchoice=6, bumps=gaussian_bumps(args.seed, args.bumps))
     g.assemble()
     if args.name == "factor":
-        for i in range(2):
-            g.factor(A.PREC_C64)
-            print(f"factor {i}: {g.stats()['t_factor']:.3f} s", flush=True)
+        for prec, name in ((A.PREC_C64, "complex64"), (A.PREC_C128, "complex128")):
+            for i in range(2):
+                g.factor(prec)
+                print(f"{name} factor {i}: {g.stats()['t_factor']:.3f} s", flush=True)
     elif args.name == "solve":
         g.factor(A.PREC_C64)
         g.set_rhs(g.rhs(0.3))

@@ -162,7 +162,7 @@ void H2Op::near(int nrhs, hipStream_t s) {
   const NearDesc d = near_desc(nrhs);
   NearDesc* dd = (NearDesc*)stage_dev.alloc(sizeof(NearDesc));
   h2d(dd, &d, sizeof(d), s);
-  launch_near(1, d.nbox, dd, s);
+  launch_near(1, d.nbox, dd, s, d.nrhs);
   stage_dev.reset();
 }
 

@@ -178,6 +178,14 @@ __device__ double2 ls_entry(const EntryCtx& C, int64_t i, int64_t j) {
   return s;
 }
 
+// lane l's value of v in every lane (l uniform)
+__device__ __forceinline__ double readlane1(double v, int l) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, l), hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ double2 readlane2(double2 v, int l) { return mk(readlane1(v.x, l), readlane1(v.y, l)); }
+
 __device__ __forceinline__ double2 near_fma(double2 a, double2 b, double2 c) {
   c.x = fma(a.x, b.x, c.x);
   c.x = fma(-a.y, b.y, c.x);
@@ -198,61 +206,84 @@ __device__ __forceinline__ double2 near_fma(double2 a, double2 b, double2 c) {
 // workgroup 6.80 ms, 2 boxes 6.71 ms, 4 boxes 6.79 ms, 8 boxes 6.92 ms.
 constexpr int NEAR_BOXES = 2;
 
+// CC right-hand-side columns per workgroup (grid z = column groups): each
+// table entry loaded once is applied to every column of the group; every
+// column's sums are the same operations in the same order for any CC.
+template <int CC>
 __global__ __launch_bounds__(256) void k_near(const NearDesc* __restrict__ ds) {
   const NearDesc D = ds[blockIdx.y];
   const int k0 = blockIdx.x * NEAR_BOXES;
-  if (k0 >= D.nbox) return;
+  const int c0 = blockIdx.z * CC;
+  if (k0 >= D.nbox || c0 >= D.nrhs) return;
+  const int nc = D.nrhs - c0 < CC ? D.nrhs - c0 : CC;
   const int R = D.R, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  __shared__ double2 red[4][NEAR_BOXES][64];
-  for (int c = 0; c < D.nrhs; ++c) {
-    double2 acc[NEAR_BOXES];
+  __shared__ double2 red[4][NEAR_BOXES][CC][64];
+  double2 acc[NEAR_BOXES][CC];
 #pragma unroll
-    for (int b = 0; b < NEAR_BOXES; ++b) acc[b] = mk(0.0, 0.0);
-    if (lane < R)
-      for (int sl = w; sl < 9; sl += 4) {
-        const double* xb[NEAR_BOXES];
+  for (int b = 0; b < NEAR_BOXES; ++b)
 #pragma unroll
-        for (int b = 0; b < NEAR_BOXES; ++b) {
-          const int k = k0 + b;
-          const int nn = k < D.nbox ? D.nbr[(size_t)k * 9 + sl] : -1;
-          xb[b] = nn < 0 ? nullptr : (const double*)(D.x + (size_t)D.N * c + (size_t)nn * R);
-        }
-        const double2* Tt = D.T + (size_t)sl * R * R + lane;
-        for (int jj = 0; jj < R; ++jj) {
-          const double2 e = Tt[(size_t)R * jj];
+    for (int u = 0; u < CC; ++u) acc[b][u] = mk(0.0, 0.0);
+  if (lane < R)
+    for (int sl = w; sl < 9; sl += 4) {
+      const double2* xb[NEAR_BOXES];
 #pragma unroll
-          for (int b = 0; b < NEAR_BOXES; ++b)
-            if (xb[b]) {
-              const double2 v = ((const double2*)xb[b])[jj];
-              acc[b] = near_fma(e, v, acc[b]);
-            }
-        }
+      for (int b = 0; b < NEAR_BOXES; ++b) {
+        const int k = k0 + b;
+        const int nn = k < D.nbox ? D.nbr[(size_t)k * 9 + sl] : -1;
+        xb[b] = nn < 0 ? nullptr : D.x + (size_t)D.N * c0 + (size_t)nn * R;
       }
+      // each neighbour segment in one coalesced load (lane jj holds x[jj]),
+      // handed to every lane by readlane as jj advances
+      double2 xv[NEAR_BOXES][CC];
 #pragma unroll
-    for (int b = 0; b < NEAR_BOXES; ++b) red[w][b][lane] = acc[b];
-    __syncthreads();
-    // wave w finishes boxes w, w + 4 of the group
-    for (int b = w; b < NEAR_BOXES; b += 4) {
-      const int k = k0 + b;
-      if (k < D.nbox && lane < R) {
-        double2 t = red[0][b][lane];
-        t = cadd(t, red[1][b][lane]);
-        t = cadd(t, red[2][b][lane]);
-        t = cadd(t, red[3][b][lane]);
-        const int64_t i = (int64_t)k * R + lane;
-        const size_t o = (size_t)D.N * c + i;
+      for (int b = 0; b < NEAR_BOXES; ++b)
+#pragma unroll
+        for (int u = 0; u < CC; ++u) xv[b][u] = xb[b] && u < nc ? xb[b][(size_t)D.N * u + lane] : mk(0.0, 0.0);
+      const double2* Tt = D.T + (size_t)sl * R * R + lane;
+      for (int jj = 0; jj < R; ++jj) {
+        const double2 e = Tt[(size_t)R * jj];
+#pragma unroll
+        for (int b = 0; b < NEAR_BOXES; ++b)
+          if (xb[b]) {
+#pragma unroll
+            for (int u = 0; u < CC; ++u)
+              if (u < nc) acc[b][u] = near_fma(e, readlane2(xv[b][u], jj), acc[b][u]);
+          }
+      }
+    }
+#pragma unroll
+  for (int b = 0; b < NEAR_BOXES; ++b)
+#pragma unroll
+    for (int u = 0; u < CC; ++u) red[w][b][u][lane] = acc[b][u];
+  __syncthreads();
+  // wave w finishes boxes w, w + 4 of the group
+  for (int b = w; b < NEAR_BOXES; b += 4) {
+    const int k = k0 + b;
+    if (k < D.nbox && lane < R) {
+      const int64_t i = (int64_t)k * R + lane;
+      for (int u = 0; u < nc; ++u) {
+        double2 t = red[0][b][u][lane];
+        t = cadd(t, red[1][b][u][lane]);
+        t = cadd(t, red[2][b][u][lane]);
+        t = cadd(t, red[3][b][u][lane]);
+        const size_t o = (size_t)D.N * (c0 + u) + i;
         double2 v = cscale(t, D.q[i]);
         v = cadd(v, D.x[o]);  // the identity
         D.y[o] = cadd(D.y[o], v);
       }
     }
-    __syncthreads();
   }
 }
 
-void launch_near(int nmem, int max_box, const NearDesc* d, hipStream_t s) {
-  if (!nmem || max_box < 1) return;
-  hipLaunchKernelGGL(k_near, dim3((max_box + NEAR_BOXES - 1) / NEAR_BOXES, nmem), dim3(256), 0, s, d);
+void launch_near(int nmem, int max_box, const NearDesc* d, hipStream_t s, int max_nrhs) {
+  if (!nmem || max_box < 1 || max_nrhs < 1) return;
+  const int gx = (max_box + NEAR_BOXES - 1) / NEAR_BOXES;
+  // measured at N = 589,824 with 256 columns: 1 column per workgroup 37.0 ms,
+  // 4 columns 32.2 ms, 8 columns 42.6 ms (fewer waves per SIMD)
+  if (max_nrhs >= 4)
+    hipLaunchKernelGGL(k_near<4>, dim3(gx, nmem, (max_nrhs + 3) / 4), dim3(256), 0, s, d);
+  else
+    hipLaunchKernelGGL(k_near<1>, dim3(gx, nmem, max_nrhs), dim3(256), 0, s, d);
   HIP_OK(hipGetLastError());
 }
 

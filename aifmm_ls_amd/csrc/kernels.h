@@ -304,7 +304,9 @@ struct NearDesc {
   int L, R, nbox, nrhs;
   int64_t N;
 };
-void launch_near(int nmem, int max_box, const NearDesc* d, hipStream_t s);
+// grid (box groups, members, column groups); max_nrhs = the members' largest
+// column count
+void launch_near(int nmem, int max_box, const NearDesc* d, hipStream_t s, int max_nrhs = 1);
 
 // The same vector operations for several independent vectors in one launch
 // (one single-column problem each; member i on grid row i).  Every member

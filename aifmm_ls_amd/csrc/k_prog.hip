@@ -7,7 +7,11 @@
 // a multiple of 64; ns = threads / rp), so a wave reads a contiguous column
 // segment of the matrix per step (coalesced), and slice partial sums are
 // combined through LDS.  The phase-1 vector lives in LDS; right-hand sides
-// are processed CH columns at a time.
+// are processed CH columns at a time.  Where the launch gives room for it
+// (xs_cols), the terms' input vectors of the CH columns are staged in LDS
+// by coalesced loads before a sum walks them, instead of every thread
+// loading each element from global memory; the values and the order of the
+// arithmetic are the same either way.
 #include <cstddef>
 
 #include "cplx_dev.h"
@@ -28,7 +32,7 @@ constexpr int PROG_LDS = 63 * 1024;
 template <class T, int CH, int NTH>
 __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __restrict__ terms, int t0, int t1,
                                               int c0, int nc, const V* ylds, int ldy, V* red, V (&acc)[CH],
-                                              int& row) {
+                                              int& row, V* xs, int xs_cols) {
   const int tid = threadIdx.x;
   const int rp = rr <= 16 ? 16 : rr <= 32 ? 32 : ((rr + 63) >> 6) << 6;
   const int ns = NTH / rp;
@@ -45,7 +49,25 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
   // 1.72 ms, and would take the complex128 operator product from 1.75 to
   // 3.2 ms per refined solve.
   const int path = CH != 1 || nt <= 0 ? 0 : sizeof(T) == 8 ? 2 : ns >= nt ? 1 : 0;
-  if (path == 1) {
+  // staged input vectors: term t's block (cols x nc, leading dimension cols)
+  // at the running offset of the earlier terms' cols x CH
+  const bool staged = CH > 1 && xs_cols > 0 && nt > 0;
+  if (staged) {
+    __syncthreads();  // the previous sum's readers are done with xs
+    int off = 0;
+    for (int t = t0; t < t1; ++t) {
+      const PTerm<T>& tm = terms[t];
+      if (!tm.x) continue;
+      const int cols = tm.cols, n = cols * nc;
+      const double2* x = tm.x + (size_t)tm.ldx * c0;
+      for (int e = tid; e < n; e += NTH) {
+        const int c = e / cols, j = e - c * cols;
+        xs[off + e] = x[j + (size_t)tm.ldx * c];
+      }
+      off += cols * CH;
+    }
+    __syncthreads();
+  }  if (path == 1) {
     // no more terms than slices: the slices are shared out among the terms
     // (spt each), so all terms stream at once
     const int spt = ns / nt;
@@ -90,7 +112,7 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
         const PTerm<T>& tm = terms[tt];
         Ab = tm.A + rb + i;
         la = tm.lda;
-        xb = tm.x ? tm.x + (size_t)tm.ldx * c0 : ylds;
+        xb = tm.x ? tm.x + (size_t)tm.ldx * c0 : ylds;  // (CH = 1 here: never staged)
         alp = tm.alpha;
         ct = tm.cols;
       };
@@ -123,12 +145,17 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
       }
     }
   } else if (s < ns && i < rr) {
+    int xoff = 0;
     for (int t = t0; t < t1; ++t) {
       const PTerm<T> tm = terms[t];
       const T* A = tm.A + rb + i;
       const V* x;
       size_t ldx;
-      if (tm.x) {
+      if (tm.x && staged) {
+        x = xs + xoff;
+        ldx = tm.cols;
+        xoff += tm.cols * CH;
+      } else if (tm.x) {
         x = tm.x + (size_t)tm.ldx * c0;
         ldx = tm.ldx;
       } else {
@@ -212,7 +239,7 @@ __device__ __forceinline__ bool row_block_sum(int rb, int rr, const PTerm<T>* __
 // vector, red: NTH x CH reduction buffer, both in LDS).
 template <class T, int CH, int NTH>
 __device__ __forceinline__ void run_program(const PProg<T>* __restrict__ P, const PTerm<T>* __restrict__ terms,
-                                            int nrhs, int ldy, V* ylds, V* red) {
+                                            int nrhs, int ldy, V* ylds, V* red, V* xs, int xs_cols) {
   const int m = P->m;
   // column groups of CH right-hand sides are independent (each has its own
   // phase-1 vector and writes its own columns): workgroup (program, g) runs
@@ -234,7 +261,7 @@ __device__ __forceinline__ void run_program(const PProg<T>* __restrict__ P, cons
           sv[c] = (P->src && threadIdx.x < rp && i < rr && c < nc) ? P->src[rb + i + (size_t)P->ldsrc * (c0 + c)]
                                                                   : czero<V>();
       }
-      if (row_block_sum<T, CH, NTH>(rb, rr, terms, P->t0, P->t1, c0, nc, ylds, ldy, red, acc, row)) {
+      if (row_block_sum<T, CH, NTH>(rb, rr, terms, P->t0, P->t1, c0, nc, ylds, ldy, red, acc, row, xs, xs_cols)) {
         const V sa = P->src_alpha;
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
@@ -253,7 +280,8 @@ __device__ __forceinline__ void run_program(const PProg<T>* __restrict__ P, cons
         const int rr = min(256, rows - rb);
         V acc[CH];
         int row;
-        if (row_block_sum<T, CH, NTH>(rb, rr, terms, P->u0[o], P->u1[o], c0, nc, ylds, ldy, red, acc, row)) {
+        if (row_block_sum<T, CH, NTH>(rb, rr, terms, P->u0[o], P->u1[o], c0, nc, ylds, ldy, red, acc, row, xs,
+                                      xs_cols)) {
           V* out = P->out[o];
           const size_t ldo = P->ldo[o];
 #pragma unroll
@@ -273,20 +301,21 @@ __device__ __forceinline__ void run_program(const PProg<T>* __restrict__ P, cons
 // more terms than a slot holds reads them from the stage-wide array).
 template <class T, int CH, int NTH>
 __global__ __launch_bounds__(NTH) void k_prog(const PSlot<T>* __restrict__ slots, const PTerm<T>* __restrict__ terms,
-                                              int nrhs, int ldy, int slot_words) {
+                                              int nrhs, int ldy, int slot_words, int xs_cols) {
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ PTerm<T> tl[PSLOT_TERMS];
   static_assert(sizeof(PSlot<T>) % 16 == 0 && offsetof(PSlot<T>, t) % 16 == 0, "slot layout");
   V* ylds = (V*)smem;
   V* red = ylds + (size_t)ldy * CH;
+  V* xs = red + (size_t)NTH * CH;
   const PSlot<T>* S = slots + blockIdx.x;
   const int tw = slot_words - (int)(offsetof(PSlot<T>, t) / 16);
   const int4* src = (const int4*)S->t;
   int4* dst = (int4*)tl;
   for (int i = threadIdx.x; i < tw; i += NTH) dst[i] = src[i];
   __syncthreads();
-  if (S->p.ext) run_program<T, CH, NTH>(&S->p, terms, nrhs, ldy, ylds, red);
-  else run_program<T, CH, NTH>(&S->p, tl, nrhs, ldy, ylds, red);
+  if (S->p.ext) run_program<T, CH, NTH>(&S->p, terms, nrhs, ldy, ylds, red, xs, xs_cols);
+  else run_program<T, CH, NTH>(&S->p, tl, nrhs, ldy, ylds, red, xs, xs_cols);
 }
 
 // right-hand-side columns per pass: 1 for a single RHS, else the most that
@@ -312,26 +341,39 @@ static int fit_ch(int nrhs, int ldy, int eb, int nth, int ch_req) {
 
 template <class T, int NTH>
 static void launch_nth(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, int sw,
-                       hipStream_t s, int ch_req) {
+                       hipStream_t s, int ch_req, int xcols) {
   if (sw < 1 || sw > (int)(sizeof(PSlot<T>) / 16)) throw std::runtime_error("bad program slot size");
   const int ch = fit_ch(nrhs, ldy, sizeof(V), NTH, ch_req);
   if (!ch) throw std::runtime_error("box program vector of " + std::to_string(ldy) + " rows exceeds LDS");
-  const size_t lds = (size_t)(ldy + NTH) * ch * sizeof(V);
+  // the staged input vectors, for complex64 passes of four or more columns
+  // in workgroups of up to 512 threads, where they fit beside the phase-1
+  // vector and the reduction buffer.  Measured on MI355X (fixed default
+  // shapes): complex64, 4 columns, 256 threads: 48.1 -> 33.6 ms over a
+  // 64-column refined solve at N = 147,456 (the 256-column direct sweep at
+  // N = 589,824: 212 -> 179 ms); but complex64, 2 columns, 1024 threads
+  // 359 -> 395 ms, complex128, 2 columns, 1024 threads 56 -> 79 ms,
+  // complex128, 4 columns, 256 threads 57.7 -> 58.7 ms, and one column
+  // 0.73 -> 1.06 ms per sweep (the copy and its two barriers outweigh the
+  // loads saved)
+  const bool stage = sizeof(T) == 8 && ch >= 4 && NTH <= 512;
+  const int xs =
+      stage && xcols > 0 && (size_t)(ldy + NTH + xcols) * ch * sizeof(V) <= (size_t)PROG_LDS ? xcols : 0;
+  const size_t lds = (size_t)(ldy + NTH + xs) * ch * sizeof(V);
   // one workgroup per program and group of ch right-hand-side columns
   const int groups = (nrhs + ch - 1) / ch;
   if (groups > 65535) throw std::runtime_error("too many right-hand-side column groups");
   const dim3 grid(nprog, groups);
   if constexpr (NTH <= 512) {
     if (ch == 8) {
-      hipLaunchKernelGGL((k_prog<T, 8, NTH>), grid, dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
+      hipLaunchKernelGGL((k_prog<T, 8, NTH>), grid, dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw, xs);
       HIP_OK(hipGetLastError());
       return;
     }
   }
-  if (ch == 4) hipLaunchKernelGGL((k_prog<T, 4, NTH>), grid, dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
+  if (ch == 4) hipLaunchKernelGGL((k_prog<T, 4, NTH>), grid, dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw, xs);
   else if (ch == 2)
-    hipLaunchKernelGGL((k_prog<T, 2, NTH>), grid, dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
-  else hipLaunchKernelGGL((k_prog<T, 1, NTH>), grid, dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw);
+    hipLaunchKernelGGL((k_prog<T, 2, NTH>), grid, dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw, xs);
+  else hipLaunchKernelGGL((k_prog<T, 1, NTH>), grid, dim3(NTH), lds, s, progs, terms, nrhs, ldy, sw, xs);
   HIP_OK(hipGetLastError());
 }
 
@@ -350,20 +392,22 @@ bool prog_shape_fits(int nrhs, int ldy, int shape) {
 
 template <class T>
 void launch_prog(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, int slot_words,
-                 hipStream_t s, int shape) {
+                 hipStream_t s, int shape, int xcols) {
   if (!nprog || !nrhs) return;
   if (ldy < 1) ldy = 1;
   int nth = shape % PROG_SHAPE_CH;
   const int ch = shape / PROG_SHAPE_CH;
   if (!nth) nth = prog_default_nth(nprog);
-  if (nth == 256) launch_nth<T, 256>(nprog, progs, terms, nrhs, ldy, slot_words, s, ch);
-  else if (nth == 512) launch_nth<T, 512>(nprog, progs, terms, nrhs, ldy, slot_words, s, ch);
-  else if (nth == 1024) launch_nth<T, 1024>(nprog, progs, terms, nrhs, ldy, slot_words, s, ch);
+  if (nth == 256) launch_nth<T, 256>(nprog, progs, terms, nrhs, ldy, slot_words, s, ch, xcols);
+  else if (nth == 512) launch_nth<T, 512>(nprog, progs, terms, nrhs, ldy, slot_words, s, ch, xcols);
+  else if (nth == 1024) launch_nth<T, 1024>(nprog, progs, terms, nrhs, ldy, slot_words, s, ch, xcols);
   else throw std::runtime_error("box program workgroup size must be 256, 512 or 1024");
 }
 
-template void launch_prog<double2>(int, const PSlot<double2>*, const PTerm<double2>*, int, int, int, hipStream_t, int);
-template void launch_prog<float2>(int, const PSlot<float2>*, const PTerm<float2>*, int, int, int, hipStream_t, int);
+template void launch_prog<double2>(int, const PSlot<double2>*, const PTerm<double2>*, int, int, int, hipStream_t, int,
+                                   int);
+template void launch_prog<float2>(int, const PSlot<float2>*, const PTerm<float2>*, int, int, int, hipStream_t, int,
+                                  int);
 template bool prog_shape_fits<double2>(int, int, int);
 template bool prog_shape_fits<float2>(int, int, int);
 

@@ -133,6 +133,17 @@ void Program<T>::finalize(Arena& a, hipStream_t s) {
       throw std::runtime_error("box program of " + std::to_string(g.maxm) + " rows exceeds the LDS limit " +
                                std::to_string(lim));
     if (g.progs.empty()) continue;
+    g.xcols = 0;
+    auto xcols = [&](int t0, int t1) {
+      int n = 0;
+      for (int t = t0; t < t1; ++t)
+        if (terms_[t].x) n += terms_[t].cols;
+      return n;
+    };
+    for (const auto& q : g.progs) {
+      g.xcols = std::max(g.xcols, xcols(q.t0, q.t1));
+      for (int o = 0; o < q.nout; ++o) g.xcols = std::max(g.xcols, xcols(q.u0[o], q.u1[o]));
+    }
     std::vector<PSlot<T>> sl(g.progs.size());
     int inl = 0;  // most terms any program of the stage keeps in its slot
     for (size_t i = 0; i < g.progs.size(); ++i) {
@@ -204,7 +215,8 @@ void Program<T>::launch_all(int nrhs, hipStream_t s, bool prof) const {
     if (g.progs.empty()) continue;
     ProfScope ps(s, std::is_same<T, float2>::value ? "k_prog<c64>" : "k_prog<c128>",
                  prof ? stage_bytes(g.progs, terms_, nrhs) : 0, prof ? stage_flops(g.progs, terms_, nrhs) : 0);
-    launch_prog<T>((int)g.progs.size(), g.dslots, dterms_, nrhs, g.maxm, g.slot_words, s, shape ? (*shape)[k] : 0);
+    launch_prog<T>((int)g.progs.size(), g.dslots, dterms_, nrhs, g.maxm, g.slot_words, s, shape ? (*shape)[k] : 0,
+                   g.xcols);
   }
 }
 
@@ -262,7 +274,7 @@ void Program<T>::tune(int nrhs, hipStream_t s, const std::function<void()>& relo
     for (int k = 0; k < ng; ++k) {
       const Stage& g = st_[k];
       if (!g.progs.empty())
-        launch_prog<T>((int)g.progs.size(), g.dslots, dterms_, nrhs, g.maxm, g.slot_words, s, shape[k]);
+        launch_prog<T>((int)g.progs.size(), g.dslots, dterms_, nrhs, g.maxm, g.slot_words, s, shape[k], g.xcols);
       if (per || k == ng - 1) HIP_OK(hipEventRecord(ev[k + 1], s));
     }
     HIP_OK(hipEventSynchronize(ev[ng]));

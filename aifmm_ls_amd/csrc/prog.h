@@ -70,9 +70,11 @@ struct PSlot {
 // plus PROG_SHAPE_CH x right-hand-side columns per pass (1, 2, 4, or 8 at up
 // to 512 threads; 0 = the most of 1, 2, 4 that fit the LDS).
 constexpr int PROG_SHAPE_CH = 4096;
+// xcols: the most input-vector columns (terms with x, summed) any one sum of
+// the stage reads; where they fit the LDS the kernel stages them there.
 template <class T>
 void launch_prog(int nprog, const PSlot<T>* progs, const PTerm<T>* terms, int nrhs, int ldy, int slot_words,
-                 hipStream_t s, int shape = 0);
+                 hipStream_t s, int shape = 0, int xcols = 0);
 int prog_default_nth(int nprog);
 // Launch tuning for programs not yet tuned at an nrhs: 0 off, 1 timed (the
 // default unless AIFMM_TUNE=0 is set in the environment), or a launch shape
@@ -205,6 +207,7 @@ class Program {
     int maxm = 0;
     PSlot<T>* dslots = nullptr;
     int slot_words = 0;
+    int xcols = 0;  // input-vector columns of the stage's largest sum
   };
   static uint64_t next_id() {
     static std::atomic<uint64_t> n{0};

@@ -76,9 +76,10 @@ void Factor<T>::init_level(int j) {
     colors[j][F.color].push_back(k);
     F.L2P = conv(S.L2P);
     F.P2M = conv(S.P2M);
-    for (auto& e : S.M2L) F.M2L[e.first] = conv(e.second);
+    // (sources in key order: each insertion goes at the end)
+    for (auto& e : S.M2L) F.M2L.emplace_hint(F.M2L.end(), e.first, conv(e.second));
     if (j == L)
-      for (auto& e : pb.P2P[k]) F.P2P[e.first] = conv(e.second);
+      for (auto& e : pb.P2P[k]) F.P2P.emplace_hint(F.P2P.end(), e.first, conv(e.second));
   }
   cv.run(sc, pb.stream);
   sc.reset();  // scratch reuse is ordered on the stream

@@ -1304,7 +1304,7 @@ void Factor<T>::dense_top_fill() {
   CopyBatch<double2> cp;
   cp.add(b2, b2, 3);
   cp.run(sc, pb.stream);
-  P.run(n, pb.stream);
+  P.run_once(n, pb.stream);
   if constexpr (std::is_same<T, double2>::value) {
     cp.add(x2, d2);
     cp.run(sc, pb.stream);

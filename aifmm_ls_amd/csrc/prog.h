@@ -172,6 +172,9 @@ class Program {
   // With `reload` (which re-issues the copy of the program's input on s), the
   // first run at each nrhs first tunes the launch shapes: see tune().
   void run(int nrhs, hipStream_t s, const std::function<void()>& reload = nullptr);
+  // One run of a program that runs once: the launches issued one by one (no
+  // graph to capture and instantiate), no tuning.
+  void run_once(int nrhs, hipStream_t s) const { launch_all(nrhs, s, false); }
   // Outcome of tuning at one nrhs: mean milliseconds of a whole run with the
   // default and with the picked workgroup sizes, stages given another size,
   // and whether the picked sizes were kept.

@@ -71,7 +71,10 @@ __device__ __forceinline__ double2 graf_sum(const double2* __restrict__ Mcol, do
   double2 pw = mk(1.0, 0.0);
   double2 acc = mk(0.0, 0.0);
   double ym1 = 0.0, yc = y0v;
-#pragma unroll
+  // Partly unrolled: fully unrolled, the compiler hoists the 121 loads of
+  // Mcol ahead of the arithmetic and spills them (2.5-4 KB per lane in the
+  // kernels that inline this); at 4 per iteration only J stays in memory.
+#pragma unroll 4
   for (int n = 0; n <= SERIES; ++n) {
     if (n == 1) {
       ym1 = y0v;

@@ -17,84 +17,142 @@ __device__ __forceinline__ double hypot2(double2 a) { return hypot(a.x, a.y); }
 
 __constant__ int c_nb_off[8][2] = {{-1, -1}, {0, -1}, {1, -1}, {1, 0}, {1, 1}, {0, 1}, {-1, 1}, {-1, 0}};
 
-// J_n(x), n = 0..60, by Miller's backward recurrence normalised with
-// J_0 + 2 sum_k J_2k = 1.  All scalings are by powers of two so they do not
-// change rounding; contraction into FMA is disabled so the recurrence
-// rounds exactly as a plain IEEE evaluation.
-__device__ __forceinline__ void bessel_J61(double x, double (&J)[SERIES + 1]) {
-#pragma clang fp contract(off)
+// The Graf series of a far entry, sum_{n=-60}^{60} M(n+60, col) H_n(x)
+// e^{i n phi} with H_{-n} = (-1)^n H_n, in registers (no per-lane array: the
+// entry, assembly, ACA and dense-product kernels all inline it).
+// sum_n H_n c_n with c_n = m(60+n) w^n +- m(60-n) w^-n, w = e^{i phi},
+// splits into sum_n J_n c_n and i sum_n Y_n c_n:
+//  * one backward pass of Miller's recurrence yields the unnormalised J_n,
+//    from which it accumulates sum_n J_n c_n (n <= 60), the normalisation
+//    J_0 + 2 sum_k J_2k, and the two Neumann sums that give Y_0 and Y_1
+//    (Abramowitz & Stegun 9.1.88, and its term-by-term derivative with
+//    Y_1 = -Y_0'):
+//      Y_0 = (2/pi) (ln(x/2) + gamma) J_0 - (4/pi) sum_k (-1)^k J_2k / k
+//      Y_1 = -(2/(pi x)) J_0 + (2/pi) (ln(x/2) + gamma) J_1
+//            + (2/pi) sum_k (-1)^k (J_2k-1 - J_2k+1) / k
+//  * one forward pass runs Y's recurrence and accumulates sum_n Y_n c_n.
+// The recurrences multiply by 1/x instead of dividing by x, and w is the
+// direction (dx, dy) / rho itself instead of sincos(atan2(dy, dx)); each
+// changes a term by rounding only (the series agree with scipy's Y_0, Y_1 to
+// 3e-14 relative for x up to 170).
+struct InvTable {
+  double v[1024];
+  constexpr InvTable() : v() {
+    for (int i = 1; i < 1024; ++i) v[i] = 1.0 / i;
+  }
+};
+__constant__ InvTable c_inv = InvTable();
+
+__device__ __forceinline__ double inv_int(int m) { return m < 1024 ? c_inv.v[m] : 1.0 / m; }
+
+__device__ __forceinline__ double2 graf_c(const double2* __restrict__ Mcol, int n, double2 pw) {
+  double2 c = cmul(Mcol[SERIES + n], pw);
+  if (n > 0) {
+    const double2 t = cmul(Mcol[SERIES - n], cconj(pw));
+    c = (n & 1) ? csub(c, t) : cadd(c, t);
+  }
+  return c;
+}
+
+__device__ __forceinline__ double2 graf_sum_dir(const double2* __restrict__ Mcol, double x, double2 e1) {
+  constexpr double kPi = 3.14159265358979323846, kGamma = 0.57721566490153286061;
+  const double rx = 1.0 / x;
+  // w^60 by squaring: w^4, w^8, w^16, w^32 and their product
+  double2 w60;
+  {
+    const double2 w2 = cmul(e1, e1), w4 = cmul(w2, w2), w8 = cmul(w4, w4), w16 = cmul(w8, w8),
+                  w32 = cmul(w16, w16);
+    w60 = cmul(cmul(w32, w16), cmul(w8, w4));
+  }
+  const double2 e1c = cconj(e1);
   int big = x > (double)SERIES ? (int)x : SERIES;
   int Mst = big + 30 + (int)sqrt(40.0 * big);
   if (Mst & 1) ++Mst;
-  double jp1 = 0.0, jc = 1e-300, sum = 0.0;
+  // jc holds the unnormalised J_n of the step's n = k - 1, jp1 J_{n+1}
+  double jp1 = 0.0, jc = 1e-300;
+  double sum = 0.0, s0 = 0.0, s1 = 0.0;  // J_0 + 2 sum J_2k;  Neumann sums of Y_0, Y_1
   for (int k = Mst; k > SERIES + 1; --k) {
-    double jm1 = (2.0 * k / x) * jc - jp1;
+    const double jm1 = (2.0 * k * rx) * jc - jp1;
     jp1 = jc;
     jc = jm1;
     if (fabs(jc) > 0x1p800) {
       jc *= 0x1p-800;
       jp1 *= 0x1p-800;
       sum *= 0x1p-800;
+      s0 *= 0x1p-800;
+      s1 *= 0x1p-800;
     }
-    if (((k - 1) & 1) == 0) sum += 2.0 * jc;
+    const int n = k - 1;
+    if ((n & 1) == 0) {
+      const int m = n >> 1;
+      sum += 2.0 * jc;
+      s0 += ((m & 1) ? -inv_int(m) : inv_int(m)) * jc;
+    } else {
+      const int a = (n + 1) >> 1;
+      const double w = inv_int(a) + inv_int(a - 1);
+      s1 += ((a & 1) ? -w : w) * jc;
+    }
   }
-  // bring j_61 to ~2^-500 with an exact power-of-two scale
   {
-    int e = ilogb(jc);
-    double f = ldexp(1.0, -500 - e);
+    const int e = ilogb(jc);
+    const double f = ldexp(1.0, -500 - e);
     jc *= f;
     jp1 *= f;
     sum *= f;
+    s0 *= f;
+    s1 *= f;
   }
-#pragma unroll
+  double2 pw = w60;
+  double accj_re = 0.0, accj_im = 0.0;  // sum J~_n c_n (unnormalised)
+#pragma unroll 2
   for (int k = SERIES + 1; k >= 1; --k) {
-    double jm1 = (2.0 * k / x) * jc - jp1;
+    const double jm1 = (2.0 * k * rx) * jc - jp1;
     jp1 = jc;
     jc = jm1;
-    J[k - 1] = jc;
-    if (k - 1 > 0 && ((k - 1) & 1) == 0) sum += 2.0 * jc;
+    const int n = k - 1;
+    if ((n & 1) == 0) {
+      if (n > 0) {
+        const int m = n >> 1;
+        sum += 2.0 * jc;
+        s0 += ((m & 1) ? -inv_int(m) : inv_int(m)) * jc;
+      }
+    } else {
+      const int a = (n + 1) >> 1;
+      const double w = a > 1 ? inv_int(a) + inv_int(a - 1) : 1.0;
+      s1 += ((a & 1) ? -w : w) * jc;
+    }
+    const double2 c = graf_c(Mcol, n, pw);
+    accj_re += jc * c.x;
+    accj_im += jc * c.y;
+    if (n > 0) pw = cmul(pw, e1c);
   }
   sum += jc;
-#pragma unroll
-  for (int n = 0; n <= SERIES; ++n) J[n] = J[n] / sum;
-}
-
-// sum_{n=-60}^{60} M(n+60, col) H_n(x) e^{i n phi} with H_{-n} = (-1)^n H_n.
-__device__ __forceinline__ double2 graf_sum(const double2* __restrict__ Mcol, double x, double phi) {
-#pragma clang fp contract(off)
-  double J[SERIES + 1];
-  bessel_J61(x, J);
-  double y0v = y0(x), y1v = y1(x);
-  double sn, cs;
-  sincos(phi, &sn, &cs);
-  double2 e1 = mk(cs, sn);
-  double2 pw = mk(1.0, 0.0);
-  double2 acc = mk(0.0, 0.0);
+  const double rs = 1.0 / sum;
+  const double j0 = jc * rs, j1 = jp1 * rs;
+  const double lg = log(0.5 * x) + kGamma;
+  const double y0v = (2.0 / kPi) * lg * j0 - (4.0 / kPi) * (s0 * rs);
+  const double y1v = -(2.0 / kPi) * rx * j0 + (2.0 / kPi) * lg * j1 + (2.0 / kPi) * (s1 * rs);
+  // forward: Y_n and w^n
+  pw = mk(1.0, 0.0);
+  double accy_re = 0.0, accy_im = 0.0;  // sum Y_n c_n
   double ym1 = 0.0, yc = y0v;
-  // Partly unrolled: fully unrolled, the compiler hoists the 121 loads of
-  // Mcol ahead of the arithmetic and spills them (2.5-4 KB per lane in the
-  // kernels that inline this); at 4 per iteration only J stays in memory.
-#pragma unroll 4
+#pragma unroll 2
   for (int n = 0; n <= SERIES; ++n) {
     if (n == 1) {
       ym1 = y0v;
       yc = y1v;
     } else if (n > 1) {
-      double yn = (2.0 * (n - 1) / x) * yc - ym1;
+      const double yn = (2.0 * (n - 1) * rx) * yc - ym1;
       ym1 = yc;
       yc = yn;
     }
-    double2 H = mk(J[n], yc);
-    double2 c = cmul(Mcol[SERIES + n], pw);
-    if (n > 0) {
-      double2 t = cmul(Mcol[SERIES - n], cconj(pw));
-      if (n & 1) c = csub(c, t);
-      else c = cadd(c, t);
-    }
-    acc = cadd(acc, cmul(H, c));
+    const double2 c = graf_c(Mcol, n, pw);
+    accy_re += yc * c.x;
+    accy_im += yc * c.y;
     pw = cmul(pw, e1);
   }
-  return acc;
+  // sum H_n c_n = sum J_n c_n + i sum Y_n c_n
+  return mk(accj_re * rs - accy_im, accj_im * rs + accy_re);
 }
 
 // The near table of target leaf li for source leaf lj (adaptive), or null
@@ -127,10 +185,9 @@ __device__ double2 ls_entry_adaptive(const EntryCtx& C, int64_t i, int64_t j) {
     return e;
   }
   const double ddy = C.py[i] - C.lcy[lj], ddx = C.px[i] - C.lcx[lj];
-  const double phi = atan2(ddy, ddx);
-  const double R2 = ddy * ddy + ddx * ddx;
-  const double arg = C.kappa * sqrt(R2);
-  double2 s = graf_sum(C.Mlev + (size_t)C.leaf_lv[lj] * NTERM * R + (size_t)NTERM * jj, arg, phi);
+  const double rho = sqrt(ddy * ddy + ddx * ddx), ir = 1.0 / rho;
+  double2 s = graf_sum_dir(C.Mlev + (size_t)C.leaf_lv[lj] * NTERM * R + (size_t)NTERM * jj, C.kappa * rho,
+                           mk(ddx * ir, ddy * ir));
   s = cmul(s, mk(0.0, 0.25));
   if (C.find_phi) {
     s = cscale(s, C.kappa * C.kappa * qi);
@@ -168,10 +225,8 @@ __device__ double2 ls_entry(const EntryCtx& C, int64_t i, int64_t j) {
   }
   double cx = -C.Ldom + (2 * xj + 1) * C.rleaf, cy = -C.Ldom + (2 * yj + 1) * C.rleaf;
   double ddy = C.py[i] - cy, ddx = C.px[i] - cx;
-  double phi = atan2(ddy, ddx);
-  double R2 = ddy * ddy + ddx * ddx;
-  double arg = C.kappa * sqrt(R2);
-  double2 s = graf_sum(C.M + (size_t)NTERM * jj, arg, phi);
+  const double rho = sqrt(ddy * ddy + ddx * ddx), ir = 1.0 / rho;
+  double2 s = graf_sum_dir(C.M + (size_t)NTERM * jj, C.kappa * rho, mk(ddx * ir, ddy * ir));
   s = cmul(s, mk(0.0, 0.25));
   if (C.find_phi) {
     s = cscale(s, C.kappa * C.kappa * qi);
@@ -572,142 +627,6 @@ __global__ __launch_bounds__(128) void k_dmv_moments(EntryCtx C, const double2* 
   double2 acc = mk(0.0, 0.0);
   for (int jj = 0; jj < C.R; ++jj) acc = cadd(acc, cmul(M[n + (size_t)NTERM * jj], xl[jj]));
   m[(size_t)l * NTERM + n] = acc;
-}
-
-// The Graf sum of graf_sum for the dense product, on cheaper arithmetic and
-// few registers.  sum_n H_n c_n with c_n = m(60+n) w^n +- m(60-n) w^-n,
-// w = e^{i phi}, splits into sum_n J_n c_n and i sum_n Y_n c_n:
-//  * one backward pass of Miller's recurrence yields the unnormalised J_n,
-//    from which it accumulates sum_n J_n c_n (n <= 60), the normalisation
-//    J_0 + 2 sum_k J_2k, and the two Neumann sums that give Y_0 and Y_1
-//    (Abramowitz & Stegun 9.1.88, and its term-by-term derivative with
-//    Y_1 = -Y_0'):
-//      Y_0 = (2/pi) (ln(x/2) + gamma) J_0 - (4/pi) sum_k (-1)^k J_2k / k
-//      Y_1 = -(2/(pi x)) J_0 + (2/pi) (ln(x/2) + gamma) J_1
-//            + (2/pi) sum_k (-1)^k (J_2k-1 - J_2k+1) / k
-//  * one forward pass runs Y's recurrence and accumulates sum_n Y_n c_n.
-// The recurrences multiply by 1/x instead of dividing by x, and w is the
-// direction (dx, dy) / rho itself instead of sincos(atan2(dy, dx)); each
-// changes a term by rounding only (the series agree with scipy's Y_0, Y_1 to
-// 3e-14 relative for x up to 170).
-struct InvTable {
-  double v[1024];
-  constexpr InvTable() : v() {
-    for (int i = 1; i < 1024; ++i) v[i] = 1.0 / i;
-  }
-};
-__constant__ InvTable c_inv = InvTable();
-
-__device__ __forceinline__ double inv_int(int m) { return m < 1024 ? c_inv.v[m] : 1.0 / m; }
-
-__device__ __forceinline__ double2 graf_c(const double2* __restrict__ Mcol, int n, double2 pw) {
-  double2 c = cmul(Mcol[SERIES + n], pw);
-  if (n > 0) {
-    const double2 t = cmul(Mcol[SERIES - n], cconj(pw));
-    c = (n & 1) ? csub(c, t) : cadd(c, t);
-  }
-  return c;
-}
-
-__device__ __forceinline__ double2 graf_sum_dir(const double2* __restrict__ Mcol, double x, double2 e1) {
-  constexpr double kPi = 3.14159265358979323846, kGamma = 0.57721566490153286061;
-  const double rx = 1.0 / x;
-  // w^60 by squaring: w^4, w^8, w^16, w^32 and their product
-  double2 w60;
-  {
-    const double2 w2 = cmul(e1, e1), w4 = cmul(w2, w2), w8 = cmul(w4, w4), w16 = cmul(w8, w8),
-                  w32 = cmul(w16, w16);
-    w60 = cmul(cmul(w32, w16), cmul(w8, w4));
-  }
-  const double2 e1c = cconj(e1);
-  int big = x > (double)SERIES ? (int)x : SERIES;
-  int Mst = big + 30 + (int)sqrt(40.0 * big);
-  if (Mst & 1) ++Mst;
-  // jc holds the unnormalised J_n of the step's n = k - 1, jp1 J_{n+1}
-  double jp1 = 0.0, jc = 1e-300;
-  double sum = 0.0, s0 = 0.0, s1 = 0.0;  // J_0 + 2 sum J_2k;  Neumann sums of Y_0, Y_1
-  for (int k = Mst; k > SERIES + 1; --k) {
-    const double jm1 = (2.0 * k * rx) * jc - jp1;
-    jp1 = jc;
-    jc = jm1;
-    if (fabs(jc) > 0x1p800) {
-      jc *= 0x1p-800;
-      jp1 *= 0x1p-800;
-      sum *= 0x1p-800;
-      s0 *= 0x1p-800;
-      s1 *= 0x1p-800;
-    }
-    const int n = k - 1;
-    if ((n & 1) == 0) {
-      const int m = n >> 1;
-      sum += 2.0 * jc;
-      s0 += ((m & 1) ? -inv_int(m) : inv_int(m)) * jc;
-    } else {
-      const int a = (n + 1) >> 1;
-      const double w = inv_int(a) + inv_int(a - 1);
-      s1 += ((a & 1) ? -w : w) * jc;
-    }
-  }
-  {
-    const int e = ilogb(jc);
-    const double f = ldexp(1.0, -500 - e);
-    jc *= f;
-    jp1 *= f;
-    sum *= f;
-    s0 *= f;
-    s1 *= f;
-  }
-  double2 pw = w60;
-  double accj_re = 0.0, accj_im = 0.0;  // sum J~_n c_n (unnormalised)
-#pragma unroll 2
-  for (int k = SERIES + 1; k >= 1; --k) {
-    const double jm1 = (2.0 * k * rx) * jc - jp1;
-    jp1 = jc;
-    jc = jm1;
-    const int n = k - 1;
-    if ((n & 1) == 0) {
-      if (n > 0) {
-        const int m = n >> 1;
-        sum += 2.0 * jc;
-        s0 += ((m & 1) ? -inv_int(m) : inv_int(m)) * jc;
-      }
-    } else {
-      const int a = (n + 1) >> 1;
-      const double w = a > 1 ? inv_int(a) + inv_int(a - 1) : 1.0;
-      s1 += ((a & 1) ? -w : w) * jc;
-    }
-    const double2 c = graf_c(Mcol, n, pw);
-    accj_re += jc * c.x;
-    accj_im += jc * c.y;
-    if (n > 0) pw = cmul(pw, e1c);
-  }
-  sum += jc;
-  const double rs = 1.0 / sum;
-  const double j0 = jc * rs, j1 = jp1 * rs;
-  const double lg = log(0.5 * x) + kGamma;
-  const double y0v = (2.0 / kPi) * lg * j0 - (4.0 / kPi) * (s0 * rs);
-  const double y1v = -(2.0 / kPi) * rx * j0 + (2.0 / kPi) * lg * j1 + (2.0 / kPi) * (s1 * rs);
-  // forward: Y_n and w^n
-  pw = mk(1.0, 0.0);
-  double accy_re = 0.0, accy_im = 0.0;  // sum Y_n c_n
-  double ym1 = 0.0, yc = y0v;
-#pragma unroll 2
-  for (int n = 0; n <= SERIES; ++n) {
-    if (n == 1) {
-      ym1 = y0v;
-      yc = y1v;
-    } else if (n > 1) {
-      const double yn = (2.0 * (n - 1) * rx) * yc - ym1;
-      ym1 = yc;
-      yc = yn;
-    }
-    const double2 c = graf_c(Mcol, n, pw);
-    accy_re += yc * c.x;
-    accy_im += yc * c.y;
-    pw = cmul(pw, e1);
-  }
-  // sum H_n c_n = sum J_n c_n + i sum Y_n c_n
-  return mk(accj_re * rs - accy_im, accj_im * rs + accy_re);
 }
 
 // grid (row tiles of DMV_T target points, leaf chunks): part[c N + i] = the

@@ -19,6 +19,7 @@ profiler sees that part on its own:
 Run under ``rocprofv3``; with ``AIFMM_GRAPH=0`` the sweeps' launches are
 issued one by one instead of replayed as HIP graphs."""
 import argparse
+import time
 import sys
 
 import numpy as np
@@ -48,7 +49,9 @@ def main(argv=None):
         print(f"stream: 1 GiB copy {ms:.3f} ms, {2 * (1 << 30) / ms / 1e6:.1f} GB/s read+write", flush=True)
         return 0
     g = A.LSProblem(args.levels, args.kappa, args.tol_pow, qchoice=6, bumps=gaussian_bumps(args.seed, args.bumps))
+    t0 = time.perf_counter()
     g.assemble()
+    print(f"assembly: {time.perf_counter() - t0:.3f} s", flush=True)
     if args.name == "factor":
         for prec, name in ((A.PREC_C64, "complex64"), (A.PREC_C128, "complex128")):
             for i in range(2):
